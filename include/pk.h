@@ -1,0 +1,164 @@
+/*
+ * pk.h -- C-ABI numerics layer of pampa_amd (MI355X / gfx950).
+ *
+ * This is the boundary that replaces the reference's PETSc/SLEPc usage on the
+ * neutronics hot path: the petsc:: wrapper (petsc.hxx:20-89) and the raw
+ * Mat/Vec/EPS calls made by the solvers (DiffusionSolver.cxx:96-98,293-316,
+ * 329-334; SNSolver.cxx:352-354,512-627,640-645).  Objects are opaque handles
+ * over device memory; every call returns 0 on success and a positive code on
+ * failure, with a one-line message available from pk_last_error() (the
+ * reference's int-return convention, utils.hxx:38-47).  Nothing in this header
+ * uses HIP or C++ types.
+ *
+ * Not re-entrant across threads for the same handles; one HIP stream per
+ * process-wide context (the reference is single-threaded per rank, pampa.cxx:8).
+ */
+#ifndef PAMPA_AMD_PK_H
+#define PAMPA_AMD_PK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_INSERT_VALUES 0 /* MatSetValues INSERT_VALUES */
+#define PK_ADD_VALUES 1    /* MatSetValues ADD_VALUES */
+
+#define PK_PREC_FP64 0  /* every solve in FP64 (the plain drop-in) */
+#define PK_PREC_MIXED 1 /* FP32 inner solves / sweeps, FP64 outer quantities */
+
+/* Boundary-condition kinds, numbered as the reference's BC::Type (utils.hxx:103-105). */
+#define PK_BC_NONE 0
+#define PK_BC_VACUUM 1
+#define PK_BC_REFLECTIVE 2
+#define PK_BC_ROBIN 3
+
+typedef struct pk_vec_s* pk_vec;
+typedef struct pk_mat_s* pk_mat;
+typedef struct pk_mesh_s* pk_mesh;
+typedef struct pk_xs_s* pk_xs;
+typedef struct pk_eig_s* pk_eig;
+typedef struct pk_lin_s* pk_lin;
+typedef struct pk_sn_s* pk_sn;
+
+/* ---- runtime ------------------------------------------------------------
+ * pk_initialize replaces petsc::initialize (petsc.cxx:18-36) for the numerics;
+ * pk_finalize replaces petsc::finalize. */
+int pk_initialize(int device);
+int pk_finalize(void);
+int pk_synchronize(void);
+const char* pk_last_error(void);
+
+/* ---- vectors (petsc::create(Vec&), destroy, set, copy; petsc.cxx:113-140,
+ *      236-345) ------------------------------------------------------------ */
+int pk_vec_create(long n, pk_vec* v);
+int pk_vec_destroy(pk_vec* v);
+long pk_vec_size(pk_vec v);
+int pk_vec_set(pk_vec v, double x);                     /* petsc::set / VecSet */
+int pk_vec_set_values(pk_vec v, const double* host);    /* petsc::copy(const double*, Vec*) */
+int pk_vec_get_values(pk_vec v, double* host);          /* petsc::copy(const Vec*, double*) */
+int pk_vec_copy(pk_vec src, pk_vec dst);                /* VecCopy */
+int pk_vec_scale(pk_vec v, double a);                   /* VecScale */
+int pk_vec_axpy(pk_vec y, double a, pk_vec x);          /* VecAXPY */
+int pk_vec_dot(pk_vec x, pk_vec y, double* r);          /* VecDot (fixed-order FP64 reduction) */
+int pk_vec_norm2(pk_vec x, double* r);                  /* VecNorm(NORM_2) */
+int pk_vec_device_ptr(pk_vec v, void** dev);            /* device address (interop) */
+
+/* ---- matrices (petsc::create(Mat&) petsc.cxx:91-111, MatSetValues,
+ *      MatAssemblyBegin/End).  CSR with PETSc-AIJ semantics: sorted columns,
+ *      explicit zeros kept, INSERT (last value wins) or ADD. ---------------- */
+int pk_mat_create(long n, int prealloc, pk_mat* M);
+int pk_mat_set_values(pk_mat M, int nrows, const int* rows, int ncols, const int* cols,
+                      const double* vals, int mode);
+int pk_mat_assemble(pk_mat M);
+int pk_mat_destroy(pk_mat* M);
+int pk_mat_get_size(pk_mat M, long* n, long* nnz);
+int pk_mat_get_csr(pk_mat M, int* row_ptr, int* col_idx, double* val); /* host copies */
+int pk_mat_mult(pk_mat M, pk_vec x, pk_vec y);                          /* MatMult */
+int pk_mat_set_block_size(pk_mat M, int bs); /* cell block (energy groups) for preconditioning */
+/* Time one MatMult kernel: average device time per launch over `reps` launches
+ * (HIP events on the library stream). */
+int pk_mat_time_mult(pk_mat M, pk_vec x, pk_vec y, int reps, int fp32, double* avg_seconds);
+
+/* ---- mesh, cross sections and device assembly ---------------------------
+ * The single-domain mesh (Cells/Faces, Mesh.hxx:12-52) in ragged face layout:
+ * cell i owns faces face_ptr[i] .. face_ptr[i+1]-1.  fnb >= 0 is a cell,
+ * < 0 is -(boundary_index + 1).  cen/fcen/fnorm are xyz triples. */
+int pk_mesh_create(int num_cells, const int* face_ptr, const double* vol, const double* cen,
+                   const int* mat, const double* farea, const double* fcen, const double* fnorm,
+                   const int* fnb, pk_mesh* mesh);
+int pk_mesh_destroy(pk_mesh* mesh);
+/* Per-material tables (ConstantNuclearData accessors, Material.hxx:129-169):
+ * sigt/nusf/kapf/chi/D are [num_mat][G]; sigs is [num_mat][G from][G to].
+ * D may be NULL (SN). */
+int pk_xs_create(int num_mat, int G, const double* sigt, const double* sigs, const double* nusf,
+                 const double* kapf, const double* chi, const double* D, pk_xs* xs);
+int pk_xs_destroy(pk_xs* xs);
+/* R and F of DiffusionSolver::buildMatrices (DiffusionSolver.cxx:88-320, n = 0)
+ * assembled on the device; bc_type/bc_value are indexed by bc id =
+ * boundary_index + 1 (size num_bc).  Replaces the MatSetValues loop. */
+int pk_diffusion_assemble(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
+                          const double* bc_value, pk_mat* R, pk_mat* F);
+/* Transient rows (DiffusionSolver.cxx:134-148,168-175): R gains
+ * V/(v dt) on the diagonal and the prompt fission -(1-beta) chi_p nuSf V / keff;
+ * b = chi_d S + V/(v dt) phi0.  velocity/chip/chid are [num_mat][G], beta is
+ * [num_mat]; S [num_cells]; phi0 and b are pk_vec of size num_cells*G. */
+int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
+                                    const double* bc_value, const double* velocity, double dt,
+                                    const double* chip, const double* chid, const double* beta,
+                                    double keff, pk_vec S, pk_vec phi0, pk_mat* R, pk_vec b);
+
+/* Power normalisation and power/production rates on the device
+ * (NeutronicSolver.cxx:46-116): phi scaled so that sum phi kapf V = power;
+ * q[i] = sum_g phi kapf V, P[i] = sum_g phi nusf V / keff.  Returns 1 with an
+ * error if a scaled flux is negative. */
+int pk_normalize_flux(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double keff, pk_vec q,
+                      pk_vec P, double* total_power);
+
+/* ---- eigen solver (petsc::create(EPS&) petsc.cxx:175-204, EPSSetInitialSpace,
+ *      petsc::solve(EPS) petsc.cxx:428-488, EPSGetEigenpair) -------------
+ * Solves R x = lambda F x for the eigenvalue of largest 1/lambda (keff) by
+ * Wielandt-shifted power iteration with restarted GMRES and cell-block Jacobi
+ * inner solves.  tol is the relative residual |R x - F x / k| / |F x / k|. */
+int pk_eig_create(pk_mat R, pk_mat F, pk_eig* E);
+int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer);
+int pk_eig_set_precision(pk_eig E, int precision);
+int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner);
+int pk_eig_set_shift(pk_eig E, double delta); /* Wielandt shift k_s = k + delta (0: none) */
+int pk_eig_set_initial(pk_eig E, pk_vec x);
+int pk_eig_solve(pk_eig E);
+int pk_eig_get_eigenpair(pk_eig E, double* lambda, pk_vec x); /* lambda = 1/keff */
+int pk_eig_get_stats(pk_eig E, int* outer, long* inner, double* residual, double* seconds);
+int pk_eig_destroy(pk_eig* E);
+
+/* ---- linear solver (petsc::create(KSP&) petsc.cxx:147-172, petsc::solve(KSP)
+ *      petsc.cxx:382-425): GMRES + cell-block Jacobi, nonzero initial guess. */
+int pk_lin_create(pk_mat A, pk_lin* L);
+int pk_lin_set_tolerances(pk_lin L, double rtol, double atol, int max_it);
+int pk_lin_solve(pk_lin L, pk_vec b, pk_vec x);
+int pk_lin_get_stats(pk_lin L, long* iterations, double* residual);
+int pk_lin_destroy(pk_lin* L);
+
+/* ---- discrete ordinates (matrix-free form of SNSolver::buildMatrices in the
+ *      upwind regime: mixed-face-interpolation 1, LS off, vacuum boundaries).
+ * dirs[M][3], wts[M] from the level-symmetric set (AngularQuadratureSet.cxx:4-209).
+ * The operator R psi = lambda F psi of SNSolver.cxx:344-631 is applied as
+ * upwind sweeps over level schedules; psi layout is the reference's
+ * [cell][group][direction]. */
+int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double* wts, double delta,
+                 int num_bc, const int* bc_type, pk_sn* S);
+int pk_sn_get_schedule(pk_sn S, int* num_levels, int* level_of_cell); /* [M], [M][C] */
+int pk_sn_set_precision(pk_sn S, int precision);
+int pk_sn_set_tolerances(pk_sn S, double tol_k, double tol_phi, int max_outer);
+/* One sweep psi = L^-1 q for all (g, m); q[cell][group] includes the volume.
+ * phiw[cell][group] = sum_m w_m psi.  Host arrays, for parity checks. */
+int pk_sn_sweep_host(pk_sn S, const double* q, double* psi, double* phiw);
+int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw);
+int pk_sn_get_stats(pk_sn S, int* outer, long* sweeps, double* seconds);
+int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_sweep);
+int pk_sn_destroy(pk_sn* S);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,872 @@
+/*
+ * oracle.c -- FP64 CPU restatement of pampa's neutronics hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker and the CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it.  The product path (pampa_amd/) never links or calls it.
+ *
+ * Every function restates, from the mathematics and the reference's observable
+ * behaviour, one piece of the reference (manuel-garcia-roura/pampa); the
+ * file:line it follows is cited above each function.  All arithmetic is FP64,
+ * all connectivity int32, exactly as in the reference.
+ *
+ * Mesh arrays use a padded face layout: face slot (i, f) lives at i*fmax + f,
+ * for f < nfaces[i].  Neighbour codes follow the reference: >= 0 is a cell,
+ * < 0 is -(boundary_index + 1).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_OK 0
+#define ORC_ERR 1
+
+/* ------------------------------------------------------------------------ */
+/* geometry (math.cxx:4-161)                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Shoelace area of a polygon (math.cxx:4-17). */
+static double poly_area(const double* xy, const int* ids, int n) {
+   double a = 0.0;
+   for (int k = 0; k < n; k++) {
+      const double* p1 = xy + 2 * ids[k];
+      const double* p2 = xy + 2 * ids[(k + 1) % n];
+      a += p1[0] * p2[1] - p2[0] * p1[1];
+   }
+   return 0.5 * a;
+}
+
+/* Polygon centroid accumulated into c[0..1] (math.cxx:20-37). */
+static void poly_centroid(double* c, const double* xy, const int* ids, int n, double a) {
+   for (int k = 0; k < n; k++) {
+      const double* p1 = xy + 2 * ids[k];
+      const double* p2 = xy + 2 * ids[(k + 1) % n];
+      double da = p1[0] * p2[1] - p2[0] * p1[1];
+      c[0] += (p1[0] + p2[0]) * da;
+      c[1] += (p1[1] + p2[1]) * da;
+   }
+   c[0] *= (1.0 / (6.0 * a));
+   c[1] *= (1.0 / (6.0 * a));
+}
+
+static double dist3(const double* a, const double* b) {
+   double d = 0.0;
+   for (int k = 0; k < 3; k++) d += (b[k] - a[k]) * (b[k] - a[k]);
+   return sqrt(d);
+}
+
+/* surface leakage factor w = (p2-p1).n / |p2-p1|^2 (math.cxx:150-161). */
+static double slf(const double* p1, const double* p2, const double* n) {
+   double dp[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+   double num = dp[0] * n[0] + dp[1] * n[1] + dp[2] * n[2];
+   double den = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+   return num / den;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cartesian mesh (CartesianMesh.cxx:146-414)                                */
+/* ------------------------------------------------------------------------ */
+
+/* Count physical cells and check that every layer has the same voids
+ * (CartesianMesh.cxx:172-200).  mat is 0-based with -1 for void. */
+int orc_cart_count(int nx, int ny, int nz, const int* mat, int* num_cells) {
+   int nyy = ny > 0 ? ny : 1, nzz = nz > 0 ? nz : 1;
+   int nxy = 0;
+   for (int k = 0, im = 0; k < nzz; k++)
+      for (int j = 0; j < nyy; j++)
+         for (int i = 0; i < nx; i++, im++) {
+            int m = mat[im];
+            if (k == 0) {
+               if (m != -1) nxy++;
+            } else {
+               int m0 = mat[im - nx * nyy];
+               if ((m != -1) != (m0 != -1)) return ORC_ERR;
+            }
+         }
+   *num_cells = nxy * nzz;
+   return ORC_OK;
+}
+
+/* Build cells and faces.  bidx[0..5] = boundary-list index of -x,+x,-y,+y,-z,+z
+ * (-1 when the axis is absent).  Face order per cell: -y,+x,+y,-x,-z,+z with
+ * the y faces only when ny > 0 and the z faces only when nz > 0
+ * (CartesianMesh.cxx:280-400).  Voids are compacted with the reference's
+ * per-row left/right void counts (CartesianMesh.cxx:172-200, 303-304, 342-343). */
+int orc_cart_build(int nx, int ny, int nz, const double* dx, const double* dy, const double* dz,
+                   const int* mat, const int* bidx, int* fmax_out, double* vol, double* cen,
+                   int* cmat, int* nfaces, double* farea, double* fcen, double* fnorm, int* fnb) {
+   int nyy = ny > 0 ? ny : 1, nzz = nz > 0 ? nz : 1;
+   double* x = calloc(nx + 1, sizeof(double));
+   double* y = calloc(nyy + 1, sizeof(double));
+   double* z = calloc(nzz + 1, sizeof(double));
+   for (int i = 0; i < nx; i++) x[i + 1] = x[i] + dx[i];
+   for (int j = 0; j < ny; j++) y[j + 1] = y[j] + dy[j];
+   for (int k = 0; k < nz; k++) z[k + 1] = z[k] + dz[k];
+   /* per-row counts of voids before (0) and after (1) the first physical cell */
+   int* nvoid = calloc(2 * nyy, sizeof(int));
+   int nxy = 0;
+   for (int j = 0, im = 0; j < nyy; j++) {
+      int l = 0;
+      for (int i = 0; i < nx; i++, im++) {
+         if (mat[im] != -1) {
+            l = 1;
+            nxy++;
+         } else
+            nvoid[2 * j + l]++;
+      }
+   }
+   int fmax = (nz > 0) ? 6 : (ny > 0) ? 4 : 2;
+   *fmax_out = fmax;
+   int ic = 0;
+   for (int k = 0, im = 0; k < nzz; k++) {
+      for (int j = 0; j < nyy; j++) {
+         for (int i = 0; i < nx; i++, im++) {
+            if (mat[im] == -1) continue;
+            double dyj = dy[j], dzk = dz[k];
+            vol[ic] = (nz > 0) ? dx[i] * dyj * dzk : (ny > 0) ? dx[i] * dyj : dx[i];
+            cen[3 * ic + 0] = x[i] + 0.5 * dx[i];
+            cen[3 * ic + 1] = y[j] + 0.5 * dyj;
+            cen[3 * ic + 2] = z[k] + 0.5 * dzk;
+            cmat[ic] = mat[im];
+            int f = 0;
+            int base = ic * fmax;
+#define FACE(A, CX, CY, CZ, NX_, NY_, NZ_, NB)                                                     \
+   do {                                                                                            \
+      farea[base + f] = (A);                                                                       \
+      fcen[3 * (base + f) + 0] = (CX);                                                             \
+      fcen[3 * (base + f) + 1] = (CY);                                                             \
+      fcen[3 * (base + f) + 2] = (CZ);                                                             \
+      fnorm[3 * (base + f) + 0] = (NX_);                                                           \
+      fnorm[3 * (base + f) + 1] = (NY_);                                                           \
+      fnorm[3 * (base + f) + 2] = (NZ_);                                                           \
+      fnb[base + f] = (NB);                                                                        \
+      f++;                                                                                         \
+   } while (0)
+            if (ny > 0) {
+               int nb;
+               if (j == 0 || mat[im - nx] == -1)
+                  nb = -bidx[2] - 1;
+               else
+                  nb = ic - nx + nvoid[2 * j + 0] + nvoid[2 * (j - 1) + 1];
+               FACE((nz > 0) ? dx[i] * dzk : dx[i], x[i] + 0.5 * dx[i], y[j], z[k] + 0.5 * dzk, 0.0,
+                    -1.0, 0.0, nb);
+            }
+            {
+               int nb;
+               if (i == nx - 1 || mat[im + 1] == -1)
+                  nb = -bidx[1] - 1;
+               else
+                  nb = ic + 1;
+               FACE((nz > 0) ? dyj * dzk : (ny > 0) ? dyj : 1.0, x[i] + dx[i], y[j] + 0.5 * dyj,
+                    z[k] + 0.5 * dzk, 1.0, 0.0, 0.0, nb);
+            }
+            if (ny > 0) {
+               int nb;
+               if (j == ny - 1 || mat[im + nx] == -1)
+                  nb = -bidx[3] - 1;
+               else
+                  nb = ic + nx - nvoid[2 * j + 1] - nvoid[2 * (j + 1) + 0];
+               FACE((nz > 0) ? dx[i] * dzk : dx[i], x[i] + 0.5 * dx[i], y[j] + dyj, z[k] + 0.5 * dzk,
+                    0.0, 1.0, 0.0, nb);
+            }
+            {
+               int nb;
+               if (i == 0 || mat[im - 1] == -1)
+                  nb = -bidx[0] - 1;
+               else
+                  nb = ic - 1;
+               FACE((nz > 0) ? dyj * dzk : (ny > 0) ? dyj : 1.0, x[i], y[j] + 0.5 * dyj,
+                    z[k] + 0.5 * dzk, -1.0, 0.0, 0.0, nb);
+            }
+            if (nz > 0) {
+               FACE(dx[i] * dyj, x[i] + 0.5 * dx[i], y[j] + 0.5 * dyj, z[k], 0.0, 0.0, -1.0,
+                    (k == 0) ? -bidx[4] - 1 : ic - nxy);
+               FACE(dx[i] * dyj, x[i] + 0.5 * dx[i], y[j] + 0.5 * dyj, z[k] + dzk, 0.0, 0.0, 1.0,
+                    (k == nz - 1) ? -bidx[5] - 1 : ic + nxy);
+            }
+#undef FACE
+            nfaces[ic] = f;
+            ic++;
+         }
+      }
+   }
+   free(x);
+   free(y);
+   free(z);
+   free(nvoid);
+   return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Unstructured extruded mesh (UnstructuredExtrudedMesh.cxx:153-364)         */
+/* ------------------------------------------------------------------------ */
+
+/* xy polygons (cptr/cpts, CCW) extruded over nz layers (nz = 0: 2-D).
+ * Boundaries: nb point lists (bptr/bpts) with global boundary indices bglob[],
+ * defb = index (into the nb lists) of the default boundary or -1;
+ * bz[0..1] = global boundary indices of -z/+z.  Lateral neighbours come from
+ * point->cell lists searched in the reference's order
+ * (UnstructuredExtrudedMesh.cxx:232-278).  Returns ORC_ERR on bad connectivity. */
+int orc_unst_build(int npts, const double* xy, int ncxy, const int* cptr, const int* cpts, int nz,
+                   const double* dz, int nb, const int* bptr, const int* bpts, const int* bglob,
+                   int defb, const int* bz, int fmax, double* vol, double* cen, int* cmat_unused,
+                   int* nfaces, double* farea, double* fcen, double* fnorm, int* fnb) {
+   (void)cmat_unused;
+   int nzz = nz > 0 ? nz : 1;
+   double* z = calloc(nzz + 1, sizeof(double));
+   for (int k = 0; k < nz; k++) z[k + 1] = z[k] + dz[k];
+   /* point -> cells (then boundary codes) lists */
+   int* cnt = calloc(npts, sizeof(int));
+   for (int i = 0; i < ncxy; i++)
+      for (int j = cptr[i]; j < cptr[i + 1]; j++) cnt[cpts[j]]++;
+   for (int b = 0; b < nb; b++)
+      for (int j = bptr[b]; j < bptr[b + 1]; j++) cnt[bpts[j]]++;
+   int* pptr = calloc(npts + 1, sizeof(int));
+   for (int p = 0; p < npts; p++) pptr[p + 1] = pptr[p] + cnt[p];
+   int* plist = malloc(sizeof(int) * (pptr[npts] + 1));
+   memset(cnt, 0, sizeof(int) * npts);
+   for (int i = 0; i < ncxy; i++)
+      for (int j = cptr[i]; j < cptr[i + 1]; j++) {
+         int p = cpts[j];
+         plist[pptr[p] + cnt[p]++] = i;
+      }
+   for (int b = 0; b < nb; b++)
+      for (int j = bptr[b]; j < bptr[b + 1]; j++) {
+         int p = bpts[j];
+         plist[pptr[p] + cnt[p]++] = -bglob[b] - 1;
+      }
+   /* lateral neighbours */
+   int* xyn = calloc(cptr[ncxy] + 1, sizeof(int));
+   int err = ORC_OK;
+   for (int i = 0; i < ncxy && !err; i++) {
+      int n = cptr[i + 1] - cptr[i];
+      for (int f = 0; f < n; f++) {
+         int ip1 = cpts[cptr[i] + f], ip2 = cpts[cptr[i] + (f + 1) % n];
+         int found = 0;
+         for (int l1 = pptr[ip1]; l1 < pptr[ip1 + 1]; l1++)
+            for (int l2 = pptr[ip2]; l2 < pptr[ip2 + 1]; l2++) {
+               int i1 = plist[l1], i2 = plist[l2];
+               if (i1 == i2 && i1 != i) {
+                  if (xyn[cptr[i] + f] > 0) err = ORC_ERR;
+                  xyn[cptr[i] + f] = i1;
+                  found = 1;
+               }
+            }
+         if (!found && defb >= 0) {
+            xyn[cptr[i] + f] = -bglob[defb] - 1;
+            found = 1;
+         }
+         if (!found) err = ORC_ERR;
+      }
+   }
+   if (!err) {
+      int ic = 0;
+      for (int k = 0; k < nzz; k++) {
+         double dzk = dz[k];
+         for (int i = 0; i < ncxy; i++, ic++) {
+            const int* ids = cpts + cptr[i];
+            int n = cptr[i + 1] - cptr[i];
+            double a = poly_area(xy, ids, n);
+            vol[ic] = (nz > 0) ? a * dzk : a;
+            double c[2] = {0.0, 0.0};
+            poly_centroid(c, xy, ids, n, a);
+            cen[3 * ic + 0] = c[0];
+            cen[3 * ic + 1] = c[1];
+            cen[3 * ic + 2] = z[k] + 0.5 * dzk;
+            int base = ic * fmax, f;
+            for (f = 0; f < n; f++) {
+               const double* p1 = xy + 2 * ids[f];
+               const double* p2 = xy + 2 * ids[(f + 1) % n];
+               double ddx = p2[0] - p1[0], ddy = p2[1] - p1[1];
+               double len = sqrt(ddx * ddx + ddy * ddy);
+               farea[base + f] = (nz > 0) ? len * dzk : len;
+               fcen[3 * (base + f) + 0] = 0.5 * (p1[0] + p2[0]);
+               fcen[3 * (base + f) + 1] = 0.5 * (p1[1] + p2[1]);
+               fcen[3 * (base + f) + 2] = z[k] + 0.5 * dzk;
+               double nx_ = p2[1] - p1[1], ny_ = p1[0] - p2[0];
+               double nn = sqrt(nx_ * nx_ + ny_ * ny_);
+               fnorm[3 * (base + f) + 0] = nx_ / nn;
+               fnorm[3 * (base + f) + 1] = ny_ / nn;
+               fnorm[3 * (base + f) + 2] = 0.0;
+               int nbr = xyn[cptr[i] + f];
+               fnb[base + f] = nbr >= 0 ? nbr + k * ncxy : nbr;
+            }
+            if (nz > 0) {
+               for (int s = 0; s < 2; s++, f++) {
+                  double fa = poly_area(xy, ids, n);
+                  double fc[2] = {0.0, 0.0};
+                  poly_centroid(fc, xy, ids, n, fa);
+                  farea[base + f] = fa;
+                  fcen[3 * (base + f) + 0] = fc[0];
+                  fcen[3 * (base + f) + 1] = fc[1];
+                  fcen[3 * (base + f) + 2] = s == 0 ? z[k] : z[k] + dzk;
+                  fnorm[3 * (base + f) + 0] = 0.0;
+                  fnorm[3 * (base + f) + 1] = 0.0;
+                  fnorm[3 * (base + f) + 2] = s == 0 ? -1.0 : 1.0;
+                  if (s == 0)
+                     fnb[base + f] = (k == 0) ? -bz[0] - 1 : ic - ncxy;
+                  else
+                     fnb[base + f] = (k == nz - 1) ? -bz[1] - 1 : ic + ncxy;
+               }
+            }
+            nfaces[ic] = f;
+         }
+      }
+   }
+   free(z);
+   free(cnt);
+   free(pptr);
+   free(plist);
+   free(xyn);
+   return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PETSc-AIJ row semantics                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Sort (stable) a row by column and merge duplicate columns: add != 0 sums the
+ * values in insertion order (ADD_VALUES), add == 0 keeps the last inserted
+ * value (INSERT_VALUES).  Explicit zeros are kept, as MatSetValues keeps them.
+ * Returns the merged length. */
+static int aij_merge(int n, int* col, double* val, int add) {
+   for (int a = 1; a < n; a++) {
+      int c = col[a];
+      double v = val[a];
+      int b = a - 1;
+      while (b >= 0 && col[b] > c) {
+         col[b + 1] = col[b];
+         val[b + 1] = val[b];
+         b--;
+      }
+      col[b + 1] = c;
+      val[b + 1] = v;
+   }
+   int m = 0;
+   for (int a = 0; a < n; a++) {
+      if (m > 0 && col[m - 1] == col[a]) {
+         if (add)
+            val[m - 1] += val[a];
+         else
+            val[m - 1] = val[a];
+      } else {
+         col[m] = col[a];
+         val[m] = val[a];
+         m++;
+      }
+   }
+   return m;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Diffusion operator (DiffusionSolver.cxx:88-320, n = 0)                    */
+/* ------------------------------------------------------------------------ */
+
+/* Boundary types, as in the reference's BC::Type (utils.hxx:103-105). */
+enum { BC_NONE = 0, BC_VACUUM = 1, BC_REFLECTIVE = 2, BC_ROBIN = 3 };
+
+/* Material tables: sigt[m*G+g], sigs[(m*G+from)*G+to], nusf[m*G+g],
+ * chi[m*G+g] (effective spectrum), D[m*G+g].  Boundary table indexed by
+ * bc id = boundary_index + 1: bct[id], bcv[id] (Robin f at t).
+ * Two-pass: with r_col == NULL only r_ptr/f_ptr are filled.
+ * Time-dependent rows (n > 0, DiffusionSolver.cxx:134-148,168-175) are added
+ * when dtv != NULL: dtv[m*G+g] = 1/(v*dt), and then F is replaced by the
+ * prompt-fission term folded into R with chip/beta/keff. */
+int orc_diff_assemble(int C, int fmax, const int* nfaces, const double* vol, const double* cen,
+                      const int* cmat, const double* farea, const double* fcen, const double* fnorm,
+                      const int* fnb, int G, const double* sigt, const double* sigs,
+                      const double* nusf, const double* chi, const double* D, const int* bct,
+                      const double* bcv, int* r_ptr, int* r_col, double* r_val, int* f_ptr,
+                      int* f_col, double* f_val) {
+   int cap = G + fmax;
+   int* cols = malloc(sizeof(int) * cap);
+   double* vals = malloc(sizeof(double) * cap);
+   int err = ORC_OK;
+   if (r_col == NULL) {
+      r_ptr[0] = 0;
+      f_ptr[0] = 0;
+   }
+   for (int i = 0; i < C && !err; i++) {
+      int m = cmat[i];
+      double V = vol[i];
+      for (int g = 0; g < G && !err; g++) {
+         int l = i * G + g;
+         int n = 0;
+         cols[n] = l;
+         vals[n] = sigt[m * G + g] * V;
+         n = 1;
+         for (int g2 = 0; g2 < G; g2++) {
+            double s = -sigs[(m * G + g2) * G + g] * V;
+            if (g2 == g)
+               vals[0] += s;
+            else {
+               cols[n] = i * G + g2;
+               vals[n] = s;
+               n++;
+            }
+         }
+         for (int f = 0; f < nfaces[i]; f++) {
+            int s = i * fmax + f;
+            int i2 = fnb[s];
+            if (i2 < 0) {
+               int id = -i2;
+               int t = bct[id];
+               if (t == BC_VACUUM) {
+                  double w = slf(cen + 3 * i, fcen + 3 * s, fnorm + 3 * s);
+                  vals[0] += w * D[m * G + g] * farea[s];
+               } else if (t == BC_REFLECTIVE) {
+               } else if (t == BC_ROBIN) {
+                  vals[0] -= bcv[id] * farea[s];
+               } else {
+                  err = ORC_ERR;
+               }
+            } else {
+               int m2 = cmat[i2];
+               double r;
+               if (m2 == m) {
+                  double w = slf(cen + 3 * i, cen + 3 * i2, fnorm + 3 * s);
+                  r = -w * D[m * G + g] * farea[s];
+               } else {
+                  double w1 = slf(cen + 3 * i, fcen + 3 * s, fnorm + 3 * s);
+                  w1 *= D[m * G + g] * farea[s];
+                  double w2 = slf(cen + 3 * i2, fcen + 3 * s, fnorm + 3 * s);
+                  w2 *= -D[m2 * G + g] * farea[s];
+                  r = -(w1 * w2) / (w1 + w2);
+               }
+               vals[0] -= r;
+               cols[n] = i2 * G + g;
+               vals[n] = r;
+               n++;
+            }
+         }
+         int nr = aij_merge(n, cols, vals, 0);
+         if (r_col == NULL) {
+            r_ptr[l + 1] = r_ptr[l] + nr;
+            f_ptr[l + 1] = f_ptr[l] + G;
+         } else {
+            memcpy(r_col + r_ptr[l], cols, sizeof(int) * nr);
+            memcpy(r_val + r_ptr[l], vals, sizeof(double) * nr);
+            for (int g2 = 0; g2 < G; g2++) {
+               f_col[f_ptr[l] + g2] = i * G + g2;
+               f_val[f_ptr[l] + g2] = chi[m * G + g] * nusf[m * G + g2] * V;
+            }
+         }
+      }
+   }
+   free(cols);
+   free(vals);
+   return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SN: face-interpolation coefficients (SNSolver.cxx:159-208)                */
+/* ------------------------------------------------------------------------ */
+
+/* coef[(i*fmax+f)*4 + 0..3] for interior faces, with weight delta between
+ * upwind (1) and linear (0) face interpolation. */
+void orc_sn_gauss(int C, int fmax, const int* nfaces, const double* cen, const double* fcen,
+                  const int* fnb, double delta, double* coef) {
+   for (int i = 0; i < C; i++)
+      for (int f = 0; f < nfaces[i]; f++) {
+         int s = i * fmax + f, i2 = fnb[s];
+         if (i2 < 0) continue;
+         double rif = dist3(fcen + 3 * s, cen + 3 * i);
+         double ri2f = dist3(fcen + 3 * s, cen + 3 * i2);
+         double rii2 = dist3(cen + 3 * i, cen + 3 * i2);
+         coef[4 * s + 0] = (ri2f + delta * rif) / rii2;
+         coef[4 * s + 1] = (1.0 - delta) * rif / rii2;
+         coef[4 * s + 2] = (1.0 - delta) * ri2f / rii2;
+         coef[4 * s + 3] = (rif + delta * ri2f) / rii2;
+      }
+}
+
+/* Least-squares boundary gradient coefficients (SNSolver.cxx:212-268) for
+ * boundary cells: lsc[(ibc*fmax+f)*3 + d].  The reference reads its
+ * num_faces x num_dims row-major matrix d as d(jg, f) and d(jd, f), i.e. the
+ * flat element jg*num_dims + f (SNSolver.cxx:252,262); that indexing is kept.
+ * Its Eigen matrix G is never zeroed (SNSolver.cxx:248); here it starts at
+ * zero, which is exact whenever the slot happens to be clean and always in 1-D
+ * where the flat indexing is also exact. */
+void orc_sn_ls(int C, int fmax, const int* nfaces, const double* cen, const double* fcen,
+               const int* fnb, int ndims, const int* ic2bc, double* lsc) {
+   for (int i = 0; i < C; i++) {
+      int ic = ic2bc[i];
+      if (ic < 0) continue;
+      int nf = nfaces[i];
+      double* d = calloc((size_t)nf * ndims + 1, sizeof(double));
+      for (int f = 0; f < nf; f++) {
+         int s = i * fmax + f, i2 = fnb[s];
+         const double* c2 = (i2 < 0) ? fcen + 3 * s : cen + 3 * i2;
+         for (int id = 0; id < ndims; id++) d[f * ndims + id] = c2[id] - cen[3 * i + id];
+      }
+      double Gm[9] = {0}, Gi[9] = {0};
+      for (int jg = 0; jg < ndims; jg++)
+         for (int ig = 0; ig < ndims; ig++)
+            for (int f = 0; f < nf; f++)
+               Gm[jg * 3 + ig] += d[jg * ndims + f] * d[f * ndims + ig];
+      if (ndims == 1) {
+         Gi[0] = 1.0 / Gm[0];
+      } else if (ndims == 2) {
+         double det = Gm[0] * Gm[4] - Gm[1] * Gm[3];
+         Gi[0] = Gm[4] / det;
+         Gi[1] = -Gm[1] / det;
+         Gi[3] = -Gm[3] / det;
+         Gi[4] = Gm[0] / det;
+      } else {
+         double a = Gm[0], b = Gm[1], c = Gm[2], e = Gm[3], f_ = Gm[4], g = Gm[5], h = Gm[6],
+                k = Gm[7], l = Gm[8];
+         double det = a * (f_ * l - g * k) - b * (e * l - g * h) + c * (e * k - f_ * h);
+         Gi[0] = (f_ * l - g * k) / det;
+         Gi[1] = (c * k - b * l) / det;
+         Gi[2] = (b * g - c * f_) / det;
+         Gi[3] = (g * h - e * l) / det;
+         Gi[4] = (a * l - c * h) / det;
+         Gi[5] = (c * e - a * g) / det;
+         Gi[6] = (e * k - f_ * h) / det;
+         Gi[7] = (b * h - a * k) / det;
+         Gi[8] = (a * f_ - b * e) / det;
+      }
+      for (int f = 0; f < nf; f++)
+         for (int id = 0; id < ndims; id++)
+            for (int jd = 0; jd < ndims; jd++)
+               lsc[(ic * fmax + f) * 3 + id] += Gi[id * 3 + jd] * d[jd * ndims + f];
+      free(d);
+   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* SN operator in reference form (SNSolver.cxx:344-631, n = 0)               */
+/* ------------------------------------------------------------------------ */
+
+/* Row l = (i*G+g)*M+m.  dirs[m*3+k], wts[m], refl[m*3+axis].  coef from
+ * orc_sn_gauss; ls != 0 enables the least-squares boundary correction with
+ * lsc/ic2bc from orc_sn_ls.  The reflective branch looks up the reflection
+ * axis from faces.normals(a, f) with a = 0,1,2 -- the loop variable shadows
+ * the cell index at SNSolver.cxx:541-545 -- and that lookup is kept.
+ * Two-pass: r_col == NULL fills only r_ptr/f_ptr. */
+int orc_sn_assemble(int C, int fmax, const int* nfaces, const double* vol, const double* cen,
+                    const int* cmat, const double* farea, const double* fcen, const double* fnorm,
+                    const int* fnb, int G, int M, const double* dirs, const double* wts,
+                    const int* refl, const double* coef, int ls, const int* ic2bc,
+                    const double* lsc, const double* sigt, const double* sigs, const double* nusf,
+                    const double* chi, const int* bct, int* r_ptr, int* r_col, double* r_val,
+                    int* f_ptr, int* f_col, double* f_val) {
+   int GM = G * M;
+   int cap = GM + fmax * (fmax + 2) + 4;
+   int* cols = malloc(sizeof(int) * cap);
+   double* vals = malloc(sizeof(double) * cap);
+   int err = ORC_OK;
+   if (r_col == NULL) {
+      r_ptr[0] = 0;
+      f_ptr[0] = 0;
+   }
+   for (int i = 0; i < C && !err; i++) {
+      int mt = cmat[i];
+      double V = vol[i];
+      for (int g = 0; g < G && !err; g++)
+         for (int m = 0; m < M && !err; m++) {
+            int l = (i * G + g) * M + m;
+            /* The per-face MatSetValues(ADD) calls come first, then the block row
+             * (SNSolver.cxx:512,551,595 before :602).  Collect in that order. */
+            int n = 0;
+            double diag = sigt[mt * G + g] * V;
+            for (int f = 0; f < nfaces[i]; f++) {
+               int s = i * fmax + f, i2 = fnb[s];
+               const double* om = dirs + 3 * m;
+               const double* nn = fnorm + 3 * s;
+               double w = om[0] * nn[0] + om[1] * nn[1] + om[2] * nn[2];
+               if (i2 < 0) {
+                  int t = bct[-i2];
+                  if (t == BC_VACUUM) {
+                     if (w > 0.0) {
+                        diag += w * farea[s];
+                        if (ls) {
+                           double dp[3] = {fcen[3 * s] - cen[3 * i], fcen[3 * s + 1] - cen[3 * i + 1],
+                                           fcen[3 * s + 2] - cen[3 * i + 2]};
+                           for (int f2 = 0; f2 < nfaces[i]; f2++) {
+                              int i3 = fnb[i * fmax + f2];
+                              if (i3 < 0) continue;
+                              const double* gc = lsc + (ic2bc[i] * fmax + f2) * 3;
+                              double wi3 = dp[0] * gc[0] + dp[1] * gc[1] + dp[2] * gc[2];
+                              diag += -wi3 * w * farea[s];
+                              cols[n] = (i3 * G + g) * M + m;
+                              vals[n] = wi3 * w * farea[s];
+                              n++;
+                           }
+                        }
+                     }
+                  } else if (t == BC_REFLECTIVE) {
+                     if (w > 0.0)
+                        diag += w * farea[s];
+                     else {
+                        int m2 = -1;
+                        for (int a = 0; a < 3; a++) {
+                           const double* na = fnorm + 3 * (a * fmax + f);
+                           if (fabs(na[a]) > 1.0 - 1.0e-6) m2 = refl[m * 3 + a];
+                        }
+                        if (m2 < 0) {
+                           err = ORC_ERR;
+                           break;
+                        }
+                        cols[n] = (i * G + g) * M + m2;
+                        vals[n] = w * farea[s];
+                        n++;
+                     }
+                  } else {
+                     err = ORC_ERR;
+                     break;
+                  }
+               } else {
+                  double wi, wi2;
+                  if (w > 0.0) {
+                     wi = coef[4 * s + 0];
+                     wi2 = coef[4 * s + 1];
+                  } else {
+                     wi = coef[4 * s + 2];
+                     wi2 = coef[4 * s + 3];
+                  }
+                  diag += wi * w * farea[s];
+                  cols[n] = (i2 * G + g) * M + m;
+                  vals[n] = wi2 * w * farea[s];
+                  n++;
+               }
+            }
+            /* block row: diagonal carries the total, scattering on the block */
+            int nd = n;
+            cols[n] = l;
+            vals[n] = diag;
+            n++;
+            for (int g2 = 0; g2 < G; g2++)
+               for (int m2 = 0; m2 < M; m2++) {
+                  int l2 = (i * G + g2) * M + m2;
+                  double sv = -sigs[(mt * G + g2) * G + g] * wts[m2] * V;
+                  if (l2 == l)
+                     vals[nd] += sv;
+                  else {
+                     cols[n] = l2;
+                     vals[n] = sv;
+                     n++;
+                  }
+               }
+            int nr = aij_merge(n, cols, vals, 1);
+            if (r_col == NULL) {
+               r_ptr[l + 1] = r_ptr[l] + nr;
+               f_ptr[l + 1] = f_ptr[l] + GM;
+            } else {
+               memcpy(r_col + r_ptr[l], cols, sizeof(int) * nr);
+               memcpy(r_val + r_ptr[l], vals, sizeof(double) * nr);
+               int p = f_ptr[l];
+               for (int g2 = 0; g2 < G; g2++)
+                  for (int m2 = 0; m2 < M; m2++, p++) {
+                     f_col[p] = (i * G + g2) * M + m2;
+                     f_val[p] = chi[mt * G + g] * nusf[mt * G + g2] * wts[m2] * V;
+                  }
+            }
+         }
+   }
+   free(cols);
+   free(vals);
+   return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SN sweeps (no reference counterpart: the reference assembles instead)     */
+/* ------------------------------------------------------------------------ */
+
+/* Upwind order per direction.  Cell i depends on interior neighbour i2 through
+ * face f when Omega_m . n_f < 0 (the incoming branch of SNSolver.cxx:574-597
+ * with delta = 1).  order[m*C + k] lists cells level by level, ascending cell
+ * index inside a level; level[m*C+i] is the level of cell i; nlev[m] the level
+ * count.  Returns ORC_ERR if a direction's dependency graph has a cycle. */
+int orc_sn_order(int C, int fmax, const int* nfaces, const double* fnorm, const int* fnb, int M,
+                 const double* dirs, int* order, int* level, int* nlev) {
+   int* indeg = malloc(sizeof(int) * C);
+   int* cur = malloc(sizeof(int) * C);
+   int* nxt = malloc(sizeof(int) * C);
+   int err = ORC_OK;
+   for (int m = 0; m < M && !err; m++) {
+      const double* om = dirs + 3 * m;
+      for (int i = 0; i < C; i++) {
+         indeg[i] = 0;
+         for (int f = 0; f < nfaces[i]; f++) {
+            int s = i * fmax + f;
+            if (fnb[s] < 0) continue;
+            const double* nn = fnorm + 3 * s;
+            double w = om[0] * nn[0] + om[1] * nn[1] + om[2] * nn[2];
+            if (w < 0.0) indeg[i]++;
+         }
+      }
+      int ncur = 0, done = 0, lev = 0;
+      for (int i = 0; i < C; i++)
+         if (indeg[i] == 0) cur[ncur++] = i;
+      while (ncur > 0) {
+         for (int k = 0; k < ncur; k++) {
+            order[(size_t)m * C + done + k] = cur[k];
+            level[(size_t)m * C + cur[k]] = lev;
+         }
+         done += ncur;
+         int nn_ = 0;
+         for (int k = 0; k < ncur; k++) {
+            int i = cur[k];
+            for (int f = 0; f < nfaces[i]; f++) {
+               int s = i * fmax + f, i2 = fnb[s];
+               if (i2 < 0) continue;
+               const double* nn = fnorm + 3 * s;
+               double w = om[0] * nn[0] + om[1] * nn[1] + om[2] * nn[2];
+               /* outgoing from i through f: i2 sees it as incoming */
+               if (w > 0.0) {
+                  if (--indeg[i2] == 0) nxt[nn_++] = i2;
+               }
+            }
+         }
+         /* ascending order within a level */
+         for (int a = 1; a < nn_; a++) {
+            int v = nxt[a], b = a - 1;
+            while (b >= 0 && nxt[b] > v) {
+               nxt[b + 1] = nxt[b];
+               b--;
+            }
+            nxt[b + 1] = v;
+         }
+         memcpy(cur, nxt, sizeof(int) * nn_);
+         ncur = nn_;
+         lev++;
+      }
+      nlev[m] = lev;
+      if (done != C) err = ORC_ERR;
+   }
+   free(indeg);
+   free(cur);
+   free(nxt);
+   return err;
+}
+
+/* One transport sweep for every (g, m): solves L psi = q with q[i*G+g] the
+ * isotropic source (already multiplied by the volume) for the delta = 1,
+ * LS-off, vacuum-boundary operator of SNSolver.cxx:397,467-482,574-597:
+ *   (Sigma_t V + sum_out w c A + sum_vac,out w A) psi_i
+ *      = q_i - sum_in w c A psi_i2            (w = Omega.n < 0 on incoming faces)
+ * with c = coef[4*s+0] on outgoing and coef[4*s+3] on incoming faces.
+ * psi[(i*G+g)*M+m]; phiw[i*G+g] = sum_m w_m psi (the weighted moment). */
+void orc_sn_sweep(int C, int fmax, const int* nfaces, const double* vol, const int* cmat,
+                  const double* farea, const double* fnorm, const int* fnb, int G, int M,
+                  const double* dirs, const double* wts, const double* coef, const double* sigt,
+                  const int* order, const double* q, double* psi, double* phiw) {
+   for (int m = 0; m < M; m++) {
+      const double* om = dirs + 3 * m;
+      for (int k = 0; k < C; k++) {
+         int i = order[(size_t)m * C + k];
+         int mt = cmat[i];
+         double lhs0 = 0.0;
+         double inflow[64];
+         for (int g = 0; g < G; g++) inflow[g] = 0.0;
+         for (int f = 0; f < nfaces[i]; f++) {
+            int s = i * fmax + f, i2 = fnb[s];
+            const double* nn = fnorm + 3 * s;
+            double w = om[0] * nn[0] + om[1] * nn[1] + om[2] * nn[2];
+            if (i2 < 0) {
+               if (w > 0.0) lhs0 += w * farea[s];
+            } else if (w > 0.0) {
+               lhs0 += coef[4 * s + 0] * w * farea[s];
+            } else {
+               double c = coef[4 * s + 3] * w * farea[s];
+               for (int g = 0; g < G; g++) inflow[g] += c * psi[((size_t)i2 * G + g) * M + m];
+            }
+         }
+         for (int g = 0; g < G; g++)
+            psi[((size_t)i * G + g) * M + m] =
+                (q[(size_t)i * G + g] - inflow[g]) / (sigt[mt * G + g] * vol[i] + lhs0);
+      }
+   }
+   for (int i = 0; i < C; i++)
+      for (int g = 0; g < G; g++) {
+         double s = 0.0;
+         for (int m = 0; m < M; m++) s += wts[m] * psi[((size_t)i * G + g) * M + m];
+         phiw[(size_t)i * G + g] = s;
+      }
+}
+
+/* Isotropic source q = V (sum_g2 Sigma_s(g2->g) phiw(g2) + chi_g/k sum_g2 nuSf(g2) phiw(g2))
+ * (the -Sigma_s w V and chi nuSf w V blocks of SNSolver.cxx:426-439 moved to the
+ * right-hand side). */
+void orc_sn_source(int C, int G, const double* vol, const int* cmat, const double* sigs,
+                   const double* nusf, const double* chi, double inv_k, const double* phiw,
+                   double* q) {
+   for (int i = 0; i < C; i++) {
+      int mt = cmat[i];
+      double fis = 0.0;
+      for (int g2 = 0; g2 < G; g2++) fis += nusf[mt * G + g2] * phiw[(size_t)i * G + g2];
+      for (int g = 0; g < G; g++) {
+         double s = 0.0;
+         for (int g2 = 0; g2 < G; g2++) s += sigs[(mt * G + g2) * G + g] * phiw[(size_t)i * G + g2];
+         q[(size_t)i * G + g] = vol[i] * (s + chi[mt * G + g] * inv_k * fis);
+      }
+   }
+}
+
+/* Source iteration inside power iteration for the SN eigenproblem
+ * R psi = (1/k) F psi in the sweepable regime.  Each outer: `inner` sweeps
+ * with the fission source frozen; k from the fission-rate ratio.  Stops when
+ * |dk| < tol_k and the relative L2 change of phiw < tol_phi, or after max_outer.
+ * phiw holds the initial guess on entry.  Returns the outer count. */
+int orc_sn_eigen(int C, int fmax, const int* nfaces, const double* vol, const int* cmat,
+                 const double* farea, const double* fnorm, const int* fnb, int G, int M,
+                 const double* dirs, const double* wts, const double* coef, const double* sigt,
+                 const double* sigs, const double* nusf, const double* chi, const int* order,
+                 int inner, int max_outer, double tol_k, double tol_phi, double* keff,
+                 double* psi, double* phiw) {
+   size_t n = (size_t)C * G;
+   double* q = malloc(sizeof(double) * n);
+   double* old = malloc(sizeof(double) * n);
+   double k = *keff;
+   int it;
+   double prod0 = 0.0;
+   for (size_t a = 0; a < n; a++) prod0 += nusf[cmat[a / G] * G + a % G] * phiw[a] * vol[a / G];
+   for (it = 0; it < max_outer; it++) {
+      memcpy(old, phiw, sizeof(double) * n);
+      /* fission source frozen at the previous outer's flux */
+      double* fis = malloc(sizeof(double) * C);
+      for (int i = 0; i < C; i++) {
+         double s = 0.0;
+         for (int g2 = 0; g2 < G; g2++) s += nusf[cmat[i] * G + g2] * old[(size_t)i * G + g2];
+         fis[i] = s;
+      }
+      for (int s_ = 0; s_ < inner; s_++) {
+         for (int i = 0; i < C; i++) {
+            int mt = cmat[i];
+            for (int g = 0; g < G; g++) {
+               double sc = 0.0;
+               for (int g2 = 0; g2 < G; g2++) sc += sigs[(mt * G + g2) * G + g] * phiw[(size_t)i * G + g2];
+               q[(size_t)i * G + g] = vol[i] * (sc + chi[mt * G + g] / k * fis[i]);
+            }
+         }
+         orc_sn_sweep(C, fmax, nfaces, vol, cmat, farea, fnorm, fnb, G, M, dirs, wts, coef, sigt,
+                      order, q, psi, phiw);
+      }
+      free(fis);
+      double prod = 0.0;
+      for (size_t a = 0; a < n; a++) prod += nusf[cmat[a / G] * G + a % G] * phiw[a] * vol[a / G];
+      double knew = k * prod / prod0;
+      /* rescale so the production rate stays prod0 */
+      double sc = prod0 / prod;
+      double dn = 0.0, nn = 0.0;
+      for (size_t a = 0; a < n; a++) {
+         phiw[a] *= sc;
+         dn += (phiw[a] - old[a]) * (phiw[a] - old[a]);
+         nn += phiw[a] * phiw[a];
+      }
+      size_t np = n * M;
+      for (size_t a = 0; a < np; a++) psi[a] *= sc;
+      double dk = fabs(knew - k);
+      k = knew;
+      if (dk < tol_k && sqrt(dn / nn) < tol_phi) {
+         it++;
+         break;
+      }
+   }
+   *keff = k;
+   free(q);
+   free(old);
+   return it;
+}

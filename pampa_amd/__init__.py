@@ -1,0 +1,260 @@
+"""pampa_amd -- MI355X-native neutronics hot path of pampa.
+
+The product is the in-tree shared library libpampa_amd.so (HIP kernels for
+gfx950, the pk_* numerics C-ABI of include/pk.h, and the C++ host that keeps
+pampa's parser, meshes, solvers and the pampa_* C-ABI of include/pampa.h).
+This module binds that library with ctypes for Python callers (tests,
+benchmarks).  It fails loudly when the library has not been built: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpampa_amd.so")
+CLI_PATH = os.path.join(HERE, "bin", "pampa")
+
+PK_INSERT_VALUES, PK_ADD_VALUES = 0, 1
+PK_PREC_FP64, PK_PREC_MIXED = 0, 1
+PK_BC_NONE, PK_BC_VACUUM, PK_BC_REFLECTIVE, PK_BC_ROBIN = 0, 1, 2, 3
+
+_lib = None
+
+
+class PkError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpampa_amd.so (raises if it is missing: build it first)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PkError(f"{LIB_PATH} is not built; run `python -m pampa_amd.build`")
+        _lib = ct.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ct.c_void_p
+_PP = ct.POINTER(ct.c_void_p)
+_I = ct.c_int
+_L = ct.c_long
+_D = ct.c_double
+
+
+def _declare(L):
+    sig = {
+        "pk_initialize": [_I], "pk_finalize": [], "pk_synchronize": [],
+        "pk_vec_create": [_L, _PP], "pk_vec_destroy": [_PP], "pk_vec_set": [_P, _D],
+        "pk_vec_set_values": [_P, _P], "pk_vec_get_values": [_P, _P], "pk_vec_copy": [_P, _P],
+        "pk_vec_scale": [_P, _D], "pk_vec_axpy": [_P, _D, _P], "pk_vec_dot": [_P, _P, _P],
+        "pk_vec_norm2": [_P, _P], "pk_vec_device_ptr": [_P, _PP],
+        "pk_mat_create": [_L, _I, _PP], "pk_mat_set_values": [_P, _I, _P, _I, _P, _P, _I],
+        "pk_mat_assemble": [_P], "pk_mat_destroy": [_PP], "pk_mat_get_size": [_P, _P, _P],
+        "pk_mat_get_csr": [_P, _P, _P, _P], "pk_mat_mult": [_P, _P, _P],
+        "pk_mat_set_block_size": [_P, _I], "pk_mat_time_mult": [_P, _P, _P, _I, _I, _P],
+        "pk_mesh_create": [_I] + [_P] * 8 + [_PP], "pk_mesh_destroy": [_PP],
+        "pk_xs_create": [_I, _I] + [_P] * 6 + [_PP], "pk_xs_destroy": [_PP],
+        "pk_diffusion_assemble": [_P, _P, _I, _P, _P, _PP, _PP],
+        "pk_normalize_flux": [_P, _P, _P, _D, _D, _P, _P, _P],
+        "pk_eig_create": [_P, _P, _PP], "pk_eig_set_tolerances": [_P, _D, _I],
+        "pk_eig_set_precision": [_P, _I], "pk_eig_set_inner": [_P, _I, _D, _I],
+        "pk_eig_set_shift": [_P, _D], "pk_eig_set_initial": [_P, _P], "pk_eig_solve": [_P],
+        "pk_eig_get_eigenpair": [_P, _P, _P], "pk_eig_get_stats": [_P, _P, _P, _P, _P],
+        "pk_eig_destroy": [_PP],
+        "pk_lin_create": [_P, _PP], "pk_lin_set_tolerances": [_P, _D, _D, _I],
+        "pk_lin_solve": [_P, _P, _P], "pk_lin_get_stats": [_P, _P, _P], "pk_lin_destroy": [_PP],
+    }
+    for name, args in sig.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _I
+    L.pk_last_error.restype = ct.c_char_p
+    L.pk_last_error.argtypes = []
+    L.pk_vec_size.restype = _L
+    L.pk_vec_size.argtypes = [_P]
+
+
+def check(rc: int):
+    if rc != 0:
+        raise PkError(lib().pk_last_error().decode())
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+class Handle:
+    """Owns one pk_* handle and destroys it with the matching function."""
+
+    def __init__(self, kind: str, h: ct.c_void_p):
+        self.kind = kind
+        self.h = h
+
+    @property
+    def value(self):
+        return self.h
+
+    def close(self):
+        if self.h is not None and self.h.value and _lib is not None:
+            getattr(_lib, f"pk_{self.kind}_destroy")(ct.byref(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _new(kind, fn, *args):
+    h = ct.c_void_p()
+    check(fn(*args, ct.byref(h)))
+    return Handle(kind, h)
+
+
+def initialize(device: int = 0):
+    check(lib().pk_initialize(device))
+
+
+class Vec(Handle):
+    def __init__(self, n: int, values=None):
+        L = lib()
+        h = ct.c_void_p()
+        check(L.pk_vec_create(n, ct.byref(h)))
+        super().__init__("vec", h)
+        self.n = n
+        if values is not None:
+            self.set(values)
+
+    def set(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        assert a.size == self.n
+        check(lib().pk_vec_set_values(self.h, ptr(a)))
+
+    def get(self) -> np.ndarray:
+        a = np.empty(self.n)
+        check(lib().pk_vec_get_values(self.h, ptr(a)))
+        return a
+
+
+class Mat(Handle):
+    def __init__(self, h: ct.c_void_p):
+        super().__init__("mat", h)
+
+    @classmethod
+    def from_csr(cls, n, row_ptr, col, val, block_size=1):
+        L = lib()
+        h = ct.c_void_p()
+        check(L.pk_mat_create(n, 0, ct.byref(h)))
+        m = cls(h)
+        row_ptr = np.asarray(row_ptr)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        for r in range(n):
+            a, b = int(row_ptr[r]), int(row_ptr[r + 1])
+            if b > a:
+                rows = np.array([r], np.int32)
+                check(L.pk_mat_set_values(h, 1, ptr(rows), b - a, ptr(col[a:b]), ptr(val[a:b]),
+                                          PK_INSERT_VALUES))
+        check(L.pk_mat_assemble(h))
+        if block_size > 1:
+            check(L.pk_mat_set_block_size(h, block_size))
+        return m
+
+    def size(self):
+        n, nnz = ct.c_long(), ct.c_long()
+        check(lib().pk_mat_get_size(self.h, ct.byref(n), ct.byref(nnz)))
+        return n.value, nnz.value
+
+    def csr(self):
+        n, nnz = self.size()
+        p = np.zeros(n + 1, np.int32)
+        c = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        check(lib().pk_mat_get_csr(self.h, ptr(p), ptr(c), ptr(v)))
+        return p, c[:nnz], v[:nnz]
+
+    def mult(self, x: Vec, y: Vec):
+        check(lib().pk_mat_mult(self.h, x.h, y.h))
+
+    def time_mult(self, x: Vec, y: Vec, reps=100, fp32=False) -> float:
+        t = ct.c_double()
+        check(lib().pk_mat_time_mult(self.h, x.h, y.h, reps, int(fp32), ct.byref(t)))
+        return t.value
+
+
+class DeviceMesh(Handle):
+    """pk_mesh from ragged face arrays (face_ptr/face data)."""
+
+    def __init__(self, face_ptr, vol, cen, mat, farea, fcen, fnorm, fnb):
+        arrs = [np.ascontiguousarray(face_ptr, np.int32), np.ascontiguousarray(vol, np.float64),
+                np.ascontiguousarray(cen, np.float64), np.ascontiguousarray(mat, np.int32),
+                np.ascontiguousarray(farea, np.float64), np.ascontiguousarray(fcen, np.float64),
+                np.ascontiguousarray(fnorm, np.float64), np.ascontiguousarray(fnb, np.int32)]
+        self._keep = arrs
+        h = ct.c_void_p()
+        check(lib().pk_mesh_create(len(arrs[1]), *[ptr(a) for a in arrs], ct.byref(h)))
+        super().__init__("mesh", h)
+        self.C = len(arrs[1])
+
+
+class DeviceXS(Handle):
+    def __init__(self, sigt, sigs, nusf, kapf, chi, D=None):
+        sigt = np.ascontiguousarray(sigt, np.float64)
+        nm, G = sigt.shape
+        arrs = [sigt, np.ascontiguousarray(sigs, np.float64), np.ascontiguousarray(nusf, np.float64),
+                np.ascontiguousarray(kapf, np.float64), np.ascontiguousarray(chi, np.float64)]
+        Dp = None if D is None else np.ascontiguousarray(D, np.float64)
+        self._keep = arrs + [Dp]
+        h = ct.c_void_p()
+        check(lib().pk_xs_create(nm, G, *[ptr(a) for a in arrs], None if Dp is None else ptr(Dp),
+                                 ct.byref(h)))
+        super().__init__("xs", h)
+        self.G = G
+
+
+def diffusion_assemble(mesh: DeviceMesh, xs: DeviceXS, bc_type, bc_value):
+    bt = np.ascontiguousarray(bc_type, np.int32)
+    bv = np.ascontiguousarray(bc_value, np.float64)
+    R, F = ct.c_void_p(), ct.c_void_p()
+    check(lib().pk_diffusion_assemble(mesh.h, xs.h, len(bt), ptr(bt), ptr(bv), ct.byref(R),
+                                      ct.byref(F)))
+    return Mat(R), Mat(F)
+
+
+class Eigen(Handle):
+    def __init__(self, R: Mat, F: Mat, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=30,
+                 reduction=0.1, max_inner=2000, max_outer=20000):
+        L = lib()
+        h = ct.c_void_p()
+        check(L.pk_eig_create(R.h, F.h, ct.byref(h)))
+        super().__init__("eig", h)
+        self._mats = (R, F)
+        check(L.pk_eig_set_tolerances(h, tol, max_outer))
+        check(L.pk_eig_set_precision(h, precision))
+        check(L.pk_eig_set_inner(h, restart, reduction, max_inner))
+        check(L.pk_eig_set_shift(h, shift))
+
+    def solve(self, x0: Vec | None = None):
+        L = lib()
+        if x0 is not None:
+            check(L.pk_eig_set_initial(self.h, x0.h))
+        check(L.pk_eig_solve(self.h))
+
+    def eigenpair(self, n):
+        lam = ct.c_double()
+        x = Vec(n)
+        check(lib().pk_eig_get_eigenpair(self.h, ct.byref(lam), x.h))
+        return 1.0 / lam.value, x
+
+    def stats(self):
+        o, i, r, s = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double()
+        check(lib().pk_eig_get_stats(self.h, ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s)))
+        return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)

@@ -1,0 +1,494 @@
+// Device assembly of the finite-volume diffusion operators R and F from the
+// face connectivity, and on-device power normalisation.
+//
+// The pattern is built on the device in closed form: row (i, g) holds the G
+// columns (i, g2) of the cell block and one column (i2, g) per distinct
+// physical neighbour, sorted by column; explicit zeros are kept.  That is the
+// pattern PETSc's AIJ storage ends up with for the reference's
+// MatSetValues(INSERT_VALUES) rows (DiffusionSolver.cxx:293-296).  Values are
+// computed with the reference's operation order and without FMA contraction,
+// so FP64 entries can be compared bit-for-bit.
+#include <hipcub/hipcub.hpp>
+
+#include "pk_internal.hpp"
+#include "pk_reduce.hpp"
+
+namespace pk {
+
+// surface leakage factor w = (p2-p1).n / |p2-p1|^2 (math.cxx:150-161)
+__device__ __forceinline__ double slf(const double* p1, const double* p2, const double* n) {
+#pragma clang fp contract(off)
+   double d0 = p2[0] - p1[0], d1 = p2[1] - p1[1], d2 = p2[2] - p1[2];
+   double num = 0.0;
+   num += d0 * n[0];
+   num += d1 * n[1];
+   num += d2 * n[2];
+   double den = 0.0;
+   den += d0 * d0;
+   den += d1 * d1;
+   den += d2 * d2;
+   return num / den;
+}
+
+struct MeshView {
+   int C;
+   const int* fptr;
+   const double* vol;
+   const double* cen;
+   const int* mat;
+   const double* farea;
+   const double* fcen;
+   const double* fnorm;
+   const int* fnb;
+};
+
+struct XSView {
+   int G;
+   const double* sigt;
+   const double* sigs;
+   const double* nusf;
+   const double* kapf;
+   const double* chi;
+   const double* D;
+};
+
+MeshView view(pk_mesh_s* m) {
+   return MeshView{m->C, m->fptr.p, m->vol.p, m->cen.p, m->mat.p, m->farea.p, m->fcen.p, m->fnorm.p,
+                   m->fnb.p};
+}
+XSView view(pk_xs_s* x) {
+   return XSView{x->G, x->sigt.p, x->sigs.p, x->nusf.p, x->kapf.p, x->chi.p, x->D.p};
+}
+
+// Distinct physical neighbours of a cell in face order (first occurrence).
+__device__ __forceinline__ bool first_occurrence(const int* fnb, int f0, int f) {
+   int i2 = fnb[f];
+   for (int f2 = f0; f2 < f; f2++)
+      if (fnb[f2] == i2) return false;
+   return true;
+}
+
+__global__ void diff_rowlen_kernel(MeshView M, int G, int* rlen) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= M.C) return;
+   int u = 0;
+   for (int f = M.fptr[i]; f < M.fptr[i + 1]; f++)
+      if (M.fnb[f] >= 0 && first_occurrence(M.fnb, M.fptr[i], f)) u++;
+   for (int g = 0; g < G; g++) rlen[(long)i * G + g] = G + u;
+}
+
+constexpr int kMaxFaces = 32;
+
+// One thread per row (i, g): values of R (and F when fcol != nullptr).
+__global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const double* bcv,
+                                 const int* rptr, int* rcol, double* rval, int* fcol,
+                                 double* fval, const double* vel, double dt, const double* chip,
+                                 const double* beta, double keff, int* err) {
+#pragma clang fp contract(off)
+   long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   const int G = X.G;
+   if (l >= (long)M.C * G) return;
+   int i = (int)(l / G), g = (int)(l % G);
+   int m = M.mat[i];
+   double V = M.vol[i];
+   const double* ci = M.cen + 3 * (long)i;
+   // diagonal: total reaction, then self-scattering (DiffusionSolver.cxx:130-131,157-158)
+   double diag = X.sigt[m * G + g] * V;
+   if (vel) diag += V / (vel[m * G + g] * dt);
+   // neighbour entries in face order; a repeated neighbour keeps the last value
+   int ncol[kMaxFaces];
+   double nval[kMaxFaces];
+   int nn = 0;
+   // scattering / prompt-fission off-diagonals of the cell block come later;
+   // the diagonal receives the self terms in g2 order
+   for (int g2 = 0; g2 < G; g2++) {
+      if (g2 == g) {
+         diag += -X.sigs[(m * G + g2) * G + g] * V;
+         if (vel) diag += -(1.0 - beta[m]) * chip[m * G + g] * X.nusf[m * G + g2] * V / keff;
+      }
+   }
+   for (int f = M.fptr[i]; f < M.fptr[i + 1]; f++) {
+      int i2 = M.fnb[f];
+      double A = M.farea[f];
+      if (i2 < 0) {
+         int id = -i2;
+         int t = bct[id];
+         if (t == PK_BC_VACUUM) {
+            double w = slf(ci, M.fcen + 3 * (long)f, M.fnorm + 3 * (long)f);
+            diag += w * X.D[m * G + g] * A;
+         } else if (t == PK_BC_REFLECTIVE) {
+         } else if (t == PK_BC_ROBIN) {
+            diag -= bcv[id] * A;
+         } else {
+            *err = 1;
+         }
+      } else {
+         int m2 = M.mat[i2];
+         double r;
+         if (m2 == m) {
+            double w = slf(ci, M.cen + 3 * (long)i2, M.fnorm + 3 * (long)f);
+            r = -w * X.D[m * G + g] * A;
+         } else {
+            double w1 = slf(ci, M.fcen + 3 * (long)f, M.fnorm + 3 * (long)f);
+            w1 *= X.D[m * G + g] * A;
+            double w2 = slf(M.cen + 3 * (long)i2, M.fcen + 3 * (long)f, M.fnorm + 3 * (long)f);
+            w2 *= -X.D[m2 * G + g] * A;
+            r = -(w1 * w2) / (w1 + w2);
+         }
+         diag -= r;
+         int k;
+         for (k = 0; k < nn; k++)
+            if (ncol[k] == i2) break;
+         if (k == nn) {
+            if (nn < kMaxFaces) {
+               ncol[nn] = i2;
+               nval[nn] = r;
+               nn++;
+            } else {
+               *err = 2;
+            }
+         } else {
+            nval[k] = r;
+         }
+      }
+   }
+   // sort neighbours by cell index (insertion sort, nn <= faces of the cell)
+   for (int a = 1; a < nn; a++) {
+      int c = ncol[a];
+      double v = nval[a];
+      int b = a - 1;
+      while (b >= 0 && ncol[b] > c) {
+         ncol[b + 1] = ncol[b];
+         nval[b + 1] = nval[b];
+         b--;
+      }
+      ncol[b + 1] = c;
+      nval[b + 1] = v;
+   }
+   long p = rptr[l];
+   int a = 0;
+   for (; a < nn && ncol[a] < i; a++, p++) {
+      rcol[p] = ncol[a] * G + g;
+      rval[p] = nval[a];
+   }
+   for (int g2 = 0; g2 < G; g2++, p++) {
+      rcol[p] = i * G + g2;
+      if (g2 == g) {
+         rval[p] = diag;
+      } else {
+         double v = -X.sigs[(m * G + g2) * G + g] * V;
+         if (vel) v += -(1.0 - beta[m]) * chip[m * G + g] * X.nusf[m * G + g2] * V / keff;
+         rval[p] = v;
+      }
+   }
+   for (; a < nn; a++, p++) {
+      rcol[p] = ncol[a] * G + g;
+      rval[p] = nval[a];
+   }
+   if (fcol) {
+      long q = l * G;
+      for (int g2 = 0; g2 < G; g2++) {
+         fcol[q + g2] = i * G + g2;
+         fval[q + g2] = X.chi[m * G + g] * X.nusf[m * G + g2] * V;
+      }
+   }
+}
+
+// b = chi_d S + V/(v dt) phi0 (DiffusionSolver.cxx:134-148)
+__global__ void diff_rhs_kernel(MeshView M, int G, const double* chid, const double* vel, double dt,
+                                const double* S, const double* phi0, double* b) {
+#pragma clang fp contract(off)
+   long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (l >= (long)M.C * G) return;
+   int i = (int)(l / G), g = (int)(l % G);
+   int m = M.mat[i];
+   double v = chid[m * G + g] * S[i];
+   double d = M.vol[i] / (vel[m * G + g] * dt);
+   v += d * phi0[l];
+   b[l] = v;
+}
+
+int diffusion_pattern(pk_mesh_s* mesh, int G, DBuf<int>& ptr, long& nnz) {
+   Ctx& c = ctx();
+   long N = (long)mesh->C * G;
+   DBuf<int> rlen;
+   PK_REQUIRE(!rlen.alloc(N + 1) && !ptr.alloc(N + 1), "device allocation failed");
+   diff_rowlen_kernel<<<grid_for(mesh->C), kBlock, 0, c.stream>>>(view(mesh), G, rlen.p);
+   PK_HIP(hipGetLastError());
+   PK_HIP(hipMemsetAsync(rlen.p + N, 0, sizeof(int), c.stream));
+   size_t tmp_bytes = 0;
+   PK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, rlen.p, ptr.p, (int)(N + 1), c.stream));
+   DBuf<char> tmp;
+   PK_REQUIRE(!tmp.alloc(tmp_bytes ? tmp_bytes : 1), "device allocation failed");
+   PK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, rlen.p, ptr.p, (int)(N + 1), c.stream));
+   int h = 0;
+   PK_HIP(hipMemcpyAsync(&h, ptr.p + N, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   nnz = h;
+   tmp.free();
+   rlen.free();
+   return 0;
+}
+
+__global__ void seq_ptr_kernel(int* ptr, long n, int w) {
+   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i <= n) ptr[i] = (int)(i * w);
+}
+
+// p0 = sum phi kapf V (NeutronicSolver.cxx:55-60)
+__global__ void __launch_bounds__(kBlock) power_kernel(MeshView M, XSView X, const double* phi,
+                                                        double* partials, unsigned* ticket,
+                                                        double* out) {
+   double s = 0.0;
+   const int G = X.G;
+   for (long l = blockIdx.x * (long)kBlock + threadIdx.x; l < (long)M.C * G;
+        l += (long)gridDim.x * kBlock) {
+      int i = (int)(l / G), g = (int)(l % G);
+      s += phi[l] * X.kapf[M.mat[i] * G + g] * M.vol[i];
+   }
+   double v[1] = {s};
+   block_reduce_finish<1>(v, partials, ticket, out);
+}
+
+// phi *= f; q_i, P_i; count of negative fluxes (NeutronicSolver.cxx:63-68,88-104)
+__global__ void __launch_bounds__(kBlock) scale_power_kernel(MeshView M, XSView X, double* phi,
+                                                              const double* p0, double power,
+                                                              double inv_keff, double* q,
+                                                              double* P, double* partials,
+                                                              unsigned* ticket, double* out) {
+   const int G = X.G;
+   double f = power / p0[0];
+   double neg = 0.0, tot = 0.0;
+   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < M.C; i += (long)gridDim.x * kBlock) {
+      int m = M.mat[i];
+      double qi = 0.0, Pi = 0.0;
+      for (int g = 0; g < G; g++) {
+         double v = phi[i * G + g] * f;
+         phi[i * G + g] = v;
+         if (v < 0.0) neg += 1.0;
+         qi += v * X.kapf[m * G + g] * M.vol[i];
+         Pi += v * X.nusf[m * G + g] * M.vol[i] * inv_keff;
+      }
+      if (q) q[i] = qi;
+      if (P) P[i] = Pi;
+      tot += qi;
+   }
+   double v[2] = {neg, tot};
+   block_reduce_finish<2>(v, partials, ticket, out);
+}
+
+}  // namespace pk
+
+using namespace pk;
+
+template <class T>
+static int upload(DBuf<T>& d, const T* h, size_t n) {
+   if (d.alloc(n ? n : 1)) return 1;
+   if (n && hipMemcpy(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess) return 1;
+   return 0;
+}
+
+extern "C" {
+
+int pk_mesh_create(int num_cells, const int* face_ptr, const double* vol, const double* cen,
+                   const int* mat, const double* farea, const double* fcen, const double* fnorm,
+                   const int* fnb, pk_mesh* mesh) {
+   PK_CALL(ensure_init());
+   PK_REQUIRE(num_cells >= 0 && face_ptr, "wrong mesh arguments");
+   auto* m = new pk_mesh_s();
+   m->C = num_cells;
+   m->F = face_ptr[num_cells];
+   int fmax = 0;
+   for (int i = 0; i < num_cells; i++) fmax = std::max(fmax, face_ptr[i + 1] - face_ptr[i]);
+   m->fmax = fmax;
+   if (fmax > kMaxFaces) {
+      delete m;
+      PK_FAIL("too many faces per cell");
+   }
+   m->h_fptr.assign(face_ptr, face_ptr + num_cells + 1);
+   m->h_vol.assign(vol, vol + num_cells);
+   m->h_cen.assign(cen, cen + 3L * num_cells);
+   m->h_mat.assign(mat, mat + num_cells);
+   m->h_farea.assign(farea, farea + m->F);
+   m->h_fcen.assign(fcen, fcen + 3L * m->F);
+   m->h_fnorm.assign(fnorm, fnorm + 3L * m->F);
+   m->h_fnb.assign(fnb, fnb + m->F);
+   int e = 0;
+   e |= upload(m->fptr, face_ptr, num_cells + 1);
+   e |= upload(m->vol, vol, num_cells);
+   e |= upload(m->cen, cen, 3L * num_cells);
+   e |= upload(m->mat, mat, num_cells);
+   e |= upload(m->farea, farea, m->F);
+   e |= upload(m->fcen, fcen, 3L * m->F);
+   e |= upload(m->fnorm, fnorm, 3L * m->F);
+   e |= upload(m->fnb, fnb, m->F);
+   if (e) {
+      pk_mesh_destroy(&m);
+      PK_FAIL("device allocation failed");
+   }
+   *mesh = m;
+   return 0;
+}
+
+int pk_mesh_destroy(pk_mesh* mesh) {
+   if (!mesh || !*mesh) return 0;
+   pk_mesh_s* m = *mesh;
+   m->fptr.free();
+   m->vol.free();
+   m->cen.free();
+   m->mat.free();
+   m->farea.free();
+   m->fcen.free();
+   m->fnorm.free();
+   m->fnb.free();
+   delete m;
+   *mesh = nullptr;
+   return 0;
+}
+
+int pk_xs_create(int num_mat, int G, const double* sigt, const double* sigs, const double* nusf,
+                 const double* kapf, const double* chi, const double* D, pk_xs* xs) {
+   PK_CALL(ensure_init());
+   PK_REQUIRE(num_mat > 0 && G > 0, "wrong cross-section table size");
+   auto* x = new pk_xs_s();
+   x->nm = num_mat;
+   x->G = G;
+   size_t n1 = (size_t)num_mat * G, n2 = n1 * G;
+   x->h_sigt.assign(sigt, sigt + n1);
+   x->h_sigs.assign(sigs, sigs + n2);
+   x->h_nusf.assign(nusf, nusf + n1);
+   x->h_kapf.assign(kapf, kapf + n1);
+   x->h_chi.assign(chi, chi + n1);
+   if (D)
+      x->h_D.assign(D, D + n1);
+   else
+      x->h_D.assign(n1, 0.0);
+   int e = 0;
+   e |= upload(x->sigt, x->h_sigt.data(), n1);
+   e |= upload(x->sigs, x->h_sigs.data(), n2);
+   e |= upload(x->nusf, x->h_nusf.data(), n1);
+   e |= upload(x->kapf, x->h_kapf.data(), n1);
+   e |= upload(x->chi, x->h_chi.data(), n1);
+   e |= upload(x->D, x->h_D.data(), n1);
+   if (e) {
+      pk_xs_destroy(&x);
+      PK_FAIL("device allocation failed");
+   }
+   *xs = x;
+   return 0;
+}
+
+int pk_xs_destroy(pk_xs* xs) {
+   if (!xs || !*xs) return 0;
+   pk_xs_s* x = *xs;
+   x->sigt.free();
+   x->sigs.free();
+   x->nusf.free();
+   x->kapf.free();
+   x->chi.free();
+   x->D.free();
+   delete x;
+   *xs = nullptr;
+   return 0;
+}
+
+static int diffusion_assemble_impl(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
+                                   const double* bc_value, const double* vel, double dt,
+                                   const double* chip, const double* beta, double keff,
+                                   pk_mat* R, pk_mat* F) {
+   PK_REQUIRE(mesh && xs && R, "null argument");
+   Ctx& c = ctx();
+   const int G = xs->G;
+   long N = (long)mesh->C * G;
+   PK_REQUIRE(N < (1L << 31), "operator too large for 32-bit indices");
+   DBuf<int> dbct, derr;
+   DBuf<double> dbcv, dvdt, dchip, dbeta;
+   PK_REQUIRE(!upload(dbct, bc_type, num_bc) && !upload(dbcv, bc_value, num_bc) && !derr.alloc(1),
+              "device allocation failed");
+   if (vel) {
+      PK_REQUIRE(!upload(dvdt, vel, (size_t)xs->nm * G) && !upload(dchip, chip, (size_t)xs->nm * G) &&
+                     !upload(dbeta, beta, (size_t)xs->nm),
+                 "device allocation failed");
+   }
+   PK_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), c.stream));
+   pk_mat_s* r = new pk_mat_s();
+   r->n = N;
+   r->bs = G;
+   PK_CALL(diffusion_pattern(mesh, G, r->ptr, r->nnz));
+   PK_REQUIRE(!r->col.alloc(r->nnz) && !r->val.alloc(r->nnz), "device allocation failed");
+   pk_mat_s* f = nullptr;
+   if (F) {
+      f = new pk_mat_s();
+      f->n = N;
+      f->bs = G;
+      f->nnz = N * G;
+      PK_REQUIRE(!f->ptr.alloc(N + 1) && !f->col.alloc(f->nnz) && !f->val.alloc(f->nnz),
+                 "device allocation failed");
+      seq_ptr_kernel<<<grid_for(N + 1), kBlock, 0, c.stream>>>(f->ptr.p, N, G);
+   }
+   diff_fill_kernel<<<grid_for(N), kBlock, 0, c.stream>>>(
+       view(mesh), view(xs), dbct.p, dbcv.p, r->ptr.p, r->col.p, r->val.p, f ? f->col.p : nullptr,
+       f ? f->val.p : nullptr, vel ? dvdt.p : nullptr, dt, dchip.p, dbeta.p, keff, derr.p);
+   PK_HIP(hipGetLastError());
+   int herr = 0;
+   PK_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   if (herr) {
+      pk_mat_destroy(&r);
+      if (f) pk_mat_destroy(&f);
+      PK_FAIL(herr == 1 ? "boundary condition not implemented" : "too many neighbours per cell");
+   }
+   PK_CALL(finalize_device_csr(r));
+   if (f) PK_CALL(finalize_device_csr(f));
+   *R = r;
+   if (F) *F = f;
+   return 0;
+}
+
+int pk_diffusion_assemble(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
+                          const double* bc_value, pk_mat* R, pk_mat* F) {
+   return diffusion_assemble_impl(mesh, xs, num_bc, bc_type, bc_value, nullptr, 0.0, nullptr,
+                                  nullptr, 1.0, R, F);
+}
+
+int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
+                                    const double* bc_value, const double* velocity, double dt,
+                                    const double* chip, const double* chid, const double* beta,
+                                    double keff, pk_vec S, pk_vec phi0, pk_mat* R, pk_vec b) {
+   PK_REQUIRE(velocity && chip && chid && beta && S && phi0 && b, "null argument");
+   PK_CALL(diffusion_assemble_impl(mesh, xs, num_bc, bc_type, bc_value, velocity, dt, chip, beta,
+                                   keff, R, nullptr));
+   Ctx& c = ctx();
+   const int G = xs->G;
+   DBuf<double> dchid, dvdt;
+   PK_REQUIRE(!upload(dchid, chid, (size_t)xs->nm * G) && !upload(dvdt, velocity, (size_t)xs->nm * G),
+              "device allocation failed");
+   diff_rhs_kernel<<<grid_for((long)mesh->C * G), kBlock, 0, c.stream>>>(
+       view(mesh), G, dchid.p, dvdt.p, dt, S->d.p, phi0->d.p, b->d.p);
+   PK_HIP(hipGetLastError());
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
+}
+
+int pk_normalize_flux(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double keff, pk_vec q,
+                      pk_vec P, double* total_power) {
+   PK_REQUIRE(mesh && xs && phi && phi->n == (long)mesh->C * xs->G, "wrong arguments");
+   Ctx& c = ctx();
+   int g = grid_for((long)mesh->C * xs->G, kBlock, kMaxRedBlocks);
+   power_kernel<<<g, kBlock, 0, c.stream>>>(view(mesh), view(xs), phi->d.p, c.partials, c.ticket,
+                                            c.scalars);
+   PK_HIP(hipGetLastError());
+   int g2 = grid_for(mesh->C, kBlock, kMaxRedBlocks);
+   scale_power_kernel<<<g2, kBlock, 0, c.stream>>>(view(mesh), view(xs), phi->d.p, c.scalars, power,
+                                                   1.0 / keff, q ? q->d.p : nullptr,
+                                                   P ? P->d.p : nullptr, c.partials, c.ticket,
+                                                   c.scalars + 1);
+   PK_HIP(hipGetLastError());
+   PK_HIP(hipMemcpyAsync(c.host_scalars, c.scalars, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   if (total_power) *total_power = c.host_scalars[2];
+   PK_REQUIRE(c.host_scalars[1] == 0.0, "negative values in the scalar-flux solution");
+   return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Internal declarations of the pk_* numerics library (HIP, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pk.h"
+
+namespace pk {
+
+// ---- errors ---------------------------------------------------------------
+// Every failure records "file:line:function(): error: message." (the reference's
+// convention, utils.hxx:41-47) and the entry point returns a positive code.
+void set_error(const char* file, int line, const char* func, const std::string& msg);
+
+#define PK_FAIL(msg)                                                  \
+   do {                                                              \
+      ::pk::set_error(__FILE__, __LINE__, __func__, (msg));          \
+      return 1;                                                      \
+   } while (0)
+
+#define PK_REQUIRE(cond, msg) \
+   do {                       \
+      if (!(cond)) PK_FAIL(msg); \
+   } while (0)
+
+#define PK_HIP(call)                                                                   \
+   do {                                                                                \
+      hipError_t e_ = (call);                                                          \
+      if (e_ != hipSuccess) PK_FAIL(std::string(#call) + ": " + hipGetErrorString(e_)); \
+   } while (0)
+
+#define PK_CALL(call)      \
+   do {                    \
+      if ((call) != 0) return 1; \
+   } while (0)
+
+// ---- device context --------------------------------------------------------
+constexpr int kBlock = 256;          // threads per workgroup for 1-D kernels
+constexpr int kMaxRedBlocks = 1024;  // workgroups of a reduction kernel
+constexpr int kMaxRedVals = 80;      // values reduced together (GMRES restart + 2)
+
+struct Ctx {
+   int device = -1;
+   hipStream_t stream = nullptr;
+   double* partials = nullptr;   // [kMaxRedBlocks * kMaxRedVals]
+   unsigned* ticket = nullptr;   // last-block-done counters
+   double* scalars = nullptr;    // small device-resident scalars [256]
+   double* host_scalars = nullptr;  // pinned mirror
+   int num_cus = 0;
+   size_t partials_cap = 0;      // doubles available in `partials`
+};
+
+Ctx& ctx();
+int ensure_init();
+// Grow the reduction scratch to at least `count` doubles (setup time only).
+int reserve_partials(size_t count);
+
+inline int grid_for(long n, int block = kBlock, int cap = 1 << 30) {
+   long g = (n + block - 1) / block;
+   if (g < 1) g = 1;
+   if (g > cap) g = cap;
+   return (int)g;
+}
+
+// Device buffer owned by a handle.
+template <class T>
+struct DBuf {
+   T* p = nullptr;
+   size_t n = 0;
+   int alloc(size_t count) {
+      free();
+      n = count;
+      if (count == 0) return 0;
+      if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+         p = nullptr;
+         n = 0;
+         return 1;
+      }
+      return 0;
+   }
+   void free() {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      n = 0;
+   }
+};
+
+// ---- deterministic reductions ----------------------------------------------
+// Sum over n of f(i) for `nv` values at once; result written to out[0..nv-1]
+// (device memory).  Fixed work split and fixed summation order, so the result
+// does not depend on timing or on what else runs on the device.
+int reduce_dot(const double* x, const double* y, long n, double* out_dev);
+int reduce_dot_f(const float* x, const float* y, long n, double* out_dev);
+
+}  // namespace pk
+
+// Sliced-ELL (slice height 64 = one wavefront) device copy of a CSR matrix.
+struct SellData {
+   long nslices = 0, padded = 0;
+   pk::DBuf<long> off;
+   pk::DBuf<int> width, col;
+   pk::DBuf<int> rlen;  // unpadded row lengths (padding sits at k >= rlen[row])
+   pk::DBuf<double> val;
+   pk::DBuf<float> val32;
+};
+
+// Handle layouts shared across translation units.
+struct pk_vec_s {
+   long n = 0;
+   pk::DBuf<double> d;
+};
+
+struct pk_mat_s {
+   long n = 0;
+   int prealloc = 0;
+   int bs = 1;  // cell block size (energy groups)
+   // host staging of MatSetValues rows: (col, val, seq) triples per row
+   std::vector<std::vector<std::pair<int, double>>> stage;
+   std::vector<std::vector<int>> stage_mode;
+   bool assembled = false;
+   long nnz = 0;
+   pk::DBuf<int> ptr, col;
+   pk::DBuf<double> val;
+   SellData sell;  // SpMV copy; FP32 values built on demand for mixed precision
+   int ensure_fp32();
+};
+
+struct pk_mesh_s {
+   int C = 0;
+   int F = 0;  // total faces
+   int fmax = 0;
+   pk::DBuf<int> fptr, mat, fnb;
+   pk::DBuf<double> vol, cen, farea, fcen, fnorm;
+   // host copies (schedules and host-side setup)
+   std::vector<int> h_fptr, h_mat, h_fnb;
+   std::vector<double> h_vol, h_cen, h_farea, h_fcen, h_fnorm;
+};
+
+struct pk_xs_s {
+   int nm = 0, G = 0;
+   pk::DBuf<double> sigt, sigs, nusf, kapf, chi, D;
+   std::vector<double> h_sigt, h_sigs, h_nusf, h_kapf, h_chi, h_D;
+};
+
+// SpMV entry points used across files.
+namespace pk {
+int spmv(pk_mat_s* A, const double* x, double* y);
+int spmv_f(pk_mat_s* A, const float* x, float* y);
+int build_sell(pk_mat_s* A);
+int finalize_device_csr(pk_mat_s* A);
+}  // namespace pk

@@ -1,0 +1,302 @@
+// Sparse matrices of the pk_* layer: PETSc-AIJ row semantics on the host side
+// of MatSetValues, CSR + sliced-ELL storage on the device, and SpMV kernels.
+//
+// Device formats
+//   CSR        row_ptr/col/val, the exported pattern (bit-exact parity object).
+//   SELL-64    slices of 64 consecutive rows (one wavefront), each slice padded
+//              to its longest row and stored column-major inside the slice, so
+//              lane r of a wavefront reads element k of row (64 s + r) at
+//              sell_off[s] + 64 k + r: every load of col/val is one contiguous
+//              256-byte (FP32) or 512-byte (FP64) access.  Padding entries carry
+//              column = row and value 0 and are not counted as work.
+#include <algorithm>
+#include <numeric>
+
+#include "pk_internal.hpp"
+#include "pk_reduce.hpp"
+
+namespace pk {
+
+constexpr int kSlice = 64;
+
+int build_sell(pk_mat_s* A) {
+   // host pattern -> SELL-64
+   long n = A->n;
+   std::vector<int> hptr(n + 1), hcol(A->nnz);
+   std::vector<double> hval(A->nnz);
+   PK_HIP(hipMemcpy(hptr.data(), A->ptr.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+   if (A->nnz) {
+      PK_HIP(hipMemcpy(hcol.data(), A->col.p, sizeof(int) * A->nnz, hipMemcpyDeviceToHost));
+      PK_HIP(hipMemcpy(hval.data(), A->val.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
+   }
+   long ns = (n + kSlice - 1) / kSlice;
+   std::vector<long> off(ns + 1, 0);
+   std::vector<int> width(ns, 0);
+   for (long s = 0; s < ns; s++) {
+      int w = 0;
+      for (long r = s * kSlice; r < std::min(n, (s + 1) * kSlice); r++) w = std::max(w, hptr[r + 1] - hptr[r]);
+      width[s] = w;
+      off[s + 1] = off[s] + (long)w * kSlice;
+   }
+   long tot = off[ns];
+   std::vector<int> scol(tot);
+   std::vector<double> sval(tot, 0.0);
+   for (long s = 0; s < ns; s++)
+      for (int r = 0; r < kSlice; r++) {
+         long row = s * kSlice + r;
+         for (int k = 0; k < width[s]; k++) {
+            long p = off[s] + (long)k * kSlice + r;
+            if (row < n && k < hptr[row + 1] - hptr[row]) {
+               scol[p] = hcol[hptr[row] + k];
+               sval[p] = hval[hptr[row] + k];
+            } else {
+               scol[p] = (int)(row < n ? row : 0);
+               sval[p] = 0.0;
+            }
+         }
+      }
+   SellData& S = A->sell;
+   S.nslices = ns;
+   S.padded = tot;
+   PK_REQUIRE(!S.off.alloc(ns + 1) && !S.width.alloc(ns) && !S.col.alloc(tot ? tot : 1) &&
+                  !S.val.alloc(tot ? tot : 1) && !S.rlen.alloc(n ? n : 1),
+              "device allocation failed");
+   std::vector<int> rl(n);
+   for (long r = 0; r < n; r++) rl[r] = hptr[r + 1] - hptr[r];
+   if (n) PK_HIP(hipMemcpy(S.rlen.p, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+   PK_HIP(hipMemcpy(S.off.p, off.data(), sizeof(long) * (ns + 1), hipMemcpyHostToDevice));
+   PK_HIP(hipMemcpy(S.width.p, width.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
+   if (tot) {
+      PK_HIP(hipMemcpy(S.col.p, scol.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
+      PK_HIP(hipMemcpy(S.val.p, sval.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+   }
+   S.val32.free();
+   return 0;
+}
+
+// ---- SpMV kernels ------------------------------------------------------------
+
+template <class T>
+__global__ void __launch_bounds__(256) sell_spmv_kernel(long n, const long* __restrict__ off,
+                                                        const int* __restrict__ width,
+                                                        const int* __restrict__ col,
+                                                        const T* __restrict__ val,
+                                                        const T* __restrict__ x, T* __restrict__ y) {
+   long row = blockIdx.x * 256L + threadIdx.x;
+   long s = row >> 6;
+   int r = threadIdx.x & 63;
+   if (row >= n) return;
+   long base = off[s] + r;
+   int w = width[s];
+   T acc = 0;
+   for (int k = 0; k < w; k++) {
+      long p = base + (long)k * kSlice;
+      acc += val[p] * x[col[p]];
+   }
+   y[row] = acc;
+}
+
+__global__ void to_f32_kernel(const double* a, float* b, long n) {
+   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) b[i] = (float)a[i];
+}
+
+int spmv(pk_mat_s* A, const double* x, double* y) {
+   if (A->n == 0) return 0;
+   SellData& S = A->sell;
+   sell_spmv_kernel<double><<<grid_for(A->n, 256), 256, 0, ctx().stream>>>(
+       A->n, S.off.p, S.width.p, S.col.p, S.val.p, x, y);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int spmv_f(pk_mat_s* A, const float* x, float* y) {
+   if (A->n == 0) return 0;
+   PK_CALL(A->ensure_fp32());
+   SellData& S = A->sell;
+   sell_spmv_kernel<float><<<grid_for(A->n, 256), 256, 0, ctx().stream>>>(
+       A->n, S.off.p, S.width.p, S.col.p, S.val32.p, x, y);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int finalize_device_csr(pk_mat_s* A) {
+   PK_CALL(build_sell(A));
+   A->assembled = true;
+   return 0;
+}
+
+}  // namespace pk
+
+int pk_mat_s::ensure_fp32() {
+   if (sell.val32.p) return 0;
+   long tot = sell.padded ? sell.padded : 1;
+   if (sell.val32.alloc(tot)) PK_FAIL("device allocation failed");
+   pk::to_f32_kernel<<<pk::grid_for(tot), pk::kBlock, 0, pk::ctx().stream>>>(sell.val.p, sell.val32.p,
+                                                                           sell.padded);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+using namespace pk;
+
+extern "C" {
+
+int pk_mat_create(long n, int prealloc, pk_mat* M) {
+   PK_CALL(ensure_init());
+   PK_REQUIRE(n >= 0, "negative matrix size");
+   auto* A = new pk_mat_s();
+   A->n = n;
+   A->prealloc = prealloc;
+   A->stage.resize(n);
+   A->stage_mode.resize(n);
+   *M = A;
+   return 0;
+}
+
+int pk_mat_set_values(pk_mat M, int nrows, const int* rows, int ncols, const int* cols,
+                      const double* vals, int mode) {
+   PK_REQUIRE(M, "null matrix");
+   PK_REQUIRE(!M->assembled, "matrix already assembled");
+   PK_REQUIRE(mode == PK_INSERT_VALUES || mode == PK_ADD_VALUES, "wrong insert mode");
+   for (int a = 0; a < nrows; a++) {
+      int r = rows[a];
+      PK_REQUIRE(r >= 0 && r < M->n, "row index out of range");
+      for (int b = 0; b < ncols; b++) {
+         int c = cols[b];
+         PK_REQUIRE(c >= 0 && c < M->n, "column index out of range");
+         M->stage[r].push_back({c, vals[(long)a * ncols + b]});
+         M->stage_mode[r].push_back(mode);
+      }
+   }
+   return 0;
+}
+
+int pk_mat_assemble(pk_mat M) {
+   PK_REQUIRE(M, "null matrix");
+   if (M->assembled) return 0;
+   long n = M->n;
+   std::vector<int> ptr(n + 1, 0);
+   std::vector<int> col;
+   std::vector<double> val;
+   for (long r = 0; r < n; r++) {
+      auto& st = M->stage[r];
+      auto& md = M->stage_mode[r];
+      std::vector<int> idx(st.size());
+      std::iota(idx.begin(), idx.end(), 0);
+      std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return st[a].first < st[b].first; });
+      for (size_t k = 0; k < idx.size(); k++) {
+         const auto& e = st[idx[k]];
+         if (!col.empty() && (long)col.size() > ptr[r] && col.back() == e.first) {
+            if (md[idx[k]] == PK_ADD_VALUES)
+               val.back() += e.second;
+            else
+               val.back() = e.second;
+         } else {
+            col.push_back(e.first);
+            val.push_back(e.second);
+         }
+      }
+      ptr[r + 1] = (int)col.size();
+      std::vector<std::pair<int, double>>().swap(st);
+      std::vector<int>().swap(md);
+   }
+   M->nnz = (long)col.size();
+   PK_REQUIRE(!M->ptr.alloc(n + 1) && !M->col.alloc(M->nnz ? M->nnz : 1) &&
+                  !M->val.alloc(M->nnz ? M->nnz : 1),
+              "device allocation failed");
+   PK_HIP(hipMemcpy(M->ptr.p, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+   if (M->nnz) {
+      PK_HIP(hipMemcpy(M->col.p, col.data(), sizeof(int) * M->nnz, hipMemcpyHostToDevice));
+      PK_HIP(hipMemcpy(M->val.p, val.data(), sizeof(double) * M->nnz, hipMemcpyHostToDevice));
+   }
+   return finalize_device_csr(M);
+}
+
+int pk_mat_destroy(pk_mat* M) {
+   if (!M || !*M) return 0;
+   (void)hipStreamSynchronize(ctx().stream);
+   pk_mat_s* A = *M;
+   A->ptr.free();
+   A->col.free();
+   A->val.free();
+   A->sell.off.free();
+   A->sell.width.free();
+   A->sell.col.free();
+   A->sell.rlen.free();
+   A->sell.val.free();
+   A->sell.val32.free();
+   delete A;
+   *M = nullptr;
+   return 0;
+}
+
+int pk_mat_get_size(pk_mat M, long* n, long* nnz) {
+   PK_REQUIRE(M, "null matrix");
+   if (n) *n = M->n;
+   if (nnz) *nnz = M->nnz;
+   return 0;
+}
+
+int pk_mat_get_csr(pk_mat M, int* row_ptr, int* col_idx, double* val) {
+   PK_REQUIRE(M && M->assembled, "matrix not assembled");
+   PK_HIP(hipStreamSynchronize(ctx().stream));
+   if (row_ptr) PK_HIP(hipMemcpy(row_ptr, M->ptr.p, sizeof(int) * (M->n + 1), hipMemcpyDeviceToHost));
+   if (M->nnz) {
+      if (col_idx) PK_HIP(hipMemcpy(col_idx, M->col.p, sizeof(int) * M->nnz, hipMemcpyDeviceToHost));
+      if (val) PK_HIP(hipMemcpy(val, M->val.p, sizeof(double) * M->nnz, hipMemcpyDeviceToHost));
+   }
+   return 0;
+}
+
+int pk_mat_mult(pk_mat M, pk_vec x, pk_vec y) {
+   PK_REQUIRE(M && M->assembled, "matrix not assembled");
+   PK_REQUIRE(x && y && x->n == M->n && y->n == M->n, "vector size mismatch");
+   return spmv(M, x->d.p, y->d.p);
+}
+
+int pk_mat_set_block_size(pk_mat M, int bs) {
+   PK_REQUIRE(M && bs >= 1 && M->n % bs == 0, "wrong block size");
+   M->bs = bs;
+   return 0;
+}
+
+int pk_mat_time_mult(pk_mat M, pk_vec x, pk_vec y, int reps, int fp32, double* avg_seconds) {
+   PK_REQUIRE(M && M->assembled && x && y && reps > 0, "wrong arguments");
+   Ctx& c = ctx();
+   float* xf = nullptr;
+   float* yf = nullptr;
+   if (fp32) {
+      PK_CALL(M->ensure_fp32());
+      PK_HIP(hipMalloc(&xf, sizeof(float) * M->n));
+      PK_HIP(hipMalloc(&yf, sizeof(float) * M->n));
+      to_f32_kernel<<<grid_for(M->n), kBlock, 0, c.stream>>>(x->d.p, xf, M->n);
+   }
+   hipEvent_t e0, e1;
+   PK_HIP(hipEventCreate(&e0));
+   PK_HIP(hipEventCreate(&e1));
+   // warm-up
+   if (fp32)
+      PK_CALL(spmv_f(M, xf, yf));
+   else
+      PK_CALL(spmv(M, x->d.p, y->d.p));
+   PK_HIP(hipEventRecord(e0, c.stream));
+   for (int r = 0; r < reps; r++) {
+      if (fp32)
+         PK_CALL(spmv_f(M, xf, yf));
+      else
+         PK_CALL(spmv(M, x->d.p, y->d.p));
+   }
+   PK_HIP(hipEventRecord(e1, c.stream));
+   PK_HIP(hipEventSynchronize(e1));
+   float ms = 0.f;
+   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+   *avg_seconds = (double)ms * 1e-3 / reps;
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   if (xf) (void)hipFree(xf);
+   if (yf) (void)hipFree(yf);
+   return 0;
+}
+
+}  // extern "C"

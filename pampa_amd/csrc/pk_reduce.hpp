@@ -1,0 +1,113 @@
+// Deterministic block and grid reductions for 64-wide wavefronts.
+//
+// A reduction kernel leaves one partial per workgroup and value in `partials`,
+// then the last workgroup to finish (ticket counter) sums the partials in a
+// fixed order and writes the result.  The split of rows over workgroups
+// depends only on the problem size and the grid, so every run gives the same
+// bits no matter how workgroups are scheduled -- no floating-point atomics.
+//
+// Latency matters more than bandwidth here (a few hundred partials): every
+// value is reduced inside each wave with shuffles, the waves meet in LDS behind
+// a single barrier, and the last workgroup loads all partials in parallel
+// (L lanes per value) before a shuffle tree, so a reduction costs about two
+// global-memory round trips however many values it carries.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pk_internal.hpp"
+
+namespace pk {
+
+__device__ __forceinline__ double wave_sum(double v) {
+   // fixed butterfly order across the 64 lanes; every lane gets the sum
+   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+   return v;
+}
+
+// Sum of one value over the whole workgroup (blockDim.x == kBlock), result in
+// every thread.  `sh` must hold kBlock/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+   v = wave_sum(v);
+   __syncthreads();
+   if (lane == 0) sh[wid] = v;
+   __syncthreads();
+   double s = 0.0;
+#pragma unroll
+   for (int w = 0; w < kBlock / 64; w++) s += sh[w];
+   return s;
+}
+
+// Grid reduction of nv <= MAXV values (runtime count).  Each thread passes its
+// contribution for value k through `get(k)`.  Writes out[0..nv-1] and returns
+// true, in every thread of the last workgroup only, once out[] is complete and
+// visible to the whole workgroup -- the place for a serial epilogue.
+// `partials` must hold gridDim.x * nv doubles.
+template <int MAXV, class Get>
+__device__ bool grid_reduce(int nv, Get get, double* partials, unsigned* ticket, double* out) {
+   __shared__ double sh[MAXV][kBlock / 64];
+   __shared__ int is_last;
+   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+   for (int k = 0; k < MAXV; k++) {
+      if (k < nv) {
+         double v = wave_sum(get(k));
+         if (lane == 0) sh[k][wid] = v;
+      }
+   }
+   __syncthreads();
+   if ((int)threadIdx.x < nv) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; w++) s += sh[threadIdx.x][w];
+      partials[(long)blockIdx.x * nv + threadIdx.x] = s;
+   }
+   __threadfence();
+   __syncthreads();
+   if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+   __syncthreads();
+   if (!is_last) return false;
+   __threadfence();
+   // L lanes per value (L divides 64, so a value's lanes sit in one wave)
+   int L = 64;
+   while (L > 1 && L * nv > kBlock) L >>= 1;
+   const int k = threadIdx.x / L, sub = threadIdx.x % L;
+   const int nb = gridDim.x;
+   double s = 0.0;
+   if (k < nv) {
+      int b = sub;
+      for (; b + 3 * L < nb; b += 4 * L) {
+         double a0 = partials[(long)b * nv + k];
+         double a1 = partials[(long)(b + L) * nv + k];
+         double a2 = partials[(long)(b + 2 * L) * nv + k];
+         double a3 = partials[(long)(b + 3 * L) * nv + k];
+         s += a0;
+         s += a1;
+         s += a2;
+         s += a3;
+      }
+      for (; b < nb; b += L) s += partials[(long)b * nv + k];
+   }
+   for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+   if (k < nv && sub == 0) out[k] = s;
+   if (threadIdx.x == 0) *ticket = 0u;
+   __threadfence_block();
+   __syncthreads();
+   return true;
+}
+
+template <int MAXV, class Get>
+__device__ __forceinline__ void grid_reduce_finish(int nv, Get get, double* partials,
+                                                   unsigned* ticket, double* out) {
+   (void)grid_reduce<MAXV>(nv, get, partials, ticket, out);
+}
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_finish(double (&v)[NV], double* partials,
+                                                    unsigned* ticket, double* out) {
+   (void)grid_reduce<NV>(
+       NV, [&](int k) { return v[k]; }, partials, ticket, out);
+}
+
+}  // namespace pk

@@ -1,0 +1,226 @@
+// Runtime, errors, vectors and deterministic reductions of the pk_* layer.
+#include <cstring>
+#include <mutex>
+
+#include "pk_internal.hpp"
+#include "pk_reduce.hpp"
+
+namespace pk {
+
+static thread_local std::string g_last_error;
+static Ctx g_ctx;
+
+void set_error(const char* file, int line, const char* func, const std::string& msg) {
+   const char* base = std::strrchr(file, '/');
+   base = base ? base + 1 : file;
+   g_last_error = std::string(base) + ":" + std::to_string(line) + ":" + func + "(): error: " + msg + ".";
+}
+
+Ctx& ctx() { return g_ctx; }
+
+int ensure_init() {
+   if (g_ctx.stream) return 0;
+   return pk_initialize(-1);
+}
+
+int reserve_partials(size_t count) {
+   Ctx& c = g_ctx;
+   if (count <= c.partials_cap) return 0;
+   PK_HIP(hipStreamSynchronize(c.stream));
+   (void)hipFree(c.partials);
+   c.partials = nullptr;
+   PK_HIP(hipMalloc(&c.partials, sizeof(double) * count));
+   c.partials_cap = count;
+   return 0;
+}
+
+// ---- reductions ------------------------------------------------------------
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) dot_kernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                     long n, double* partials, unsigned* ticket,
+                                                     double* out) {
+   double s = 0.0;
+   for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < n; i += (long)gridDim.x * kBlock)
+      s += (double)x[i] * (double)y[i];
+   double v[1] = {s};
+   block_reduce_finish<1>(v, partials, ticket, out);
+}
+
+int reduce_dot(const double* x, const double* y, long n, double* out_dev) {
+   Ctx& c = ctx();
+   int g = grid_for(n, kBlock, kMaxRedBlocks);
+   dot_kernel<double><<<g, kBlock, 0, c.stream>>>(x, y, n, c.partials, c.ticket, out_dev);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int reduce_dot_f(const float* x, const float* y, long n, double* out_dev) {
+   Ctx& c = ctx();
+   int g = grid_for(n, kBlock, kMaxRedBlocks);
+   dot_kernel<float><<<g, kBlock, 0, c.stream>>>(x, y, n, c.partials, c.ticket, out_dev);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+// ---- elementwise vector kernels ---------------------------------------------
+
+__global__ void set_kernel(double* x, long n, double a) {
+   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) x[i] = a;
+}
+__global__ void scale_kernel(double* x, long n, double a) {
+   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) x[i] *= a;
+}
+__global__ void axpy_kernel(double* y, long n, double a, const double* x) {
+   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) y[i] += a * x[i];
+}
+
+}  // namespace pk
+
+using namespace pk;
+
+extern "C" {
+
+const char* pk_last_error(void) { return g_last_error.c_str(); }
+
+int pk_initialize(int device) {
+   Ctx& c = ctx();
+   if (c.stream) return 0;
+   int n = 0;
+   PK_HIP(hipGetDeviceCount(&n));
+   PK_REQUIRE(n > 0, "no HIP device available");
+   if (device < 0) device = 0;
+   PK_REQUIRE(device < n, "device index out of range");
+   PK_HIP(hipSetDevice(device));
+   c.device = device;
+   hipDeviceProp_t prop;
+   PK_HIP(hipGetDeviceProperties(&prop, device));
+   c.num_cus = prop.multiProcessorCount;
+   PK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+   PK_HIP(hipMalloc(&c.partials, sizeof(double) * kMaxRedBlocks * kMaxRedVals));
+   c.partials_cap = (size_t)kMaxRedBlocks * kMaxRedVals;
+   PK_HIP(hipMalloc(&c.ticket, sizeof(unsigned) * 64));
+   PK_HIP(hipMemsetAsync(c.ticket, 0, sizeof(unsigned) * 64, c.stream));
+   PK_HIP(hipMalloc(&c.scalars, sizeof(double) * 256));
+   PK_HIP(hipMemsetAsync(c.scalars, 0, sizeof(double) * 256, c.stream));
+   PK_HIP(hipHostMalloc(&c.host_scalars, sizeof(double) * 256, hipHostMallocDefault));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
+}
+
+int pk_finalize(void) {
+   Ctx& c = ctx();
+   if (!c.stream) return 0;
+   PK_HIP(hipStreamSynchronize(c.stream));
+   (void)hipFree(c.partials);
+   (void)hipFree(c.ticket);
+   (void)hipFree(c.scalars);
+   (void)hipHostFree(c.host_scalars);
+   (void)hipStreamDestroy(c.stream);
+   c = Ctx();
+   return 0;
+}
+
+int pk_synchronize(void) {
+   PK_CALL(ensure_init());
+   PK_HIP(hipStreamSynchronize(ctx().stream));
+   return 0;
+}
+
+int pk_vec_create(long n, pk_vec* v) {
+   PK_CALL(ensure_init());
+   PK_REQUIRE(n >= 0, "negative vector size");
+   auto* p = new pk_vec_s();
+   p->n = n;
+   if (p->d.alloc(n ? n : 1)) {
+      delete p;
+      PK_FAIL("device allocation failed");
+   }
+   PK_HIP(hipMemsetAsync(p->d.p, 0, sizeof(double) * (n ? n : 1), ctx().stream));
+   *v = p;
+   return 0;
+}
+
+int pk_vec_destroy(pk_vec* v) {
+   if (!v || !*v) return 0;
+   (void)hipStreamSynchronize(ctx().stream);
+   (*v)->d.free();
+   delete *v;
+   *v = nullptr;
+   return 0;
+}
+
+long pk_vec_size(pk_vec v) { return v ? v->n : -1; }
+
+int pk_vec_set(pk_vec v, double x) {
+   PK_REQUIRE(v, "null vector");
+   if (v->n == 0) return 0;
+   set_kernel<<<grid_for(v->n), kBlock, 0, ctx().stream>>>(v->d.p, v->n, x);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int pk_vec_set_values(pk_vec v, const double* host) {
+   PK_REQUIRE(v && host, "null argument");
+   PK_HIP(hipMemcpyAsync(v->d.p, host, sizeof(double) * v->n, hipMemcpyHostToDevice, ctx().stream));
+   PK_HIP(hipStreamSynchronize(ctx().stream));
+   return 0;
+}
+
+int pk_vec_get_values(pk_vec v, double* host) {
+   PK_REQUIRE(v && host, "null argument");
+   PK_HIP(hipMemcpyAsync(host, v->d.p, sizeof(double) * v->n, hipMemcpyDeviceToHost, ctx().stream));
+   PK_HIP(hipStreamSynchronize(ctx().stream));
+   return 0;
+}
+
+int pk_vec_copy(pk_vec src, pk_vec dst) {
+   PK_REQUIRE(src && dst && src->n == dst->n, "vector size mismatch");
+   PK_HIP(hipMemcpyAsync(dst->d.p, src->d.p, sizeof(double) * src->n, hipMemcpyDeviceToDevice,
+                         ctx().stream));
+   return 0;
+}
+
+int pk_vec_scale(pk_vec v, double a) {
+   PK_REQUIRE(v, "null vector");
+   if (v->n == 0) return 0;
+   scale_kernel<<<grid_for(v->n), kBlock, 0, ctx().stream>>>(v->d.p, v->n, a);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int pk_vec_axpy(pk_vec y, double a, pk_vec x) {
+   PK_REQUIRE(x && y && x->n == y->n, "vector size mismatch");
+   if (y->n == 0) return 0;
+   axpy_kernel<<<grid_for(y->n), kBlock, 0, ctx().stream>>>(y->d.p, y->n, a, x->d.p);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int pk_vec_dot(pk_vec x, pk_vec y, double* r) {
+   PK_REQUIRE(x && y && x->n == y->n && r, "vector size mismatch");
+   Ctx& c = ctx();
+   PK_CALL(reduce_dot(x->d.p, y->d.p, x->n, c.scalars));
+   PK_HIP(hipMemcpyAsync(c.host_scalars, c.scalars, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   *r = c.host_scalars[0];
+   return 0;
+}
+
+int pk_vec_norm2(pk_vec x, double* r) {
+   double d = 0.0;
+   PK_CALL(pk_vec_dot(x, x, &d));
+   *r = sqrt(d);
+   return 0;
+}
+
+int pk_vec_device_ptr(pk_vec v, void** dev) {
+   PK_REQUIRE(v && dev, "null argument");
+   *dev = v->d.p;
+   return 0;
+}
+
+}  // extern "C"
