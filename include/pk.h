@@ -108,9 +108,10 @@ int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const in
                                     double keff, pk_vec S, pk_vec phi0, pk_mat* R, pk_vec b);
 
 /* Power normalisation and power/production rates on the device
- * (NeutronicSolver.cxx:46-116): phi scaled so that sum phi kapf V = power;
- * q[i] = sum_g phi kapf V, P[i] = sum_g phi nusf V / keff.  Returns 1 with an
- * error if a scaled flux is negative. */
+ * (NeutronicSolver.cxx:46-116): phi scaled so that sum phi kapf V = power
+ * (power < 0: phi left as is, rates only); q[i] = sum_g phi kapf V,
+ * P[i] = sum_g phi nusf V / keff.  Returns 1 with an error if a flux is
+ * negative. */
 int pk_normalize_flux(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double keff, pk_vec q,
                       pk_vec P, double* total_power);
 
@@ -156,6 +157,15 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw);
 int pk_sn_get_stats(pk_sn S, int* outer, long* sweeps, double* seconds);
 int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_sweep);
 int pk_sn_destroy(pk_sn* S);
+
+/* ---- kernel profiler (replaces the timing printed by petsc::solve with
+ *      -petsc_verbose, petsc.cxx:382-425,428-488): with profiling on, the hot
+ *      kernels ("gmres_spmv_dot", "sn_sweep_step", "spmv") are bracketed by HIP
+ *      events on the library stream; per kernel name this returns the launch
+ *      count, summed device seconds and summed algorithmic bytes. */
+int pk_profile_enable(int on);
+int pk_profile_reset(void);
+int pk_profile_get(const char* name, long* launches, double* seconds, double* bytes);
 
 #ifdef __cplusplus
 }

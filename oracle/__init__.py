@@ -65,7 +65,7 @@ def _declare(L):
     L.orc_sn_order.argtypes = [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P]
     L.orc_sn_sweep.argtypes = [_I, _I] + [_P] * 6 + [_I, _I] + [_P] * 8
     L.orc_sn_source.argtypes = [_I, _I, _P, _P, _P, _P, _P, _D, _P, _P]
-    L.orc_sn_eigen.argtypes = ([_I, _I] + [_P] * 6 + [_I, _I] + [_P] * 9 + [_I, _I, _D, _D]
+    L.orc_sn_eigen.argtypes = ([_I, _I] + [_P] * 6 + [_I, _I] + [_P] * 8 + [_I, _I, _D, _D]
                                + [_P] * 3)
 
 
@@ -259,7 +259,7 @@ def eigen_sinvert(R: CSR, F: CSR, tol=1e-13, v0=None):
     import scipy.sparse.linalg as sla
     Rs = R.scipy().tocsc()
     Fs = F.scipy()
-    lu = sla.splu(Rs, permc_spec="COLAMD")
+    lu = sla.splu(Rs, permc_spec="MMD_AT_PLUS_A")
     op = sla.LinearOperator(Rs.shape, matvec=lambda v: lu.solve(Fs @ v), dtype=np.float64)
     if v0 is None:
         v0 = np.ones(R.n)

@@ -75,6 +75,36 @@ def _declare(L):
             f = getattr(L, name)
             f.argtypes = args
             f.restype = _I
+    _IP = ct.POINTER(ct.c_int)
+    drop = {
+        "pampa_initialize": [_I, ct.POINTER(ct.c_char_p), ct.POINTER(ct.POINTER(ct.c_double)), _IP, _IP],
+        "pampa_initialize_steady_state": [_I, ct.POINTER(ct.c_char_p), _IP],
+        "pampa_solve": [_I, _D, _D, _IP], "pampa_solve_steady_state": [_IP],
+        "pampa_finalize": [ct.POINTER(ct.POINTER(ct.c_double)), _IP],
+        "pampa_finalize_steady_state": [_IP],
+        "pampa_get_field": [_P, ct.c_char_p, _IP], "pampa_set_field": [_P, ct.c_char_p, _IP],
+        "pampa_ext_get_result": [_P, _P, _IP], "pampa_ext_get_operator": [ct.c_char_p, _P, _P, _P, _P, _P, _IP],
+        "pampa_ext_get_stats": [_P, _P, _P, _P, _IP], "pampa_ext_get_sizes": [_P, _P, _P, _IP],
+        "pampa_ext_time_kernel": [_I, _I, _P, _P, _IP],
+    }
+    for name, args in {"pk_profile_enable": [_I], "pk_profile_reset": [],
+                       "pk_profile_get": [ct.c_char_p, _P, _P, _P]}.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _I
+    for name, args in drop.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = None
+    L.pk_sn_create.argtypes = [_P, _P, _I, _P, _P, _D, _I, _P, _PP]
+    L.pk_sn_create.restype = _I
+    for name, args in {"pk_sn_get_schedule": [_P, _P, _P], "pk_sn_set_precision": [_P, _I],
+                       "pk_sn_set_tolerances": [_P, _D, _D, _I], "pk_sn_sweep_host": [_P, _P, _P, _P],
+                       "pk_sn_solve": [_P, _P, _P, _P], "pk_sn_get_stats": [_P, _P, _P, _P],
+                       "pk_sn_time_sweep": [_P, _I, _P, _P], "pk_sn_destroy": [_PP]}.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _I
     L.pk_last_error.restype = ct.c_char_p
     L.pk_last_error.argtypes = []
     L.pk_vec_size.restype = _L
@@ -121,6 +151,18 @@ def _new(kind, fn, *args):
 
 def initialize(device: int = 0):
     check(lib().pk_initialize(device))
+
+
+def profile(on: bool):
+    check(lib().pk_profile_enable(int(on)))
+    if on:
+        check(lib().pk_profile_reset())
+
+
+def profile_get(name: str):
+    n, s, b = ct.c_long(), ct.c_double(), ct.c_double()
+    check(lib().pk_profile_get(name.encode(), ct.byref(n), ct.byref(s), ct.byref(b)))
+    return n.value, s.value, b.value
 
 
 class Vec(Handle):
@@ -258,3 +300,142 @@ class Eigen(Handle):
         o, i, r, s = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double()
         check(lib().pk_eig_get_stats(self.h, ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s)))
         return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
+
+
+class DropIn:
+    """In-process use of the drop-in C-ABI (pampa_* entry points): one
+    calculation at a time per process, as in the reference."""
+
+    def __init__(self, input_path: str, *options: str):
+        self.L = lib()
+        args = [b"pampa", input_path.encode()] + [o.encode() for o in options]
+        self._argv = (ct.c_char_p * len(args))(*args)
+        self._dt = ct.POINTER(ct.c_double)()
+        self.ndt = ct.c_int(0)
+        err = ct.c_int(0)
+        self.L.pampa_initialize(len(args), self._argv, ct.byref(self._dt), ct.byref(self.ndt),
+                                ct.byref(err))
+        if err.value:
+            raise PkError("pampa_initialize failed")
+        self.open = True
+
+    def dt(self):
+        return [self._dt[i] for i in range(self.ndt.value)]
+
+    def solve(self, n=0, dt=0.0, t=0.0):
+        err = ct.c_int(0)
+        self.L.pampa_solve(n, dt, t, ct.byref(err))
+        if err.value:
+            raise PkError("pampa_solve failed")
+
+    def sizes(self):
+        c, g, m, err = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
+        self.L.pampa_ext_get_sizes(ct.byref(c), ct.byref(g), ct.byref(m), ct.byref(err))
+        return c.value, g.value, m.value
+
+    def field(self, name: str) -> np.ndarray:
+        C, G, M = self.sizes()
+        n = {"scalar-flux": C * G, "angular-flux": C * G * M}.get(name, C)
+        a = np.empty(n)
+        err = ct.c_int(0)
+        self.L.pampa_get_field(ptr(a), name.encode(), ct.byref(err))
+        if err.value:
+            raise PkError(f"pampa_get_field({name}) failed")
+        return a
+
+    def set_field(self, name: str, values):
+        a = np.ascontiguousarray(values, np.float64)
+        err = ct.c_int(0)
+        self.L.pampa_set_field(ptr(a), name.encode(), ct.byref(err))
+        if err.value:
+            raise PkError(f"pampa_set_field({name}) failed")
+
+    def result(self):
+        k, p, err = ct.c_double(), ct.c_double(), ct.c_int()
+        self.L.pampa_ext_get_result(ct.byref(k), ct.byref(p), ct.byref(err))
+        return k.value, p.value
+
+    def stats(self):
+        o, i, r, s, err = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double(), ct.c_int()
+        self.L.pampa_ext_get_stats(ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s), ct.byref(err))
+        return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
+
+    def time_kernel(self, reps=50, fp32=False):
+        s, b, err = ct.c_double(), ct.c_double(), ct.c_int()
+        self.L.pampa_ext_time_kernel(reps, int(fp32), ct.byref(s), ct.byref(b), ct.byref(err))
+        if err.value:
+            raise PkError("pampa_ext_time_kernel failed")
+        return s.value, b.value
+
+    def operator(self, name: str):
+        n, nnz, err = ct.c_long(), ct.c_long(), ct.c_int()
+        self.L.pampa_ext_get_operator(name.encode(), ct.byref(n), ct.byref(nnz), None, None, None,
+                                      ct.byref(err))
+        if err.value:
+            raise PkError(f"operator {name} not available")
+        p = np.zeros(n.value + 1, np.int32)
+        c = np.zeros(max(nnz.value, 1), np.int32)
+        v = np.zeros(max(nnz.value, 1))
+        self.L.pampa_ext_get_operator(name.encode(), ct.byref(n), ct.byref(nnz), ptr(p), ptr(c), ptr(v),
+                                      ct.byref(err))
+        return p, c[:nnz.value], v[:nnz.value]
+
+    def close(self):
+        if self.open:
+            err = ct.c_int(0)
+            self.L.pampa_finalize(ct.byref(self._dt), ct.byref(err))
+            self.open = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def ragged_faces(nfaces, fmax, *padded):
+    """Padded [C][fmax] face arrays -> ragged (face_ptr, arrays...)."""
+    nfaces = np.asarray(nfaces)
+    C = len(nfaces)
+    fptr = np.zeros(C + 1, np.int32)
+    fptr[1:] = np.cumsum(nfaces)
+    sel = np.concatenate([np.arange(i * fmax, i * fmax + nfaces[i]) for i in range(C)]) if C else np.zeros(0, int)
+    out = [fptr]
+    for a in padded:
+        a = np.asarray(a)
+        flat = a.reshape(C * fmax, -1) if a.ndim == 3 else a.reshape(C * fmax)
+        out.append(np.ascontiguousarray(flat[sel]))
+    return out
+
+
+class Sweep(Handle):
+    """pk_sn handle (matrix-free S_N operator)."""
+
+    def __init__(self, mesh: DeviceMesh, xs: DeviceXS, dirs, wts, bc_type, precision=PK_PREC_FP64):
+        d = np.ascontiguousarray(dirs, np.float64)
+        w = np.ascontiguousarray(wts, np.float64)
+        bt = np.ascontiguousarray(bc_type, np.int32)
+        self._keep = (mesh, xs, d, w, bt)
+        h = ct.c_void_p()
+        check(lib().pk_sn_create(mesh.h, xs.h, len(w), ptr(d), ptr(w), 1.0, len(bt), ptr(bt), ct.byref(h)))
+        super().__init__("sn", h)
+        self.C, self.G, self.M = mesh.C, xs.G, len(w)
+        check(lib().pk_sn_set_precision(h, precision))
+
+    def schedule(self):
+        nl = np.zeros(self.M, np.int32)
+        lv = np.zeros((self.M, self.C), np.int32)
+        check(lib().pk_sn_get_schedule(self.h, ptr(nl), ptr(lv)))
+        return nl, lv
+
+    def sweep(self, q):
+        q = np.ascontiguousarray(q, np.float64)
+        psi = np.zeros(self.C * self.G * self.M)
+        phiw = np.zeros(self.C * self.G)
+        check(lib().pk_sn_sweep_host(self.h, ptr(q), ptr(psi), ptr(phiw)))
+        return psi, phiw
+
+    def time_sweep(self, reps=10):
+        t, b = ct.c_double(), ct.c_double()
+        check(lib().pk_sn_time_sweep(self.h, reps, ct.byref(t), ct.byref(b)))
+        return t.value, b.value

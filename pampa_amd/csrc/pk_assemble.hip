@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) scale_power_kernel(MeshView M, XSView 
                                                               double* P, double* partials,
                                                               unsigned* ticket, double* out) {
    const int G = X.G;
-   double f = power / p0[0];
+   double f = power >= 0.0 ? power / p0[0] : 1.0;  // power < 0: rates only, no scaling
    double neg = 0.0, tot = 0.0;
    for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < M.C; i += (long)gridDim.x * kBlock) {
       int m = M.mat[i];

@@ -57,6 +57,13 @@ struct Ctx {
 
 Ctx& ctx();
 int ensure_init();
+
+// Kernel profiler: with profiling on, instrumented launches are bracketed by
+// a pair of HIP events on the library stream; durations are resolved lazily.
+// `bytes` is the launch's algorithmic traffic.
+bool profiling();
+void prof_begin(const char* name, double bytes);
+void prof_end();
 // Grow the reduction scratch to at least `count` doubles (setup time only).
 int reserve_partials(size_t count);
 
@@ -67,11 +74,20 @@ inline int grid_for(long n, int block = kBlock, int cap = 1 << 30) {
    return (int)g;
 }
 
-// Device buffer owned by a handle.
+// Device buffer owned by a handle (move-only; freed on destruction).  A buffer
+// that borrows another's pointer must clear `p` before it is destroyed.
 template <class T>
 struct DBuf {
    T* p = nullptr;
    size_t n = 0;
+   DBuf() = default;
+   DBuf(const DBuf&) = delete;
+   DBuf& operator=(const DBuf&) = delete;
+   DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+      o.p = nullptr;
+      o.n = 0;
+   }
+   ~DBuf() { free(); }
    int alloc(size_t count) {
       free();
       n = count;
@@ -102,6 +118,7 @@ int reduce_dot_f(const float* x, const float* y, long n, double* out_dev);
 // Sliced-ELL (slice height 64 = one wavefront) device copy of a CSR matrix.
 struct SellData {
    long nslices = 0, padded = 0;
+   long nnz = 0;  // unpadded nonzeros (the algorithmic count)
    pk::DBuf<long> off;
    pk::DBuf<int> width, col;
    pk::DBuf<int> rlen;  // unpadded row lengths (padding sits at k >= rlen[row])

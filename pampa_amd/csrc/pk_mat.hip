@@ -58,6 +58,7 @@ int build_sell(pk_mat_s* A) {
    SellData& S = A->sell;
    S.nslices = ns;
    S.padded = tot;
+   S.nnz = A->nnz;
    PK_REQUIRE(!S.off.alloc(ns + 1) && !S.width.alloc(ns) && !S.col.alloc(tot ? tot : 1) &&
                   !S.val.alloc(tot ? tot : 1) && !S.rlen.alloc(n ? n : 1),
               "device allocation failed");
@@ -101,11 +102,18 @@ __global__ void to_f32_kernel(const double* a, float* b, long n) {
    if (i < n) b[i] = (float)a[i];
 }
 
+// algorithmic bytes of a CSR SpMV: nnz (value + column) + row pointers + x, y
+static double spmv_bytes(const pk_mat_s* A, double s) {
+   return (double)A->sell.nnz * (s + 4.0) + (double)(A->n + 1) * 4.0 + 2.0 * A->n * s;
+}
+
 int spmv(pk_mat_s* A, const double* x, double* y) {
    if (A->n == 0) return 0;
    SellData& S = A->sell;
+   prof_begin("spmv", spmv_bytes(A, 8.0));
    sell_spmv_kernel<double><<<grid_for(A->n, 256), 256, 0, ctx().stream>>>(
        A->n, S.off.p, S.width.p, S.col.p, S.val.p, x, y);
+   prof_end();
    PK_HIP(hipGetLastError());
    return 0;
 }
@@ -114,8 +122,10 @@ int spmv_f(pk_mat_s* A, const float* x, float* y) {
    if (A->n == 0) return 0;
    PK_CALL(A->ensure_fp32());
    SellData& S = A->sell;
+   prof_begin("spmv_fp32", spmv_bytes(A, 4.0));
    sell_spmv_kernel<float><<<grid_for(A->n, 256), 256, 0, ctx().stream>>>(
        A->n, S.off.p, S.width.p, S.col.p, S.val32.p, x, y);
+   prof_end();
    PK_HIP(hipGetLastError());
    return 0;
 }

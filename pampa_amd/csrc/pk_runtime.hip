@@ -10,6 +10,68 @@ namespace pk {
 static thread_local std::string g_last_error;
 static Ctx g_ctx;
 
+// ---- kernel profiler ------------------------------------------------------------
+struct ProfRec {
+   std::string name;
+   double bytes;
+   hipEvent_t e0, e1;
+};
+struct ProfAcc {
+   long launches = 0;
+   double seconds = 0.0, bytes = 0.0;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_pending;
+static std::vector<hipEvent_t> g_event_pool;
+static std::vector<std::pair<std::string, ProfAcc>> g_prof_acc;
+
+static hipEvent_t take_event() {
+   if (!g_event_pool.empty()) {
+      hipEvent_t e = g_event_pool.back();
+      g_event_pool.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   (void)hipEventCreate(&e);
+   return e;
+}
+
+bool profiling() { return g_prof; }
+
+void prof_begin(const char* name, double bytes) {
+   if (!g_prof) return;
+   ProfRec r{name, bytes, take_event(), take_event()};
+   (void)hipEventRecord(r.e0, g_ctx.stream);
+   g_pending.push_back(r);
+}
+
+void prof_end() {
+   if (!g_prof || g_pending.empty()) return;
+   (void)hipEventRecord(g_pending.back().e1, g_ctx.stream);
+}
+
+static void prof_resolve() {
+   if (g_pending.empty()) return;
+   (void)hipStreamSynchronize(g_ctx.stream);
+   for (auto& r : g_pending) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, r.e0, r.e1);
+      ProfAcc* acc = nullptr;
+      for (auto& p : g_prof_acc)
+         if (p.first == r.name) acc = &p.second;
+      if (!acc) {
+         g_prof_acc.push_back({r.name, ProfAcc()});
+         acc = &g_prof_acc.back().second;
+      }
+      acc->launches++;
+      acc->seconds += ms * 1e-3;
+      acc->bytes += r.bytes;
+      g_event_pool.push_back(r.e0);
+      g_event_pool.push_back(r.e1);
+   }
+   g_pending.clear();
+}
+
 void set_error(const char* file, int line, const char* func, const std::string& msg) {
    const char* base = std::strrchr(file, '/');
    base = base ? base + 1 : file;
@@ -220,6 +282,35 @@ int pk_vec_norm2(pk_vec x, double* r) {
 int pk_vec_device_ptr(pk_vec v, void** dev) {
    PK_REQUIRE(v && dev, "null argument");
    *dev = v->d.p;
+   return 0;
+}
+
+int pk_profile_enable(int on) {
+   PK_CALL(ensure_init());
+   prof_resolve();
+   g_prof = on != 0;
+   return 0;
+}
+
+int pk_profile_reset(void) {
+   prof_resolve();
+   g_prof_acc.clear();
+   return 0;
+}
+
+int pk_profile_get(const char* name, long* launches, double* seconds, double* bytes) {
+   PK_REQUIRE(name, "null argument");
+   prof_resolve();
+   for (auto& p : g_prof_acc)
+      if (p.first == name) {
+         if (launches) *launches = p.second.launches;
+         if (seconds) *seconds = p.second.seconds;
+         if (bytes) *bytes = p.second.bytes;
+         return 0;
+      }
+   if (launches) *launches = 0;
+   if (seconds) *seconds = 0.0;
+   if (bytes) *bytes = 0.0;
    return 0;
 }
 

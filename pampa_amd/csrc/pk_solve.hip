@@ -405,12 +405,31 @@ struct Gmres {
       PK_HIP(hipMemcpyAsync(S.resid + 1, c.host_scalars, sizeof(double), hipMemcpyHostToDevice, c.stream));
       while (true) {
          int steps = (int)std::min<long>(m, max_it - its);
+         const bool prof = profiling();
+         const double s = sizeof(T);
          for (int j = 0; j < steps; j++) {
+            // algorithmic bytes: prec reads V[j], the block inverses and writes z, V[j];
+            // spmv_dot is a CSR SpMV plus the j+1 basis vectors; orth reads w and
+            // j+1 basis vectors and writes V[j+1]
+            prof_begin("gmres_prec", (double)n * s * (3.0 + bs));
             prec_kernel<T><<<gb, kBlock, 0, c.stream>>>(n, bs, V.p + (size_t)j * n, binv.p, z.p, S);
+            prof_end();
+            prof_begin("gmres_spmv_dot", (double)A->nnz * (s + 4.0) + (double)(n + 1) * 4.0 +
+                                             2.0 * n * s + (double)(j + 1) * n * s);
             spmv_dot_kernel<T><<<gb, kBlock, 0, c.stream>>>(n, A->off.p, A->width.p, A->col.p, Aval,
                                                             z.p, w.p, V.p, j, m, c.partials,
                                                             c.ticket, S);
+            prof_end();
+            prof_begin("gmres_orth", (double)n * s * (j + 3.0));
             orth_kernel<T><<<gb, kBlock, 0, c.stream>>>(n, w.p, V.p, j, m, c.partials, c.ticket, ww, S);
+            prof_end();
+            if (prof) {
+               // profiling: issue only steps that do work
+               int f0 = 0;
+               PK_HIP(hipMemcpyAsync(&f0, S.flags, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+               PK_HIP(hipStreamSynchronize(c.stream));
+               if (f0) break;
+            }
          }
          PK_HIP(hipGetLastError());
          gmres_trisolve_kernel<<<1, kBlock, 0, c.stream>>>(m, S);
@@ -526,7 +545,8 @@ struct pk_eig_s {
    pk_mat_s* R = nullptr;
    pk_mat_s* F = nullptr;
    long n = 0;
-   int bs = 1;
+   int bs = 1;   // cell block of F (groups, or groups x directions)
+   int pbs = 1;  // block of the Jacobi preconditioner (divides bs and 256, <= 16)
    double tol = 1e-8;
    int max_outer = 10000;
    int precision = PK_PREC_FP64;
@@ -581,9 +601,9 @@ static int eig_setup(pk_eig_s* E) {
    S.col.p = E->R->sell.col.p;
    S.rlen.p = E->R->sell.rlen.p;
    if (E->precision == PK_PREC_MIXED)
-      PK_CALL(E->g32.setup(&E->Ash, E->Ash.val32.p, n, E->bs, E->restart));
+      PK_CALL(E->g32.setup(&E->Ash, E->Ash.val32.p, n, E->pbs, E->restart));
    else
-      PK_CALL(E->g64.setup(&E->Ash, E->Ash.val.p, n, E->bs, E->restart));
+      PK_CALL(E->g64.setup(&E->Ash, E->Ash.val.p, n, E->pbs, E->restart));
    E->ready = true;
    return 0;
 }
@@ -602,6 +622,11 @@ int pk_eig_create(pk_mat R, pk_mat F, pk_eig* E) {
    e->F = F;
    e->n = R->n;
    e->bs = R->bs;
+   // largest power of two <= kMaxBs dividing the cell block (cells of the
+   // preconditioner must not straddle 256-thread workgroups)
+   e->pbs = 1;
+   while (e->pbs * 2 <= kMaxBs && e->bs % (e->pbs * 2) == 0) e->pbs *= 2;
+   if (e->bs <= kMaxBs && kBlock % e->bs == 0) e->pbs = e->bs;
    *E = e;
    return 0;
 }

@@ -1,0 +1,94 @@
+"""The pk_* numerics C-ABI on the GPU: AIJ row semantics, SpMV, reductions."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _aij(n, triples):
+    """PETSc-AIJ semantics on the host: per row, stable sort by column; INSERT
+    keeps the last value of a repeated column, ADD sums in insertion order."""
+    rows = [[] for _ in range(n)]
+    for r, c, v, mode in triples:
+        rows[r].append((c, v, mode))
+    ptr, col, val = [0], [], []
+    for r in range(n):
+        ent = sorted(enumerate(rows[r]), key=lambda t: (t[1][0], t[0]))
+        for _, (c, v, mode) in ent:
+            if col and len(col) > ptr[-1] and col[-1] == c:
+                val[-1] = val[-1] + v if mode == 1 else v
+            else:
+                col.append(c)
+                val.append(v)
+        ptr.append(len(col))
+    return np.array(ptr), np.array(col), np.array(val)
+
+
+def test_set_values_semantics_and_spmv(pk):
+    import ctypes as ct
+    L = pk.lib()
+    rng = np.random.default_rng(3)
+    n = 700
+    h = ct.c_void_p()
+    pk.check(L.pk_mat_create(n, 8, ct.byref(h)))
+    M = pk.Mat(h)
+    triples = []
+    for r in range(n):
+        for _ in range(2):
+            k = int(rng.integers(1, 9))
+            cols = rng.integers(0, n, k).astype(np.int32)
+            vals = rng.normal(size=k)
+            mode = int(rng.integers(0, 2))
+            rr = np.array([r], np.int32)
+            pk.check(L.pk_mat_set_values(h, 1, pk.ptr(rr), k, pk.ptr(cols), pk.ptr(vals), mode))
+            triples += [(r, int(c), float(v), mode) for c, v in zip(cols, vals)]
+    pk.check(L.pk_mat_assemble(h))
+    p, c, v = M.csr()
+    ep, ec, ev = _aij(n, triples)
+    assert np.array_equal(p, ep) and np.array_equal(c, ec)
+    assert np.allclose(v, ev, rtol=0, atol=1e-15)
+    x = rng.normal(size=n)
+    X, Y = pk.Vec(n, x), pk.Vec(n)
+    M.mult(X, Y)
+    import scipy.sparse as sp
+    y_ref = sp.csr_matrix((ev, ec, ep), shape=(n, n)) @ x
+    assert np.allclose(Y.get(), y_ref, rtol=1e-13, atol=1e-13)
+
+
+def test_dot_norm_deterministic(pk):
+    import ctypes as ct
+    L = pk.lib()
+    rng = np.random.default_rng(4)
+    for n in (1, 255, 256, 1000, 300001):
+        a, b = rng.normal(size=n), rng.normal(size=n)
+        A, B = pk.Vec(n, a), pk.Vec(n, b)
+        r1, r2, nr = ct.c_double(), ct.c_double(), ct.c_double()
+        pk.check(L.pk_vec_dot(A.h, B.h, ct.byref(r1)))
+        pk.check(L.pk_vec_dot(A.h, B.h, ct.byref(r2)))
+        pk.check(L.pk_vec_norm2(A.h, ct.byref(nr)))
+        assert r1.value == r2.value  # fixed reduction order
+        assert abs(r1.value - a @ b) <= 1e-12 * max(1.0, np.abs(a) @ np.abs(b))
+        assert abs(nr.value - np.linalg.norm(a)) <= 1e-12 * np.linalg.norm(a)
+
+
+def test_linear_solver(pk):
+    """GMRES + cell-block Jacobi on a 2-group diffusion-like operator."""
+    import ctypes as ct
+    import scipy.sparse as sp
+    L = pk.lib()
+    n1 = 400
+    T = sp.diags([-1.0, 2.05, -1.0], [-1, 0, 1], shape=(n1, n1))
+    A = sp.kron(T, sp.eye(2)) + sp.kron(sp.eye(n1), sp.csr_matrix([[0.1, 0.0], [-0.05, 0.2]]))
+    A = A.tocsr()
+    A.sort_indices()
+    M = pk.Mat.from_csr(A.shape[0], A.indptr, A.indices, A.data, block_size=2)
+    b = np.random.default_rng(5).uniform(size=A.shape[0])
+    B, X = pk.Vec(len(b), b), pk.Vec(len(b))
+    h = ct.c_void_p()
+    pk.check(L.pk_lin_create(M.h, ct.byref(h)))
+    lin = pk.Handle("lin", h)
+    pk.check(L.pk_lin_set_tolerances(h, 1e-10, 0.0, 20000))
+    pk.check(L.pk_lin_solve(h, B.h, X.h))
+    x = X.get()
+    assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+    lin.close()
