@@ -57,41 +57,112 @@ __device__ bool grid_reduce(int nv, Get get, double* partials, unsigned* ticket,
       }
    }
    __syncthreads();
+   // Partials are written with system-scope stores, which go through to memory
+   // instead of sitting dirty in this XCD's L2, and waited for before the
+   // ticket is taken; the last workgroup reads them with system-scope loads.
+   // That gives cross-XCD visibility of the partials without the agent-scope
+   // fence, whose L2 writeback in every workgroup dominated the kernel time.
    if ((int)threadIdx.x < nv) {
       double s = 0.0;
 #pragma unroll
       for (int w = 0; w < kBlock / 64; w++) s += sh[threadIdx.x][w];
-      partials[(long)blockIdx.x * nv + threadIdx.x] = s;
+      __hip_atomic_store(&partials[(long)blockIdx.x * nv + threadIdx.x], s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
    }
-   __threadfence();
+   __builtin_amdgcn_s_waitcnt(0);
    __syncthreads();
-   if (threadIdx.x == 0) is_last = (atomicAdd(ticket, 1u) == gridDim.x - 1);
+   if (threadIdx.x == 0)
+      is_last = (__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 gridDim.x - 1);
    __syncthreads();
    if (!is_last) return false;
-   __threadfence();
    // L lanes per value (L divides 64, so a value's lanes sit in one wave)
    int L = 64;
    while (L > 1 && L * nv > kBlock) L >>= 1;
    const int k = threadIdx.x / L, sub = threadIdx.x % L;
    const int nb = gridDim.x;
    double s = 0.0;
+   auto ld = [&](long p) {
+      return __hip_atomic_load(&partials[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   };
    if (k < nv) {
       int b = sub;
       for (; b + 3 * L < nb; b += 4 * L) {
-         double a0 = partials[(long)b * nv + k];
-         double a1 = partials[(long)(b + L) * nv + k];
-         double a2 = partials[(long)(b + 2 * L) * nv + k];
-         double a3 = partials[(long)(b + 3 * L) * nv + k];
+         double a0 = ld((long)b * nv + k);
+         double a1 = ld((long)(b + L) * nv + k);
+         double a2 = ld((long)(b + 2 * L) * nv + k);
+         double a3 = ld((long)(b + 3 * L) * nv + k);
          s += a0;
          s += a1;
          s += a2;
          s += a3;
       }
-      for (; b < nb; b += L) s += partials[(long)b * nv + k];
+      for (; b < nb; b += L) s += ld((long)b * nv + k);
    }
    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
    if (k < nv && sub == 0) out[k] = s;
-   if (threadIdx.x == 0) *ticket = 0u;
+   if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+   __threadfence_block();
+   __syncthreads();
+   return true;
+}
+
+// Segmented form: the workgroups of one segment (one batched problem) reduce
+// among themselves.  `seg_block` is this workgroup's index inside its segment,
+// `seg_blocks` the segment's workgroup count, `partials` the segment's base
+// (seg_blocks * nv doubles) and `ticket` the segment's counter.
+template <int MAXV, class Get>
+__device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, double* out,
+                           int seg_block, int seg_blocks) {
+   __shared__ double sh[MAXV][kBlock / 64];
+   __shared__ int is_last;
+   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+   for (int k = 0; k < MAXV; k++) {
+      if (k < nv) {
+         double v = wave_sum(get(k));
+         if (lane == 0) sh[k][wid] = v;
+      }
+   }
+   __syncthreads();
+   if ((int)threadIdx.x < nv) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; w++) s += sh[threadIdx.x][w];
+      __hip_atomic_store(&partials[(long)seg_block * nv + threadIdx.x], s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+   }
+   __builtin_amdgcn_s_waitcnt(0);
+   __syncthreads();
+   if (threadIdx.x == 0)
+      is_last = (__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 (unsigned)seg_blocks - 1);
+   __syncthreads();
+   if (!is_last) return false;
+   int L = 64;
+   while (L > 1 && L * nv > kBlock) L >>= 1;
+   const int k = threadIdx.x / L, sub = threadIdx.x % L;
+   auto ld = [&](long p) {
+      return __hip_atomic_load(&partials[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   };
+   double s = 0.0;
+   if (k < nv) {
+      int b = sub;
+      for (; b + 3 * L < seg_blocks; b += 4 * L) {
+         double a0 = ld((long)b * nv + k);
+         double a1 = ld((long)(b + L) * nv + k);
+         double a2 = ld((long)(b + 2 * L) * nv + k);
+         double a3 = ld((long)(b + 3 * L) * nv + k);
+         s += a0;
+         s += a1;
+         s += a2;
+         s += a3;
+      }
+      for (; b < seg_blocks; b += L) s += ld((long)b * nv + k);
+   }
+   for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+   if (k < nv && sub == 0) out[k] = s;
+   if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    __threadfence_block();
    __syncthreads();
    return true;
