@@ -38,6 +38,7 @@ typedef struct pk_mesh_s* pk_mesh;
 typedef struct pk_xs_s* pk_xs;
 typedef struct pk_eig_s* pk_eig;
 typedef struct pk_lin_s* pk_lin;
+typedef struct pk_batch_s* pk_batch;
 typedef struct pk_sn_s* pk_sn;
 
 /* ---- runtime ------------------------------------------------------------
@@ -130,6 +131,24 @@ int pk_eig_solve(pk_eig E);
 int pk_eig_get_eigenpair(pk_eig E, double* lambda, pk_vec x); /* lambda = 1/keff */
 int pk_eig_get_stats(pk_eig E, int* outer, long* inner, double* residual, double* seconds);
 int pk_eig_destroy(pk_eig* E);
+
+/* ---- batched eigen solver: `num` independent problems R_b x = lambda F_b x
+ *      (each one what pk_eig solves; petsc::solve(EPS) petsc.cxx:428-488 called
+ *      for many inputs) advanced together, one kernel launch per step for all
+ *      of them, every convergence decision taken per problem on the device.
+ *      Members may differ in size and block size; F_b must hold one cell block
+ *      per row (the diffusion fission operator).  Options as for pk_eig. */
+int pk_batch_create(int num, const pk_mat* R, const pk_mat* F, pk_batch* B);
+int pk_batch_set_tolerances(pk_batch B, double tol, int max_outer);
+int pk_batch_set_precision(pk_batch B, int precision);
+int pk_batch_set_inner(pk_batch B, int restart, double reduction, int max_inner);
+int pk_batch_set_shift(pk_batch B, double delta);
+int pk_batch_set_initial(pk_batch B, int index, pk_vec x); /* all-zero x: no initial space */
+int pk_batch_solve(pk_batch B);
+int pk_batch_get_eigenpair(pk_batch B, int index, double* lambda, pk_vec x);
+int pk_batch_get_stats(pk_batch B, int index, int* outer, long* inner, double* residual,
+                       double* seconds);
+int pk_batch_destroy(pk_batch* B);
 
 /* ---- linear solver (petsc::create(KSP&) petsc.cxx:147-172, petsc::solve(KSP)
  *      petsc.cxx:382-425): GMRES + cell-block Jacobi, nonzero initial guess. */

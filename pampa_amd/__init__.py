@@ -69,6 +69,11 @@ def _declare(L):
         "pk_eig_destroy": [_PP],
         "pk_lin_create": [_P, _PP], "pk_lin_set_tolerances": [_P, _D, _D, _I],
         "pk_lin_solve": [_P, _P, _P], "pk_lin_get_stats": [_P, _P, _P], "pk_lin_destroy": [_PP],
+        "pk_batch_create": [_I, _P, _P, _PP], "pk_batch_set_tolerances": [_P, _D, _I],
+        "pk_batch_set_precision": [_P, _I], "pk_batch_set_inner": [_P, _I, _D, _I],
+        "pk_batch_set_shift": [_P, _D], "pk_batch_set_initial": [_P, _I, _P], "pk_batch_solve": [_P],
+        "pk_batch_get_eigenpair": [_P, _I, _P, _P], "pk_batch_get_stats": [_P, _I, _P, _P, _P, _P],
+        "pk_batch_destroy": [_PP],
     }
     for name, args in sig.items():
         if hasattr(L, name):
@@ -299,6 +304,44 @@ class Eigen(Handle):
     def stats(self):
         o, i, r, s = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double()
         check(lib().pk_eig_get_stats(self.h, ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s)))
+        return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
+
+
+class Batch(Handle):
+    """Batched eigen solver over pairs (R_b, F_b): one launch per step for all."""
+
+    def __init__(self, Rs, Fs, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=30,
+                 reduction=0.1, max_inner=2000, max_outer=20000):
+        L = lib()
+        n = len(Rs)
+        assert n == len(Fs) and n > 0
+        ra = (ct.c_void_p * n)(*[r.h.value for r in Rs])
+        fa = (ct.c_void_p * n)(*[f.h.value for f in Fs])
+        h = ct.c_void_p()
+        check(L.pk_batch_create(n, ra, fa, ct.byref(h)))
+        super().__init__("batch", h)
+        self._mats = (list(Rs), list(Fs))
+        self.sizes = [r.size()[0] for r in Rs]
+        check(L.pk_batch_set_tolerances(h, tol, max_outer))
+        check(L.pk_batch_set_precision(h, precision))
+        check(L.pk_batch_set_inner(h, restart, reduction, max_inner))
+        check(L.pk_batch_set_shift(h, shift))
+
+    def set_initial(self, index: int, x: "Vec"):
+        check(lib().pk_batch_set_initial(self.h, index, x.h))
+
+    def solve(self):
+        check(lib().pk_batch_solve(self.h))
+
+    def eigenpair(self, index: int):
+        lam = ct.c_double()
+        x = Vec(self.sizes[index])
+        check(lib().pk_batch_get_eigenpair(self.h, index, ct.byref(lam), x.h))
+        return 1.0 / lam.value, x
+
+    def stats(self, index: int):
+        o, i, r, s = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double()
+        check(lib().pk_batch_get_stats(self.h, index, ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s)))
         return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
 
 
