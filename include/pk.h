@@ -150,6 +150,29 @@ int pk_batch_get_stats(pk_batch B, int index, int* outer, long* inner, double* r
                        double* seconds);
 int pk_batch_destroy(pk_batch* B);
 
+/* ---- eigen pool: the engine behind pk_batch, for continuous batching.  A pool
+ *      of fixed capacity (rows, stored entries of R and F, block-inverse values,
+ *      slots) holds problems that are admitted and released while others
+ *      iterate; each step advances every live problem by one tick (an outer
+ *      update where due, then one GMRES restart cycle).  Results of a problem do
+ *      not depend on what else is in the pool. */
+#define PK_POOL_FULL 2 /* pk_pool_admit: no room now, nothing changed */
+/* device bytes a pool of this capacity allocates */
+double pk_pool_bytes(long max_rows, long max_entries, long max_fentries, long max_binv, int max_slots,
+                     int precision, int restart);
+int pk_pool_create(long max_rows, long max_entries, long max_fentries, long max_binv, int max_slots,
+                   int precision, int restart, pk_batch* P);
+int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction, int max_inner,
+                        double delta);
+/* copies R, F (and x0 when not NULL) into the pool; the caller may destroy them after */
+int pk_pool_admit(pk_batch P, pk_mat R, pk_mat F, pk_vec x0, int* slot);
+int pk_pool_step(pk_batch P, int ticks, int* live, int* done);
+int pk_pool_collect(pk_batch P, int max, int* slots, int* count); /* finished, admission order */
+int pk_pool_result(pk_batch P, int slot, double* keff, pk_vec x, int* outer, long* inner,
+                   double* residual, int* converged);
+int pk_pool_release(pk_batch P, int slot);
+int pk_pool_capacity(pk_batch P, long* free_rows, long* free_entries, int* free_slots, long* ticks);
+
 /* ---- linear solver (petsc::create(KSP&) petsc.cxx:147-172, petsc::solve(KSP)
  *      petsc.cxx:382-425): GMRES + cell-block Jacobi, nonzero initial guess. */
 int pk_lin_create(pk_mat A, pk_lin* L);

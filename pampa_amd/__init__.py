@@ -74,6 +74,11 @@ def _declare(L):
         "pk_batch_set_shift": [_P, _D], "pk_batch_set_initial": [_P, _I, _P], "pk_batch_solve": [_P],
         "pk_batch_get_eigenpair": [_P, _I, _P, _P], "pk_batch_get_stats": [_P, _I, _P, _P, _P, _P],
         "pk_batch_destroy": [_PP],
+        "pk_pool_create": [_L, _L, _L, _L, _I, _I, _I, _PP],
+        "pk_pool_set_options": [_P, _D, _I, _D, _I, _D], "pk_pool_admit": [_P, _P, _P, _P, _P],
+        "pk_pool_step": [_P, _I, _P, _P], "pk_pool_collect": [_P, _I, _P, _P],
+        "pk_pool_result": [_P, _I, _P, _P, _P, _P, _P, _P], "pk_pool_release": [_P, _I],
+        "pk_pool_capacity": [_P, _P, _P, _P, _P],
     }
     for name, args in sig.items():
         if hasattr(L, name):
@@ -110,6 +115,8 @@ def _declare(L):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = _I
+    L.pk_pool_bytes.argtypes = [_L, _L, _L, _L, _I, _I, _I]
+    L.pk_pool_bytes.restype = _D
     L.pk_last_error.restype = ct.c_char_p
     L.pk_last_error.argtypes = []
     L.pk_vec_size.restype = _L
@@ -343,6 +350,69 @@ class Batch(Handle):
         o, i, r, s = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double()
         check(lib().pk_batch_get_stats(self.h, index, ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s)))
         return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
+
+
+PK_POOL_FULL = 2
+
+
+class Pool(Handle):
+    """Eigen pool for continuous batching: admit (R, F[, x0]) problems while
+    others iterate, step, collect finished slots, release them."""
+
+    def __init__(self, max_rows, max_entries, max_fentries, max_binv, max_slots,
+                 precision=PK_PREC_FP64, restart=30):
+        h = ct.c_void_p()
+        check(lib().pk_pool_create(max_rows, max_entries, max_fentries, max_binv, max_slots, precision,
+                                   restart, ct.byref(h)))
+        super().__init__("batch", h)  # a pool is a pk_batch handle
+        self.sizes = {}
+
+    @staticmethod
+    def bytes(max_rows, max_entries, max_fentries, max_binv, max_slots, precision=PK_PREC_FP64,
+              restart=30):
+        return lib().pk_pool_bytes(max_rows, max_entries, max_fentries, max_binv, max_slots, precision,
+                                   restart)
+
+    def options(self, tol=1e-8, max_outer=10000, reduction=0.1, max_inner=2000, shift=0.1):
+        check(lib().pk_pool_set_options(self.h, tol, max_outer, reduction, max_inner, shift))
+
+    def admit(self, R: "Mat", F: "Mat", x0: "Vec | None" = None):
+        """Slot index, or None when the pool has no room now."""
+        s = ct.c_int(-1)
+        rc = lib().pk_pool_admit(self.h, R.h, F.h, x0.h if x0 is not None else None, ct.byref(s))
+        if rc == PK_POOL_FULL:
+            return None
+        check(rc)
+        self.sizes[s.value] = R.size()[0]
+        return s.value
+
+    def step(self, ticks=1):
+        live, done = ct.c_int(), ct.c_int()
+        check(lib().pk_pool_step(self.h, ticks, ct.byref(live), ct.byref(done)))
+        return live.value, done.value
+
+    def collect(self, max_n=4096):
+        arr = (ct.c_int * max_n)()
+        n = ct.c_int()
+        check(lib().pk_pool_collect(self.h, max_n, arr, ct.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    def result(self, slot, with_vector=True):
+        k, o, i, r, c = ct.c_double(), ct.c_int(), ct.c_long(), ct.c_double(), ct.c_int()
+        x = Vec(self.sizes[slot]) if with_vector else None
+        check(lib().pk_pool_result(self.h, slot, ct.byref(k), x.h if x is not None else None, ct.byref(o),
+                                   ct.byref(i), ct.byref(r), ct.byref(c)))
+        return dict(keff=k.value, x=x, outer=o.value, inner=i.value, residual=r.value,
+                    converged=bool(c.value))
+
+    def release(self, slot):
+        check(lib().pk_pool_release(self.h, slot))
+        self.sizes.pop(slot, None)
+
+    def capacity(self):
+        fr, fe, fs, t = ct.c_long(), ct.c_long(), ct.c_int(), ct.c_long()
+        check(lib().pk_pool_capacity(self.h, ct.byref(fr), ct.byref(fe), ct.byref(fs), ct.byref(t)))
+        return dict(free_rows=fr.value, free_entries=fe.value, free_slots=fs.value, ticks=t.value)
 
 
 class DropIn:

@@ -1,49 +1,54 @@
-// Batched k-eigenvalue solver: B independent problems R_b x = lambda F_b x
-// advanced together, one kernel launch per step for all of them.
+// Pooled k-eigenvalue engine: many independent problems R_b x = lambda F_b x
+// resident on the device at once, every kernel launch advancing all of them,
+// problems admitted and released while others run (continuous batching).
 //
-// Layout.  The instances are concatenated into one block-diagonal system.
-// Instance b owns rows row_off[b] .. row_off[b] + n_b - 1 and is padded to a
-// multiple of 256 rows, so a 256-thread workgroup never holds two instances;
-// wg_inst[] maps a workgroup to its instance.  The SELL-64 arrays of R, F and
-// of the shifted operator A = R - F/ks are concatenated on the device with the
-// columns moved by row_off[b]; padding slices have width 0.
+// Layout.  The pool owns one row space (in units of 256 rows, one workgroup
+// each) and entry spaces for the SELL-64 values of R, F and the shifted
+// operator A = R - F/ks.  A problem in slot b owns a range of row units
+// (n_b rounded up to 256, so a workgroup never holds two problems) and ranges
+// of entries; the columns of its SELL arrays are moved by its row offset, so
+// the pool is one block-diagonal system.  unit_slot[] maps a row unit to its
+// slot (-1 when free) and kernels return at once for free units.
 //
-// Reductions are segmented: the workgroups of one instance reduce among
-// themselves (pk_reduce.hpp seg_reduce), and the last one runs that
-// instance's serial epilogue (Givens rotations, eigenvalue update, stop
-// decisions).  Every decision is taken on the device, per instance; converged
-// or stopped instances skip their workgroups at kernel entry.  The host syncs
-// once per GMRES restart cycle and once per outer iteration, only to learn
-// whether every instance is done.  A restart cycle (m GMRES steps and the
-// update) is captured once as a HIP graph and relaunched.
-//
-// Algorithm per instance (FP64 outer, FP32 or FP64 inner):
-//   k = <Fx,Fx>/<Fx,Rx>,  phi = x/|x|,  r = F phi/k - R phi,  res = |r|/|F phi/k|
-//   stop when res <= tol; after the estimate settles, shift ks = k + delta;
-//   solve (R - F/ks) e = r by right-preconditioned GMRES(m) with cell-block
-//   Jacobi to a relative reduction, x = phi + e.
+// Every decision is per slot and on the device.  Reductions are segmented
+// (pk_reduce.hpp seg_reduce): the workgroups of one slot reduce among
+// themselves and the last one runs that slot's serial epilogue.  A slot walks
+// a state machine, one "tick" at a time:
+//   OUTER  k = <Fx,Fx>/<Fx,Rx>, phi = x/|x|, r = F phi/k - R phi,
+//          res = |r|/|F phi/k|; res <= tol -> DONE; after the estimate settles
+//          shift ks = k + delta (rebuild A and its block inverses) -> INNER
+//   INNER  right-preconditioned GMRES(m) with cell-block Jacobi on A e = r,
+//          one restart cycle per tick, to |r - A e| <= reduction |r|;
+//          then x = phi + e -> OUTER
+// A tick is a fixed kernel sequence, captured once as a HIP graph (per grid
+// size) and relaunched; the host syncs once per tick to learn which slots are
+// DONE.  FP64 outer quantities; the GMRES basis, operator copy and block
+// inverses in FP32 (mixed) or FP64, with FP64 reductions and Givens rotations.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
 
 namespace pk {
 
-constexpr int kRows = 256;
-constexpr int kBMaxM = 64;
-constexpr int kBMaxBs = 16;
+constexpr int kRows = 256;   // rows per unit (= workgroup)
+constexpr int kBMaxM = 64;   // largest GMRES restart
+constexpr int kBMaxBs = 16;  // largest preconditioner block
+
+enum Phase { kFree = 0, kOuter = 1, kInner = 2, kDone = 3 };
 
 struct BLayout {
-   const int* wg_inst;    // [nwg]
-   const int* wg_first;   // [B]
-   const int* wg_count;   // [B]
-   const long* row_off;   // [B]
-   const int* nrows;      // [B]
-   const int* pbs;        // [B] preconditioner block
-   const int* fbs;        // [B] cell block of F
-   const long* binv_off;  // [B]
+   const int* unit_slot;   // [units]
+   const int* wg_first;    // [S] first unit
+   const int* wg_count;    // [S] units
+   const long* row_off;    // [S]
+   const int* nrows;       // [S]
+   const int* pbs;         // [S] preconditioner block
+   const int* fbs;         // [S] cell block of F
+   const long* binv_off;   // [S]
 };
 
 struct BSell {
@@ -53,82 +58,84 @@ struct BSell {
    const int* rlen;
 };
 
-struct BG {  // per-instance GMRES state (arrays of B)
-   double* H;      // [B][(m+1) m] column-major per instance
-   double* cs;     // [B][m]
-   double* sn;     // [B][m]
-   double* g;      // [B][m+1]
-   double* y;      // [B][m]
-   double* scale;  // [B]
-   double* resid;  // [B][2]: estimate, target
-   int* flags;     // [B][4]: stop, steps in cycle, breakdown, steps total
-   double* ww;     // [B]
-   const int* active;  // [B]
+struct BState {  // per-slot state (arrays over slots)
    int m;
+   // GMRES
+   double* H;      // [S][(m+1) m] column-major
+   double* cs;     // [S][m]
+   double* sn;     // [S][m]
+   double* g;      // [S][m+1]
+   double* y;      // [S][m]
+   double* scale;  // [S]
+   double* resid;  // [S][2]: estimate, target
+   double* ww;     // [S]
+   int* flags;     // [S][4]: stop, steps in cycle, breakdown, steps in this solve
+   // outer
+   double* dots;   // [S][3]: <Fx,Rx>, <Fx,Fx>, <x,x>
+   double* rq;     // [S][2]: |r|^2, |F phi/k|^2
+   double* k;      // [S]
+   double* kprev;  // [S]
+   double* ks;     // [S] 0 = no shift
+   double* res;    // [S]
+   int* phase;     // [S]
+   int* start;     // [S] 1: the inner solve starts this tick
+   int* rebuild;   // [S] 1: rebuild A and its block inverses this tick
+   int* shifted;   // [S]
+   int* it;        // [S] corrections done
+   int* inner;     // [S] GMRES steps in all corrections
+   int* status;    // [S] 0 running, 1 converged, 2 failed
 };
 
-struct BE {  // per-instance outer state
-   double* dots;   // [B][3]: <y2,y1>, <y2,y2>, <x,x>
-   double* rq;     // [B][2]: |r|^2, |q|^2
-   double* k;      // [B]
-   double* kprev;  // [B]
-   double* ks;     // [B] 0 = no shift
-   double* res;    // [B]
-   int* it;        // [B]
-   int* conv;      // [B]
-   int* active;    // [B]
-   int* rebuild;   // [B]
-   int* shifted;   // [B]
+struct BOpts {
+   double tol, delta, reduction;
+   int max_outer, max_inner;
 };
 
-__device__ __forceinline__ int seg_inst(const BLayout& L) { return L.wg_inst[blockIdx.x]; }
+__device__ __forceinline__ int slot_of_wg(const BLayout& L) { return L.unit_slot[blockIdx.x]; }
 
-// ---- SELL concatenation ---------------------------------------------------------
+__device__ __forceinline__ int slot_of_row(const BLayout& L, long row) { return L.unit_slot[row / kRows]; }
 
-__global__ void cat_slices_kernel(long ns_src, long ns_dst, const long* soff, const int* swidth,
-                                  long dst_slice0, long base, long* doff, int* dwidth) {
+// ---- admission: place a problem's SELL arrays in the pool --------------------------
+
+__global__ void place_slices_kernel(long ns_src, long ns_dst, const long* soff, const int* swidth,
+                                    long dst_slice0, long base, long* doff, int* dwidth) {
    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (s >= ns_dst) return;
    if (s < ns_src) {
       doff[dst_slice0 + s] = base + (soff[s] - soff[0]);
       dwidth[dst_slice0 + s] = swidth[s];
    } else {
-      doff[dst_slice0 + s] = base + (soff[ns_src] - soff[0]);
+      doff[dst_slice0 + s] = base;
       dwidth[dst_slice0 + s] = 0;
    }
 }
 
-__global__ void cat_entries_kernel(long n, const int* scol, int shift, int* dcol) {
+__global__ void place_cols_kernel(long n, const int* scol, int shift, int* dcol) {
    long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (p < n) dcol[p] = scol[p] + shift;
 }
 
-__global__ void cat_rlen_kernel(long n_src, long n_dst, const int* src, int* dst) {
+__global__ void place_rlen_kernel(long n_src, long n_dst, const int* src, int* dst) {
    long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (r < n_dst) dst[r] = r < n_src ? src[r] : 0;
 }
 
-__global__ void set_last_off_kernel(long* off, long ns, long total) {
-   if (blockIdx.x == 0 && threadIdx.x == 0) off[ns] = total;
+// initial iterate of a slot: x0 (n values) or ones; zero on padding rows
+__global__ void place_x_kernel(long n, long padded, const double* x0, double* x) {
+   long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (r >= padded) return;
+   x[r] = r < n ? (x0 ? x0[r] : 1.0) : 0.0;
 }
 
-template <class T>
-__global__ void to_t_kernel(const double* a, T* b, long n) {
-   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (i < n) b[i] = (T)a[i];
-}
+// ---- shifted operator and block inverses (slots flagged for rebuild) -------------
 
-// ---- shifted operator and block inverses ------------------------------------------
-
-// A = R - F/ks for instances flagged for rebuild (all instances when force)
 __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BSell F,
-                              const double* fval, const double* ks, const int* rebuild, int force,
-                              double* aval) {
+                              const double* fval, BState S, double* aval) {
    long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= N) return;
-   const int b = L.wg_inst[row / kRows];
-   if (!force && !rebuild[b]) return;
-   const double inv_ks = ks[b] > 0.0 ? 1.0 / ks[b] : 0.0;
+   const int b = slot_of_row(L, row);
+   if (b < 0 || !S.rebuild[b]) return;
+   const double inv_ks = S.ks[b] > 0.0 ? 1.0 / S.ks[b] : 0.0;
    const long local = row - L.row_off[b];
    const long s = row >> 6;
    const int lane = (int)(row & 63);
@@ -136,24 +143,37 @@ __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BS
    const int fb = L.fbs[b];
    const long cell = local / fb;
    const long fbase = F.off[s] + lane;
+   const int len = R.rlen[row];
    for (int k = 0; k < R.width[s]; k++) {
       long p = base + (long)k * 64;
       double v = rval[p];
-      if (k < R.rlen[row]) {
+      if (k < len && inv_ks != 0.0) {
          long c = (long)R.col[p] - L.row_off[b];
-         if (c / fb == cell && inv_ks != 0.0) v -= fval[fbase + (c % fb) * 64] * inv_ks;
+         if (c / fb == cell) v -= fval[fbase + (c % fb) * 64] * inv_ks;
       }
       aval[p] = v;
    }
 }
 
+// values of rebuilt slots, converted to the inner precision
 template <class T>
-__global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* aval,
-                                      const int* rebuild, int force, T* binv, int* err) {
+__global__ void to_t_kernel(long N, BLayout L, const int* rebuild, const double* a, T* out, BSell A) {
    long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= N) return;
-   const int b = L.wg_inst[row / kRows];
-   if (!force && !rebuild[b]) return;
+   const int b = slot_of_row(L, row);
+   if (b < 0 || !rebuild[b]) return;
+   const long s = row >> 6;
+   const long base = A.off[s] + (row & 63);
+   for (int k = 0; k < A.width[s]; k++) out[base + (long)k * 64] = (T)a[base + (long)k * 64];
+}
+
+template <class T>
+__global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* aval,
+                                      const int* rebuild, T* binv, int* err) {
+   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (row >= N) return;
+   const int b = slot_of_row(L, row);
+   if (b < 0 || !rebuild[b]) return;
    const long local = row - L.row_off[b];
    const int bs = L.pbs[b];
    if (local >= L.nrows[b] || local % bs != 0) return;
@@ -174,6 +194,7 @@ __global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* 
          if (c / bs == cell) a[r * bs + (c % bs)] += aval[p];
       }
    }
+   // Gauss-Jordan with partial pivoting
    for (int k = 0; k < bs; k++) {
       int piv = k;
       double best = fabs(a[k * bs + k]);
@@ -214,18 +235,139 @@ __global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* 
    for (int p = 0; p < bs * bs; p++) dst[p] = (T)m[p];
 }
 
-// ---- GMRES kernels ------------------------------------------------------------------
+__global__ void bclear_rebuild_kernel(int nslots, BState S) {
+   int b = blockIdx.x * blockDim.x + threadIdx.x;
+   if (b < nslots) S.rebuild[b] = 0;
+}
 
+// ---- outer step ------------------------------------------------------------------------
+
+// y1 = R x, y2 = F x and <y2,y1>, <y2,y2>, <x,x> per slot
+__global__ void __launch_bounds__(kRows) brf_kernel(BLayout L, BSell R, const double* rval, BSell F,
+                                                     const double* fval, const double* x, double* y1,
+                                                     double* y2, double* partials, unsigned* tickets,
+                                                     BState S) {
+   const int b = slot_of_wg(L);
+   if (b < 0 || S.phase[b] != kOuter) return;
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const long s = row >> 6;
+   const int lane = threadIdx.x & 63;
+   double a = 0.0, c = 0.0;
+   long base = R.off[s] + lane;
+   for (int k = 0; k < R.width[s]; k++) {
+      long p = base + (long)k * 64;
+      a += rval[p] * x[R.col[p]];
+   }
+   base = F.off[s] + lane;
+   for (int k = 0; k < F.width[s]; k++) {
+      long p = base + (long)k * 64;
+      c += fval[p] * x[F.col[p]];
+   }
+   y1[row] = a;
+   y2[row] = c;
+   const double xv = x[row];
+   seg_reduce<3>(
+       3, [&](int k) { return k == 0 ? c * a : (k == 1 ? c * c : xv * xv); },
+       partials + (long)L.wg_first[b] * 3, tickets + b, S.dots + 3 * b, (int)blockIdx.x - L.wg_first[b],
+       L.wg_count[b]);
+}
+
+// phi = x/|x|, r = (y2/k - y1)/|x|; residual; convergence, shift and phase decisions
+__global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* x, const double* y1,
+                                                        const double* y2, double* phi, double* r,
+                                                        BOpts o, double* partials, unsigned* tickets,
+                                                        BState S) {
+   const int b = slot_of_wg(L);
+   if (b < 0 || S.phase[b] != kOuter) return;
+   const double* d = S.dots + 3 * b;
+   const double inv_k = d[0] / d[1];
+   const double inv_nx = 1.0 / sqrt(d[2]);
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const double q = y2[row] * inv_k * inv_nx;
+   const double v = q - y1[row] * inv_nx;
+   phi[row] = x[row] * inv_nx;
+   r[row] = v;
+   const double rr = v * v, qq = q * q;
+   bool last = seg_reduce<2>(
+       2, [&](int k) { return k == 0 ? rr : qq; }, partials + (long)L.wg_first[b] * 2, tickets + b,
+       S.rq + 2 * b, (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
+   if (!last || threadIdx.x != 0) return;
+   const double k = d[1] / d[0];
+   const double res = sqrt(S.rq[2 * b] / S.rq[2 * b + 1]);
+   S.k[b] = k;
+   S.res[b] = res;
+   if (res <= o.tol) {
+      S.status[b] = 1;
+      S.phase[b] = kDone;
+      return;
+   }
+   if (!(res == res) || S.it[b] >= o.max_outer) {
+      S.status[b] = 2;
+      S.phase[b] = kDone;
+      return;
+   }
+   if (o.delta > 0.0) {
+      const bool settle = S.it[b] >= 2 && fabs(k - S.kprev[b]) < 1e-2 * k;
+      if (settle && (!S.shifted[b] || fabs(k + o.delta - S.ks[b]) > 0.2 * o.delta)) {
+         S.ks[b] = k + o.delta;
+         S.shifted[b] = 1;
+         S.rebuild[b] = 1;
+      }
+   }
+   S.kprev[b] = k;
+   S.it[b] += 1;
+   S.phase[b] = kInner;
+   S.start[b] = 1;
+}
+
+// ---- GMRES ------------------------------------------------------------------------------
+
+// V0 = rhs = r (in T), e = 0, beta = |V0|, target = reduction beta
 template <class T>
-__global__ void __launch_bounds__(kRows) bprec_kernel(long N, BLayout L, T* Vj, const T* binv, T* z, BG S) {
-   const int b = seg_inst(L);
-   if (S.flags[4 * b]) return;
+__global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* r, T* rhs, T* e, T* V0,
+                                                        BOpts o, double* partials, unsigned* tickets,
+                                                        BState S) {
+   const int b = slot_of_wg(L);
+   if (b < 0 || S.phase[b] != kInner || !S.start[b]) return;
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const T v = (T)r[row];
+   rhs[row] = v;
+   e[row] = 0;
+   V0[row] = v;
+   const double u = (double)v;
+   bool last = seg_reduce<1>(
+       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, S.ww + b,
+       (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
+   if (!last || threadIdx.x != 0) return;
+   const double beta = sqrt(S.ww[b]);
+   int* fl = S.flags + 4 * b;
+   S.resid[2 * b + 1] = o.reduction * beta;
+   S.resid[2 * b] = beta;
+   S.g[(long)b * (S.m + 1)] = beta;
+   S.scale[b] = beta > 0.0 ? 1.0 / beta : 0.0;
+   fl[0] = beta > 0.0 ? 0 : 1;
+   fl[1] = 0;
+   fl[2] = 0;
+   fl[3] = 0;
+   S.start[b] = 0;
+}
+
+__device__ __forceinline__ bool gmres_on(const BState& S, int b) {
+   return b >= 0 && S.phase[b] == kInner && !S.flags[4 * b];
+}
+
+// z = B^-1 (V[j] * scale), V[j] *= scale (normalises the new basis vector)
+template <class T>
+__global__ void __launch_bounds__(kRows) bprec_kernel(BLayout L, T* Vj, const T* binv, T* z, BState S) {
+   const int b = slot_of_wg(L);
+   if (!gmres_on(S, b)) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long local = row - L.row_off[b];
    const int bs = L.pbs[b];
    const T sc = (T)S.scale[b];
    T own = 0, acc = 0;
-   if (local < L.nrows[b]) {
+   const bool real = local < L.nrows[b];
+   if (real) {
       const long cell = local / bs;
       const int gr = (int)(local - cell * bs);
       const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
@@ -234,101 +376,95 @@ __global__ void __launch_bounds__(kRows) bprec_kernel(long N, BLayout L, T* Vj, 
       own = Vj[row] * sc;
    }
    __syncthreads();
-   if (row < N) {
-      z[row] = acc;
-      if (local < L.nrows[b]) Vj[row] = own;
-   }
+   z[row] = acc;
+   if (real) Vj[row] = own;
 }
 
+// w = A z and h(i,j) = <V[i], w>, i <= j
 template <class T>
 __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSell A, const T* aval,
                                                           const T* z, T* w, const T* V, int j,
-                                                          double* partials, unsigned* tickets, BG S) {
-   const int b = seg_inst(L);
-   if (S.flags[4 * b]) return;
+                                                          double* partials, unsigned* tickets, BState S) {
+   const int b = slot_of_wg(L);
+   if (!gmres_on(S, b)) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const long s = row >> 6;
+   const long base = A.off[s] + (threadIdx.x & 63);
+   const int wd = A.width[s];
    T acc = 0;
-   if (row < N) {
-      const long s = row >> 6;
-      const long base = A.off[s] + (threadIdx.x & 63);
-      const int wd = A.width[s];
-      for (int k = 0; k < wd; k++) {
-         long p = base + (long)k * 64;
-         acc += aval[p] * z[A.col[p]];
-      }
-      w[row] = acc;
+   for (int k = 0; k < wd; k++) {
+      long p = base + (long)k * 64;
+      acc += aval[p] * z[A.col[p]];
    }
+   w[row] = acc;
    const double wr = (double)acc;
    const int nv = j + 1;
    seg_reduce<kBMaxM>(
-       nv, [&](int k) { return row < N ? (double)V[(long)k * N + row] * wr : 0.0; },
-       partials + (long)L.wg_first[b] * nv, tickets + b, S.H + (long)b * (S.m + 1) * S.m + (long)j * (S.m + 1),
-       (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
+       nv, [&](int k) { return (double)V[(long)k * N + row] * wr; }, partials + (long)L.wg_first[b] * nv,
+       tickets + b, S.H + (long)b * (S.m + 1) * S.m + (long)j * (S.m + 1), (int)blockIdx.x - L.wg_first[b],
+       L.wg_count[b]);
 }
 
+// V[j+1] = w - sum_i h(i,j) V[i], h(j+1,j) = |V[j+1]|, Givens update, stop test
 template <class T>
 __global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, int j,
-                                                      double* partials, unsigned* tickets, BG S) {
-   const int b = seg_inst(L);
-   if (S.flags[4 * b]) return;
+                                                      double* partials, unsigned* tickets, BState S) {
+   const int b = slot_of_wg(L);
+   if (!gmres_on(S, b)) return;
    const int m = S.m;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    double* Hb = S.H + (long)b * (m + 1) * m;
-   double u = 0.0;
-   if (row < N) {
-      const double* h = Hb + (long)j * (m + 1);
-      T t = w[row];
-      for (int i = 0; i <= j; i++) t -= (T)h[i] * V[(long)i * N + row];
-      V[(long)(j + 1) * N + row] = t;
-      u = (double)t;
-   }
+   const double* h = Hb + (long)j * (m + 1);
+   T t = w[row];
+   for (int i = 0; i <= j; i++) t -= (T)h[i] * V[(long)i * N + row];
+   V[(long)(j + 1) * N + row] = t;
+   const double u = (double)t;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, S.ww + b,
        (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    if (!last) return;
    __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], gj;
    const int ld = m + 1;
-   const int t = threadIdx.x;
-   if (t <= j) hc[t] = Hb[t + (long)j * ld];
-   if (t < j) {
-      cc[t] = S.cs[(long)b * m + t];
-      ss[t] = S.sn[(long)b * m + t];
+   const int tid = threadIdx.x;
+   if (tid <= j) hc[tid] = Hb[tid + (long)j * ld];
+   if (tid < j) {
+      cc[tid] = S.cs[(long)b * m + tid];
+      ss[tid] = S.sn[(long)b * m + tid];
    }
-   if (t == 0) gj = S.g[(long)b * (m + 1) + j];
+   if (tid == 0) gj = S.g[(long)b * (m + 1) + j];
    __syncthreads();
-   if (t == 0) {
-      double hn = sqrt(fmax(S.ww[b], 0.0));
-      hc[j + 1] = hn;
-      for (int i = 0; i < j; i++) {
-         double a = hc[i], c2 = hc[i + 1];
-         hc[i] = cc[i] * a + ss[i] * c2;
-         hc[i + 1] = -ss[i] * a + cc[i] * c2;
-      }
-      double a = hc[j], c2 = hc[j + 1];
-      double rr = sqrt(a * a + c2 * c2);
-      double c = rr > 0.0 ? a / rr : 1.0, s = rr > 0.0 ? c2 / rr : 0.0;
-      hc[j] = rr;
-      hc[j + 1] = 0.0;
-      for (int i = 0; i <= j + 1; i++) Hb[i + (long)j * ld] = hc[i];
-      S.cs[(long)b * m + j] = c;
-      S.sn[(long)b * m + j] = s;
-      S.g[(long)b * (m + 1) + j + 1] = -s * gj;
-      S.g[(long)b * (m + 1) + j] = c * gj;
-      double res = fabs(s * gj);
-      S.resid[2 * b] = res;
-      int* fl = S.flags + 4 * b;
-      fl[1] = j + 1;
-      bool breakdown = !(hn > 1e-300) || !(rr > 0.0);
-      if (breakdown) fl[2] = 1;
-      if (res <= S.resid[2 * b + 1] || breakdown || j + 1 == m) fl[0] = 1;
-      S.scale[b] = hn > 0.0 ? 1.0 / hn : 0.0;
+   if (tid != 0) return;
+   const double hn = sqrt(fmax(S.ww[b], 0.0));
+   hc[j + 1] = hn;
+   for (int i = 0; i < j; i++) {
+      double a = hc[i], c2 = hc[i + 1];
+      hc[i] = cc[i] * a + ss[i] * c2;
+      hc[i + 1] = -ss[i] * a + cc[i] * c2;
    }
+   const double a = hc[j], c2 = hc[j + 1];
+   const double rr = sqrt(a * a + c2 * c2);
+   const double c = rr > 0.0 ? a / rr : 1.0, sn = rr > 0.0 ? c2 / rr : 0.0;
+   hc[j] = rr;
+   hc[j + 1] = 0.0;
+   for (int i = 0; i <= j + 1; i++) Hb[i + (long)j * ld] = hc[i];
+   S.cs[(long)b * m + j] = c;
+   S.sn[(long)b * m + j] = sn;
+   S.g[(long)b * (m + 1) + j + 1] = -sn * gj;
+   S.g[(long)b * (m + 1) + j] = c * gj;
+   const double res = fabs(sn * gj);
+   S.resid[2 * b] = res;
+   int* fl = S.flags + 4 * b;
+   fl[1] = j + 1;
+   const bool breakdown = !(hn > 1e-300) || !(rr > 0.0);
+   if (breakdown) fl[2] = 1;
+   if (res <= S.resid[2 * b + 1] || breakdown || j + 1 == m) fl[0] = 1;
+   S.scale[b] = hn > 0.0 ? 1.0 / hn : 0.0;
 }
 
-// y = H^-1 g per instance (one workgroup per instance)
-__global__ void __launch_bounds__(kRows) btrisolve_kernel(int B, BG S) {
+// y = H^-1 g, one workgroup per slot that took steps this cycle
+__global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) {
    const int b = blockIdx.x;
-   if (b >= B) return;
+   if (b >= nslots || S.phase[b] != kInner) return;
    const int m = S.m;
    const int k = S.flags[4 * b + 1];
    if (k == 0) return;
@@ -344,19 +480,20 @@ __global__ void __launch_bounds__(kRows) btrisolve_kernel(int B, BG S) {
       for (int i = k - 1; i >= 0; i--) {
          double s = gg[i];
          for (int q = i + 1; q < k; q++) s -= h[i + q * kBMaxM] * yy[q];
-         double d = h[i + i * kBMaxM];
-         yy[i] = d != 0.0 ? s / d : 0.0;
+         double dd = h[i + i * kBMaxM];
+         yy[i] = dd != 0.0 ? s / dd : 0.0;
       }
    }
    __syncthreads();
    if ((int)threadIdx.x < k) S.y[(long)b * m + threadIdx.x] = yy[threadIdx.x];
 }
 
-// e += B^-1 (sum_i y_i V[i]) for instances that took steps in this cycle
+// e += B^-1 (sum_i y_i V[i])
 template <class T>
 __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const T* V, const T* binv,
-                                                        BG S, T* e) {
-   const int b = seg_inst(L);
+                                                        BState S, T* e) {
+   const int b = slot_of_wg(L);
+   if (b < 0 || S.phase[b] != kInner) return;
    const int k = S.flags[4 * b + 1];
    if (k == 0) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
@@ -364,593 +501,660 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const
    const int bs = L.pbs[b];
    __shared__ T u[kRows];
    T acc = 0;
-   if (row < N)
-      for (int i = 0; i < k; i++) acc += (T)S.y[(long)b * S.m + i] * V[(long)i * N + row];
+   for (int i = 0; i < k; i++) acc += (T)S.y[(long)b * S.m + i] * V[(long)i * N + row];
    u[threadIdx.x] = acc;
    __syncthreads();
    if (local < L.nrows[b]) {
       const long cell = local / bs;
       const int gr = (int)(local - cell * bs);
       const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
-      const int first = (int)(threadIdx.x - gr);  // cells never straddle a workgroup
+      const int first = (int)(threadIdx.x - gr);  // a cell never straddles a workgroup
       T s = 0;
       for (int q = 0; q < bs; q++) s += bi[q] * u[first + q];
       e[row] += s;
    }
 }
 
-// V[0] = rhs - A e; beta; (re)arm the instance's cycle.  first != 0: e = 0,
-// the target becomes max(reduction beta, abs_target) and the step totals restart.
+// true residual V[0] = rhs - A e after the cycle, and the stop decision
 template <class T>
-__global__ void __launch_bounds__(kRows) brestart_kernel(long N, BLayout L, BSell A, const T* aval,
-                                                         const T* e, const T* rhs, T* V, int first,
-                                                         double reduction, double abs_target,
-                                                         int max_it, double* partials,
-                                                         unsigned* tickets, BG S) {
-   const int b = seg_inst(L);
-   // finished instances (converged outer loop, or stopped before this cycle
-   // took a step) keep the flags and residual they have
-   if (!first && (!S.active[b] || (S.flags[4 * b] && S.flags[4 * b + 1] == 0))) return;
+__global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, const T* aval, const T* e,
+                                                         const T* rhs, T* V0, BOpts o, double* partials,
+                                                         unsigned* tickets, BState S) {
+   const int b = slot_of_wg(L);
+   // slots not in the inner phase, or stopped before this cycle took a step,
+   // keep their flags
+   if (b < 0 || S.phase[b] != kInner || (S.flags[4 * b] && S.flags[4 * b + 1] == 0)) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   double u = 0.0;
-   if (row < N) {
-      T acc = 0;
-      if (!first) {
-         const long s = row >> 6;
-         const long base = A.off[s] + (threadIdx.x & 63);
-         const int wd = A.width[s];
-         for (int k = 0; k < wd; k++) {
-            long p = base + (long)k * 64;
-            acc += aval[p] * e[A.col[p]];
-         }
-      }
-      T r = rhs[row] - acc;
-      V[row] = r;
-      u = (double)r;
+   const long s = row >> 6;
+   const long base = A.off[s] + (threadIdx.x & 63);
+   const int wd = A.width[s];
+   T acc = 0;
+   for (int k = 0; k < wd; k++) {
+      long p = base + (long)k * 64;
+      acc += aval[p] * e[A.col[p]];
    }
+   const T rv = rhs[row] - acc;
+   V0[row] = rv;
+   const double u = (double)rv;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, S.ww + b,
        (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
-   if (last && threadIdx.x == 0) {
-      double beta = sqrt(S.ww[b]);
-      int* fl = S.flags + 4 * b;
-      if (first) {
-         S.resid[2 * b + 1] = fmax(reduction * beta, abs_target);
-         fl[3] = 0;
-      } else {
-         fl[3] += fl[1];
-      }
-      S.g[(long)b * (S.m + 1)] = beta;
-      S.resid[2 * b] = beta;
-      S.scale[b] = beta > 0.0 ? 1.0 / beta : 0.0;
-      fl[0] = (!S.active[b] || beta <= S.resid[2 * b + 1] || !(beta > 0.0) || fl[3] >= max_it) ? 1 : 0;
-      fl[1] = 0;
-      fl[2] = 0;
-   }
+   if (!last || threadIdx.x != 0) return;
+   const double beta = sqrt(S.ww[b]);
+   int* fl = S.flags + 4 * b;
+   fl[3] += fl[1];
+   S.g[(long)b * (S.m + 1)] = beta;
+   S.resid[2 * b] = beta;
+   S.scale[b] = beta > 0.0 ? 1.0 / beta : 0.0;
+   fl[0] = (beta <= S.resid[2 * b + 1] || !(beta > 0.0) || fl[3] >= o.max_inner) ? 1 : 0;
+   fl[1] = 0;
+   fl[2] = 0;
 }
 
-// ---- outer (eigen) kernels ----------------------------------------------------------
-
-// y1 = R x, y2 = F x, dots per instance
-__global__ void __launch_bounds__(kRows) brf_kernel(long N, BLayout L, BSell R, const double* rval,
-                                                     BSell F, const double* fval, const double* x,
-                                                     double* y1, double* y2, double* partials,
-                                                     unsigned* tickets, BE E) {
-   const int b = seg_inst(L);
-   if (E.conv[b]) return;
-   const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   double a = 0.0, c = 0.0, xv = 0.0;
-   if (row < N) {
-      const long s = row >> 6;
-      const int lane = threadIdx.x & 63;
-      long base = R.off[s] + lane;
-      for (int k = 0; k < R.width[s]; k++) {
-         long p = base + (long)k * 64;
-         a += rval[p] * x[R.col[p]];
-      }
-      base = F.off[s] + lane;
-      for (int k = 0; k < F.width[s]; k++) {
-         long p = base + (long)k * 64;
-         c += fval[p] * x[F.col[p]];
-      }
-      y1[row] = a;
-      y2[row] = c;
-      xv = x[row];
-   }
-   seg_reduce<3>(
-       3, [&](int k) { return k == 0 ? c * a : (k == 1 ? c * c : xv * xv); },
-       partials + (long)L.wg_first[b] * 3, tickets + b, E.dots + 3 * b, (int)blockIdx.x - L.wg_first[b],
-       L.wg_count[b]);
-}
-
-// phi = x/|x|, r = (y2/k - y1)/|x|; residual; convergence and shift decisions
-__global__ void __launch_bounds__(kRows) bresid_kernel(long N, BLayout L, const double* x,
-                                                        const double* y1, const double* y2,
-                                                        double* phi, double* r, double tol,
-                                                        double delta, double* partials,
-                                                        unsigned* tickets, BE E) {
-   const int b = seg_inst(L);
-   if (E.conv[b]) return;
-   const double* d = E.dots + 3 * b;
-   const double inv_k = d[0] / d[1];
-   const double inv_nx = 1.0 / sqrt(d[2]);
-   const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   double rr = 0.0, qq = 0.0;
-   if (row < N) {
-      double q = y2[row] * inv_k * inv_nx;
-      double v = q - y1[row] * inv_nx;
-      phi[row] = x[row] * inv_nx;
-      r[row] = v;
-      rr = v * v;
-      qq = q * q;
-   }
-   bool last = seg_reduce<2>(
-       2, [&](int k) { return k == 0 ? rr : qq; }, partials + (long)L.wg_first[b] * 2, tickets + b,
-       E.rq + 2 * b, (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
-   if (last && threadIdx.x == 0) {
-      const double k = d[1] / d[0];
-      const double res = sqrt(E.rq[2 * b] / E.rq[2 * b + 1]);
-      E.k[b] = k;
-      E.res[b] = res;
-      if (res <= tol || !(res == res)) {
-         E.conv[b] = 1;
-         E.active[b] = 0;
-      } else if (delta > 0.0) {
-         const int it = E.it[b];
-         const bool settle = it >= 2 && fabs(k - E.kprev[b]) < 1e-2 * k;
-         if (settle && (!E.shifted[b] || fabs(k + delta - E.ks[b]) > 0.2 * delta)) {
-            E.ks[b] = k + delta;
-            E.shifted[b] = 1;
-            E.rebuild[b] = 1;
-         }
-      }
-      E.kprev[b] = k;
-      E.it[b] += 1;
-   }
-}
-
+// x = phi + e for slots whose inner solve ended
 template <class T>
-__global__ void badd_kernel(long N, BLayout L, const double* phi, const T* e, const int* active,
-                            double* x) {
+__global__ void bfinish_kernel(long N, BLayout L, const double* phi, const T* e, BState S, double* x) {
    long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= N) return;
-   const int b = L.wg_inst[row / kRows];
-   x[row] = active[b] ? phi[row] + (double)e[row] : phi[row];
+   const int b = slot_of_row(L, row);
+   if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b]) return;
+   x[row] = phi[row] + (double)e[row];
 }
 
-__global__ void bfill_kernel(double* x, long N, BLayout L, double v) {
-   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (row >= N) return;
-   const int b = L.wg_inst[row / kRows];
-   x[row] = (row - L.row_off[b] < L.nrows[b]) ? v : 0.0;
+__global__ void bphase_kernel(int nslots, BState S) {
+   int b = blockIdx.x * blockDim.x + threadIdx.x;
+   if (b >= nslots || S.phase[b] != kInner || !S.flags[4 * b]) return;
+   S.inner[b] += S.flags[4 * b + 3];
+   S.phase[b] = kOuter;
 }
 
 }  // namespace pk
 
 using namespace pk;
 
-// A concatenated SELL matrix (device).
-struct CatSell {
-   long nslices = 0, entries = 0;
-   DBuf<long> off;
-   DBuf<int> width, col, rlen;
-   DBuf<double> val;
-   BSell view() const { return BSell{off.p, width.p, col.p, rlen.p}; }
-};
+namespace {
 
-template <class T>
-struct BGmresT {
-   int m = 30;
-   DBuf<T> V, z, w, e, rhs, binv, aval;
-   hipGraphExec_t graph = nullptr;
-   ~BGmresT() {
-      if (graph) (void)hipGraphExecDestroy(graph);
+// First-fit allocator of [0, cap) in whole units, with coalescing.
+struct RangeAlloc {
+   long cap = 0;
+   std::map<long, long> free_;  // start -> length
+   void reset(long c) {
+      cap = c;
+      free_.clear();
+      if (c > 0) free_[0] = c;
+   }
+   long alloc(long len) {
+      if (len <= 0) return 0;
+      for (auto it = free_.begin(); it != free_.end(); ++it) {
+         if (it->second >= len) {
+            long start = it->first, rest = it->second - len;
+            free_.erase(it);
+            if (rest > 0) free_[start + len] = rest;
+            return start;
+         }
+      }
+      return -1;
+   }
+   void release(long start, long len) {
+      if (len <= 0) return;
+      auto it = free_.emplace(start, len).first;
+      auto nx = std::next(it);
+      if (nx != free_.end() && it->first + it->second == nx->first) {
+         it->second += nx->second;
+         free_.erase(nx);
+      }
+      if (it != free_.begin()) {
+         auto pv = std::prev(it);
+         if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+         }
+      }
+   }
+   long high_water() const {  // end of the highest allocated unit
+      if (free_.empty()) return cap;
+      auto last = std::prev(free_.end());
+      return last->first + last->second == cap ? last->first : cap;
+   }
+   long largest_free() const {
+      long m = 0;
+      for (auto& kv : free_) m = std::max(m, kv.second);
+      return m;
    }
 };
 
+struct Slot {
+   bool used = false, harvested = false;
+   long n = 0;
+   int bs = 1, pbs = 1;
+   long unit0 = 0, units = 0;  // row units
+   long re0 = 0, re = 0;       // R/A entries (x64)
+   long fe0 = 0, fe = 0;       // F entries (x64)
+   long bi0 = 0, bi = 0;       // block-inverse values (x64)
+   long nnz = 0;               // stored entries of R (algorithmic count)
+   long ticket = 0;            // admission order
+};
+
+template <class T>
+struct Inner {
+   DBuf<T> V, z, w, e, rhs, binv, aval;
+};
+
+struct Graph {
+   hipGraphExec_t exec = nullptr;
+   long units = -1;
+   void clear() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+      units = -1;
+   }
+   ~Graph() { clear(); }
+};
+
+}  // namespace
+
 struct pk_batch_s {
-   int B = 0;
-   std::vector<pk_mat_s*> R, F;
-   std::vector<long> row_off;
-   std::vector<int> n, pbs, fbs, wg_first, wg_count;
-   long N = 0;
-   int nwg = 0;
-   // layout
-   DBuf<int> d_wg_inst, d_wg_first, d_wg_count, d_nrows, d_pbs, d_fbs, d_active_g;
-   DBuf<long> d_row_off, d_binv_off;
-   long binv_total = 0;
-   CatSell Rc, Fc;
-   DBuf<double> aval64;  // shifted operator values (FP64)
-   // reduction scratch of its own: captured graphs keep these addresses
+   // capacity
+   long max_units = 0, max_rentries = 0, max_fentries = 0, max_binv = 0;
+   int max_slots = 0;
+   int m = 30;
+   int precision = PK_PREC_FP64;
+   BOpts opts{1e-8, 0.1, 0.1, 10000, 2000};
+   // device arrays
+   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs;
+   DBuf<long> row_off, binv_off;
+   DBuf<long> r_off, f_off;
+   DBuf<int> r_width, f_width, r_col, f_col, rlen;
+   DBuf<double> r_val, f_val, a_val;
+   DBuf<double> x, phi, y1, y2, r;
+   DBuf<double> sd;
+   DBuf<int> si;
    DBuf<double> part;
    DBuf<unsigned> tick;
    DBuf<int> err;
-   // options
-   double tol = 1e-8;
-   int max_outer = 10000;
-   int precision = PK_PREC_FP64;
-   int restart = 30;
-   double reduction = 0.1;
-   int max_inner = 2000;
-   double delta = 0.1;
-   // state
-   DBuf<double> x, phi, y1, y2, r;
-   DBuf<double> st_d;  // H, cs, sn, g, y, scale, resid, ww, dots, rq, k, kprev, ks, res
-   DBuf<int> st_i;     // flags, it, conv, active, rebuild, shifted
-   BG gs{};
-   BE es{};
-   BGmresT<double> g64;
-   BGmresT<float> g32;
-   bool ready = false;
-   std::vector<char> has_init;
-   DBuf<double> init;
-   // stats
-   std::vector<int> outer;
-   std::vector<long> inner;
-   std::vector<double> keff, resid;
+   Inner<double> i64;
+   Inner<float> i32;
+   BState S{};
+   // host bookkeeping
+   RangeAlloc ua, ra, fa, ba;
+   std::vector<Slot> slots;
+   std::vector<int> h_unit_slot;
+   std::vector<int> h_phase;
+   long admitted = 0;
+   Graph graph;
+   double tick_seconds = 0.0;
+   long ticks = 0;
+   // fixed-batch use (pk_batch_create): members and optional initial vectors
+   std::vector<pk_mat_s*> fixR, fixF;
+   std::vector<DBuf<double>> fix_init;
+   std::vector<char> fix_has_init;
+   std::vector<int> fix_slot;
    double seconds = 0.0;
 
    BLayout layout() const {
-      return BLayout{d_wg_inst.p, d_wg_first.p, d_wg_count.p, d_row_off.p,
-                     d_nrows.p,   d_pbs.p,      d_fbs.p,      d_binv_off.p};
+      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p};
    }
+   BSell rsell() const { return BSell{r_off.p, r_width.p, r_col.p, rlen.p}; }
+   BSell fsell() const { return BSell{f_off.p, f_width.p, f_col.p, rlen.p}; }
 };
 
-static int cat_sell(pk_batch_s* Bt, const std::vector<pk_mat_s*>& mats, CatSell& out) {
-   Ctx& c = ctx();
-   const int B = Bt->B;
-   long total_slices = Bt->N / 64;
-   std::vector<long> base(B + 1, 0);
-   for (int b = 0; b < B; b++) base[b + 1] = base[b] + mats[b]->sell.padded;
-   out.nslices = total_slices;
-   out.entries = base[B];
-   PK_REQUIRE(!out.off.alloc(total_slices + 1) && !out.width.alloc(total_slices) &&
-                  !out.col.alloc(out.entries ? out.entries : 1) && !out.val.alloc(out.entries ? out.entries : 1) &&
-                  !out.rlen.alloc(Bt->N),
-              "device allocation failed");
-   for (int b = 0; b < B; b++) {
-      const SellData& S = mats[b]->sell;
-      long ns_dst = (b + 1 < B ? Bt->row_off[b + 1] : Bt->N) / 64 - Bt->row_off[b] / 64;
-      cat_slices_kernel<<<grid_for(ns_dst), kBlock, 0, c.stream>>>(S.nslices, ns_dst, S.off.p, S.width.p,
-                                                                  Bt->row_off[b] / 64, base[b], out.off.p,
-                                                                  out.width.p);
-      if (S.padded) {
-         cat_entries_kernel<<<grid_for(S.padded), kBlock, 0, c.stream>>>(S.padded, S.col.p,
-                                                                        (int)Bt->row_off[b],
-                                                                        out.col.p + base[b]);
-         PK_HIP(hipMemcpyAsync(out.val.p + base[b], S.val.p, sizeof(double) * S.padded,
-                               hipMemcpyDeviceToDevice, c.stream));
-      }
-      long n_dst = (b + 1 < B ? Bt->row_off[b + 1] : Bt->N) - Bt->row_off[b];
-      cat_rlen_kernel<<<grid_for(n_dst), kBlock, 0, c.stream>>>(mats[b]->n, n_dst, S.rlen.p,
-                                                               out.rlen.p + Bt->row_off[b]);
-   }
-   set_last_off_kernel<<<1, 1, 0, c.stream>>>(out.off.p, total_slices, out.entries);
-   PK_HIP(hipGetLastError());
-   return 0;
+static int preconditioner_block(int bs) {
+   // largest power of two <= 16 dividing the cell block, or the block itself
+   // when 256-row workgroups hold whole cells of it
+   int p = 1;
+   while (p * 2 <= kBMaxBs && bs % (p * 2) == 0) p *= 2;
+   if (bs <= kBMaxBs && kRows % bs == 0) p = bs;
+   return p;
 }
 
-template <class T>
-static BGmresT<T>& gm(pk_batch_s* Bt);
-template <>
-BGmresT<double>& gm<double>(pk_batch_s* Bt) {
-   return Bt->g64;
-}
-template <>
-BGmresT<float>& gm<float>(pk_batch_s* Bt) {
-   return Bt->g32;
+static inline long units_of(long n) { return std::max(1L, (n + kRows - 1) / kRows); }
+static inline long blocks64(long n) { return (n + 63) / 64; }
+
+// device bytes of a pool of the given capacity (units of 256 rows, entries and
+// block-inverse values in units of 64)
+static double pool_bytes(long units, long rent, long fent, long binv, int slots, int m, int precision) {
+   const double rows = (double)units * kRows, sl = rows / 64.0;
+   const double t = precision == PK_PREC_MIXED ? 4.0 : 8.0;
+   double b = 0.0;
+   b += units * 4.0 + sl * 2 * (8.0 + 4.0) + rows * 4.0;        // unit map, slice tables, rlen
+   b += rent * 64.0 * (4.0 + 8.0 + 8.0) + fent * 64.0 * 12.0;    // R cols+vals, A (FP64), F
+   b += rows * 8.0 * 5;                                          // x, phi, y1, y2, r
+   b += rows * t * (m + 1 + 4) + binv * 64.0 * t;                // basis, z w e rhs, block inverses
+   if (precision == PK_PREC_MIXED) b += rent * 64.0 * t;         // A copy in FP32
+   b += (double)units * (m + 4) * 8.0;                           // reduction partials
+   b += (double)slots * ((m + 1.0) * m + 4.0 * m + 24.0) * 8.0;  // per-slot state
+   return b;
 }
 
-static int batch_setup(pk_batch_s* Bt) {
-   if (Bt->ready) return 0;
+static int pool_alloc(pk_batch_s* P) {
    Ctx& c = ctx();
-   const int B = Bt->B;
-   const int m = Bt->restart;
-   // layout
-   Bt->row_off.assign(B, 0);
-   Bt->wg_first.assign(B, 0);
-   Bt->wg_count.assign(B, 0);
-   long rows = 0;
-   std::vector<int> wg_inst;
-   std::vector<long> binv_off(B, 0);
-   long binv = 0;
-   for (int b = 0; b < B; b++) {
-      Bt->row_off[b] = rows;
-      long padded = ((long)Bt->n[b] + kRows - 1) / kRows * kRows;
-      if (padded == 0) padded = kRows;
-      Bt->wg_first[b] = (int)(rows / kRows);
-      Bt->wg_count[b] = (int)(padded / kRows);
-      for (long w = 0; w < padded / kRows; w++) wg_inst.push_back(b);
-      rows += padded;
-      binv_off[b] = binv;
-      binv += (long)Bt->n[b] * Bt->pbs[b];
-   }
-   Bt->N = rows;
-   Bt->nwg = (int)wg_inst.size();
-   Bt->binv_total = binv;
-   PK_REQUIRE(rows < (1L << 31), "batch too large for 32-bit columns");
-   auto up_i = [&](DBuf<int>& d, const std::vector<int>& h) {
-      return d.alloc(h.size()) || hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice);
+   const long units = P->max_units, rows = units * kRows, sl = rows / 64;
+   const int S = P->max_slots, m = P->m;
+   const long rent = P->max_rentries * 64, fent = P->max_fentries * 64, binv = P->max_binv * 64;
+   bool bad = P->unit_slot.alloc(units) || P->wg_first.alloc(S) || P->wg_count.alloc(S) || P->nrows.alloc(S) ||
+              P->pbs.alloc(S) || P->fbs.alloc(S) || P->row_off.alloc(S) || P->binv_off.alloc(S) ||
+              P->r_off.alloc(sl + 1) || P->f_off.alloc(sl + 1) || P->r_width.alloc(sl) || P->f_width.alloc(sl) ||
+              P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
+              P->f_col.alloc(fent) || P->f_val.alloc(fent) || P->x.alloc(rows) || P->phi.alloc(rows) ||
+              P->y1.alloc(rows) || P->y2.alloc(rows) || P->r.alloc(rows) ||
+              P->part.alloc((size_t)units * (m + 4)) || P->tick.alloc(S) || P->err.alloc(1);
+   PK_REQUIRE(!bad, "device allocation failed for the eigen pool");
+   auto alloc_inner = [&](auto& I, bool own_a) {
+      return I.V.alloc((size_t)(m + 1) * rows) || I.z.alloc(rows) || I.w.alloc(rows) || I.e.alloc(rows) ||
+             I.rhs.alloc(rows) || I.binv.alloc(binv) || (own_a && I.aval.alloc(rent));
    };
-   auto up_l = [&](DBuf<long>& d, const std::vector<long>& h) {
-      return d.alloc(h.size()) || hipMemcpy(d.p, h.data(), sizeof(long) * h.size(), hipMemcpyHostToDevice);
-   };
-   PK_REQUIRE(!up_i(Bt->d_wg_inst, wg_inst) && !up_i(Bt->d_wg_first, Bt->wg_first) &&
-                  !up_i(Bt->d_wg_count, Bt->wg_count) && !up_i(Bt->d_nrows, Bt->n) &&
-                  !up_i(Bt->d_pbs, Bt->pbs) && !up_i(Bt->d_fbs, Bt->fbs) && !up_l(Bt->d_row_off, Bt->row_off) &&
-                  !up_l(Bt->d_binv_off, binv_off),
-              "device allocation failed");
-   PK_CALL(cat_sell(Bt, Bt->R, Bt->Rc));
-   PK_CALL(cat_sell(Bt, Bt->F, Bt->Fc));
-   PK_REQUIRE(!Bt->aval64.alloc(Bt->Rc.entries ? Bt->Rc.entries : 1), "device allocation failed");
-   const long N = Bt->N;
-   PK_REQUIRE(!Bt->x.alloc(N) && !Bt->phi.alloc(N) && !Bt->y1.alloc(N) && !Bt->y2.alloc(N) && !Bt->r.alloc(N),
-              "device allocation failed");
-   // per-instance state
-   const size_t nd = (size_t)B * ((size_t)(m + 1) * m + 2 * m + (m + 1) + m + 1 + 2 + 1 + 3 + 2 + 4);
-   PK_REQUIRE(!Bt->st_d.alloc(nd) && !Bt->st_i.alloc((size_t)B * 10), "device allocation failed");
-   PK_HIP(hipMemsetAsync(Bt->st_d.p, 0, sizeof(double) * nd, c.stream));
-   PK_HIP(hipMemsetAsync(Bt->st_i.p, 0, sizeof(int) * B * 10, c.stream));
-   double* p = Bt->st_d.p;
-   BG& g = Bt->gs;
-   g.m = m;
-   g.H = p;
-   p += (size_t)B * (m + 1) * m;
-   g.cs = p;
-   p += (size_t)B * m;
-   g.sn = p;
-   p += (size_t)B * m;
-   g.g = p;
-   p += (size_t)B * (m + 1);
-   g.y = p;
-   p += (size_t)B * m;
-   g.scale = p;
-   p += B;
-   g.resid = p;
-   p += 2 * (size_t)B;
-   g.ww = p;
-   p += B;
-   BE& e = Bt->es;
-   e.dots = p;
-   p += 3 * (size_t)B;
-   e.rq = p;
-   p += 2 * (size_t)B;
-   e.k = p;
-   p += B;
-   e.kprev = p;
-   p += B;
-   e.ks = p;
-   p += B;
-   e.res = p;
-   p += B;
-   int* q = Bt->st_i.p;
-   g.flags = q;
-   q += 4 * B;
-   e.it = q;
-   q += B;
-   e.conv = q;
-   q += B;
-   e.active = q;
-   q += B;
-   e.rebuild = q;
-   q += B;
-   e.shifted = q;
-   q += B;
-   g.active = e.active;
-   // GMRES buffers
-   auto alloc_g = [&](auto& G) {
-      return G.V.alloc((size_t)(m + 1) * N) || G.z.alloc(N) || G.w.alloc(N) || G.e.alloc(N) ||
-             G.rhs.alloc(N) || G.binv.alloc(Bt->binv_total ? Bt->binv_total : 1) ||
-             G.aval.alloc(Bt->Rc.entries ? Bt->Rc.entries : 1);
-   };
-   if (Bt->precision == PK_PREC_MIXED) {
-      PK_REQUIRE(!alloc_g(Bt->g32), "device allocation failed");
-      Bt->g32.m = m;
-   } else {
-      PK_REQUIRE(!alloc_g(Bt->g64), "device allocation failed");
-      Bt->g64.m = m;
-   }
-   PK_HIP(hipMemsetAsync(Bt->x.p, 0, sizeof(double) * N, c.stream));
-   PK_REQUIRE(!Bt->part.alloc((size_t)Bt->nwg * (m + 4)) && !Bt->tick.alloc(B), "device allocation failed");
-   PK_HIP(hipMemsetAsync(Bt->tick.p, 0, sizeof(unsigned) * B, c.stream));
-   PK_HIP(hipStreamSynchronize(c.stream));
-   Bt->ready = true;
-   return 0;
-}
-
-// A values and block inverses for the instances flagged for rebuild (or all)
-template <class T>
-static int batch_rebuild(pk_batch_s* Bt, int force) {
-   Ctx& c = ctx();
-   const BLayout L = Bt->layout();
-   const long N = Bt->N;
-   bshift_kernel<<<grid_for(N), kBlock, 0, c.stream>>>(N, L, Bt->Rc.view(), Bt->Rc.val.p, Bt->Fc.view(),
-                                                      Bt->Fc.val.p, Bt->es.ks, Bt->es.rebuild, force,
-                                                      Bt->aval64.p);
-   auto& G = gm<T>(Bt);
-   if (sizeof(T) == sizeof(double))
-      PK_HIP(hipMemcpyAsync(G.aval.p, Bt->aval64.p, sizeof(double) * Bt->Rc.entries, hipMemcpyDeviceToDevice,
-                            c.stream));
+   if (P->precision == PK_PREC_MIXED)
+      PK_REQUIRE(!alloc_inner(P->i32, true), "device allocation failed for the eigen pool");
    else
-      to_t_kernel<T><<<grid_for(Bt->Rc.entries), kBlock, 0, c.stream>>>(Bt->aval64.p, G.aval.p, Bt->Rc.entries);
-   DBuf<int>& err = Bt->err;
-   if (!err.p) PK_REQUIRE(!err.alloc(1), "device allocation failed");
-   PK_HIP(hipMemsetAsync(err.p, 0, sizeof(int), c.stream));
-   bblock_inverse_kernel<T><<<grid_for(N, 64), 64, 0, c.stream>>>(N, L, Bt->Rc.view(), Bt->aval64.p,
-                                                                 Bt->es.rebuild, force, G.binv.p, err.p);
-   PK_HIP(hipGetLastError());
-   int h = 0;
-   PK_HIP(hipMemcpyAsync(&h, err.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-   PK_HIP(hipMemsetAsync(Bt->es.rebuild, 0, sizeof(int) * Bt->B, c.stream));
+      PK_REQUIRE(!alloc_inner(P->i64, false), "device allocation failed for the eigen pool");
+   // per-slot state
+   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 4 * (size_t)m + 24);
+   const size_t ni = (size_t)S * 12;
+   PK_REQUIRE(!P->sd.alloc(nd) && !P->si.alloc(ni), "device allocation failed for the eigen pool");
+   PK_HIP(hipMemsetAsync(P->sd.p, 0, sizeof(double) * nd, c.stream));
+   PK_HIP(hipMemsetAsync(P->si.p, 0, sizeof(int) * ni, c.stream));
+   double* d = P->sd.p;
+   BState& B = P->S;
+   B.m = m;
+   auto take = [&](size_t k) {
+      double* p = d;
+      d += k;
+      return p;
+   };
+   B.H = take((size_t)S * (m + 1) * m);
+   B.cs = take((size_t)S * m);
+   B.sn = take((size_t)S * m);
+   B.g = take((size_t)S * (m + 1));
+   B.y = take((size_t)S * m);
+   B.scale = take(S);
+   B.resid = take(2 * (size_t)S);
+   B.ww = take(S);
+   B.dots = take(3 * (size_t)S);
+   B.rq = take(2 * (size_t)S);
+   B.k = take(S);
+   B.kprev = take(S);
+   B.ks = take(S);
+   B.res = take(S);
+   int* q = P->si.p;
+   auto takei = [&](size_t k) {
+      int* p = q;
+      q += k;
+      return p;
+   };
+   B.flags = takei(4 * (size_t)S);
+   B.phase = takei(S);
+   B.start = takei(S);
+   B.rebuild = takei(S);
+   B.shifted = takei(S);
+   B.it = takei(S);
+   B.inner = takei(S);
+   B.status = takei(S);
+   // tables: every unit free, every slice empty
+   P->h_unit_slot.assign(units, -1);
+   PK_HIP(hipMemcpyAsync(P->unit_slot.p, P->h_unit_slot.data(), sizeof(int) * units, hipMemcpyHostToDevice,
+                         c.stream));
+   PK_HIP(hipMemsetAsync(P->r_width.p, 0, sizeof(int) * sl, c.stream));
+   PK_HIP(hipMemsetAsync(P->f_width.p, 0, sizeof(int) * sl, c.stream));
+   PK_HIP(hipMemsetAsync(P->r_off.p, 0, sizeof(long) * (sl + 1), c.stream));
+   PK_HIP(hipMemsetAsync(P->f_off.p, 0, sizeof(long) * (sl + 1), c.stream));
+   PK_HIP(hipMemsetAsync(P->rlen.p, 0, sizeof(int) * rows, c.stream));
+   PK_HIP(hipMemsetAsync(P->x.p, 0, sizeof(double) * rows, c.stream));
+   PK_HIP(hipMemsetAsync(P->phi.p, 0, sizeof(double) * rows, c.stream));
+   PK_HIP(hipMemsetAsync(P->tick.p, 0, sizeof(unsigned) * S, c.stream));
+   PK_HIP(hipMemsetAsync(P->err.p, 0, sizeof(int), c.stream));
+   auto zero_inner = [&](auto& I, size_t t) {
+      return hipMemsetAsync(I.V.p, 0, t * (m + 1) * rows, c.stream) ||
+             hipMemsetAsync(I.z.p, 0, t * rows, c.stream) || hipMemsetAsync(I.w.p, 0, t * rows, c.stream) ||
+             hipMemsetAsync(I.e.p, 0, t * rows, c.stream) || hipMemsetAsync(I.rhs.p, 0, t * rows, c.stream);
+   };
+   if (P->precision == PK_PREC_MIXED)
+      PK_REQUIRE(!zero_inner(P->i32, 4), "clearing the pool failed");
+   else
+      PK_REQUIRE(!zero_inner(P->i64, 8), "clearing the pool failed");
+   P->ua.reset(units);
+   P->ra.reset(P->max_rentries);
+   P->fa.reset(P->max_fentries);
+   P->ba.reset(P->max_binv);
+   P->slots.assign(S, Slot());
+   P->h_phase.assign(S, 0);
    PK_HIP(hipStreamSynchronize(c.stream));
-   PK_REQUIRE(h == 0, "singular diagonal block in the block-Jacobi preconditioner");
    return 0;
 }
 
-// one restart cycle (m steps + update + true residual) on the stream
-template <class T>
-static void launch_cycle(pk_batch_s* Bt) {
-   Ctx& c = ctx();
-   auto& G = gm<T>(Bt);
-   const BLayout L = Bt->layout();
-   const long N = Bt->N;
-   const int nwg = Bt->nwg, m = G.m;
-   for (int j = 0; j < m; j++) {
-      bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, G.V.p + (size_t)j * N, G.binv.p, G.z.p, Bt->gs);
-      bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Bt->Rc.view(), G.aval.p, G.z.p, G.w.p, G.V.p, j,
-                                                       Bt->part.p, Bt->tick.p, Bt->gs);
-      borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, G.w.p, G.V.p, j, Bt->part.p, Bt->tick.p, Bt->gs);
-   }
-   btrisolve_kernel<<<Bt->B, kRows, 0, c.stream>>>(Bt->B, Bt->gs);
-   bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, G.V.p, G.binv.p, Bt->gs, G.e.p);
-   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Bt->Rc.view(), G.aval.p, G.e.p, G.rhs.p, G.V.p, 0,
-                                                   Bt->reduction, 0.0, Bt->max_inner, Bt->part.p, Bt->tick.p,
-                                                   Bt->gs);
+// ---- admission / release ---------------------------------------------------------------
+
+static int check_member(pk_mat_s* R, pk_mat_s* F) {
+   PK_REQUIRE(R && F && R->assembled && F->assembled, "matrices not assembled");
+   PK_REQUIRE(R->n == F->n && R->bs == F->bs, "R and F differ in size or block size");
+   PK_REQUIRE(F->nnz == R->n * (long)R->bs, "F must hold one full cell block per row");
+   PK_REQUIRE(R->n > 0, "empty problem");
+   return 0;
 }
 
-// Profiled restart cycle: every kernel bracketed by events, and only steps in
-// which some instance still works are issued.  Algorithmic bytes count the
-// instances that run the step: prec reads V[j] and the block inverses and
-// writes z and V[j]; spmv_dot is a SELL SpMV (values + columns, gathered z,
-// w written) plus j+1 basis vectors; orth reads w and j+1 basis vectors and
-// writes V[j+1].
-template <class T>
-static int launch_cycle_profiled(pk_batch_s* Bt) {
+// 0: admitted into *slot; 2: no room now (nothing changed); 1: error
+static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_dev, int* slot) {
    Ctx& c = ctx();
-   auto& G = gm<T>(Bt);
-   const BLayout L = Bt->layout();
-   const long N = Bt->N;
-   const int nwg = Bt->nwg, m = G.m, B = Bt->B;
+   PK_CALL(check_member(R, F));
+   const long n = R->n;
+   const int bs = R->bs, pbs = preconditioner_block(bs);
+   const long units = units_of(n);
+   const long re = blocks64(R->sell.padded), fe = blocks64(F->sell.padded), bi = blocks64(n * pbs);
+   PK_REQUIRE(units <= P->max_units && re <= P->max_rentries && fe <= P->max_fentries && bi <= P->max_binv,
+              "problem larger than the eigen pool");
+   int sidx = -1;
+   for (int s = 0; s < P->max_slots; s++)
+      if (!P->slots[s].used) {
+         sidx = s;
+         break;
+      }
+   if (sidx < 0) return 2;
+   long u0 = P->ua.alloc(units);
+   if (u0 < 0) return 2;
+   long r0 = P->ra.alloc(re);
+   if (r0 < 0) {
+      P->ua.release(u0, units);
+      return 2;
+   }
+   long f0 = P->fa.alloc(fe);
+   if (f0 < 0) {
+      P->ua.release(u0, units);
+      P->ra.release(r0, re);
+      return 2;
+   }
+   long b0 = P->ba.alloc(bi);
+   if (b0 < 0) {
+      P->ua.release(u0, units);
+      P->ra.release(r0, re);
+      P->fa.release(f0, fe);
+      return 2;
+   }
+   Slot& S = P->slots[sidx];
+   S = Slot();
+   S.used = true;
+   S.n = n;
+   S.bs = bs;
+   S.pbs = pbs;
+   S.unit0 = u0;
+   S.units = units;
+   S.re0 = r0;
+   S.re = re;
+   S.fe0 = f0;
+   S.fe = fe;
+   S.bi0 = b0;
+   S.bi = bi;
+   S.nnz = R->sell.nnz;
+   S.ticket = P->admitted++;
+   const long row0 = u0 * kRows, padded = units * kRows;
+   const long slice0 = row0 / 64, nsl = padded / 64;
+   // SELL arrays of R and F, columns moved by the row offset
+   place_slices_kernel<<<grid_for(nsl), kBlock, 0, c.stream>>>(R->sell.nslices, nsl, R->sell.off.p,
+                                                              R->sell.width.p, slice0, r0 * 64, P->r_off.p,
+                                                              P->r_width.p);
+   place_slices_kernel<<<grid_for(nsl), kBlock, 0, c.stream>>>(F->sell.nslices, nsl, F->sell.off.p,
+                                                              F->sell.width.p, slice0, f0 * 64, P->f_off.p,
+                                                              P->f_width.p);
+   if (R->sell.padded) {
+      place_cols_kernel<<<grid_for(R->sell.padded), kBlock, 0, c.stream>>>(R->sell.padded, R->sell.col.p,
+                                                                          (int)row0, P->r_col.p + r0 * 64);
+      PK_HIP(hipMemcpyAsync(P->r_val.p + r0 * 64, R->sell.val.p, sizeof(double) * R->sell.padded,
+                            hipMemcpyDeviceToDevice, c.stream));
+   }
+   if (F->sell.padded) {
+      place_cols_kernel<<<grid_for(F->sell.padded), kBlock, 0, c.stream>>>(F->sell.padded, F->sell.col.p,
+                                                                          (int)row0, P->f_col.p + f0 * 64);
+      PK_HIP(hipMemcpyAsync(P->f_val.p + f0 * 64, F->sell.val.p, sizeof(double) * F->sell.padded,
+                            hipMemcpyDeviceToDevice, c.stream));
+   }
+   place_rlen_kernel<<<grid_for(padded), kBlock, 0, c.stream>>>(n, padded, R->sell.rlen.p, P->rlen.p + row0);
+   place_x_kernel<<<grid_for(padded), kBlock, 0, c.stream>>>(n, padded, x0_dev, P->x.p + row0);
+   PK_HIP(hipGetLastError());
+   // slot tables
+   const int i_first = (int)u0, i_count = (int)units, i_n = (int)n, i_bs = bs, i_pbs = pbs;
+   const long l_row0 = row0, l_b0 = b0 * 64;
+   PK_HIP(hipMemcpyAsync(P->wg_first.p + sidx, &i_first, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->wg_count.p + sidx, &i_count, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->nrows.p + sidx, &i_n, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->fbs.p + sidx, &i_bs, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->pbs.p + sidx, &i_pbs, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->row_off.p + sidx, &l_row0, sizeof(long), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->binv_off.p + sidx, &l_b0, sizeof(long), hipMemcpyHostToDevice, c.stream));
+   // state: OUTER, rebuild (A = R, block inverses) on the first tick
+   const int zi[4] = {0, 0, 0, 0};
+   const int one = 1, outer = kOuter;
+   const double zd = 0.0;
+   PK_HIP(hipMemcpyAsync(P->S.flags + 4 * sidx, zi, sizeof(int) * 4, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.start + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.shifted + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.it + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.inner + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.status + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.ks + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.kprev + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.rebuild + sidx, &one, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.phase + sidx, &outer, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   // the slot's units become live last
+   for (long u = u0; u < u0 + units; u++) P->h_unit_slot[u] = sidx;
+   PK_HIP(hipMemcpyAsync(P->unit_slot.p + u0, P->h_unit_slot.data() + u0, sizeof(int) * units,
+                         hipMemcpyHostToDevice, c.stream));
+   // the host values above live on this stack frame: finish the copies here
+   PK_HIP(hipStreamSynchronize(c.stream));
+   P->h_phase[sidx] = kOuter;
+   *slot = sidx;
+   return 0;
+}
+
+static int pool_release(pk_batch_s* P, int sidx) {
+   Ctx& c = ctx();
+   PK_REQUIRE(sidx >= 0 && sidx < P->max_slots && P->slots[sidx].used, "no such slot");
+   Slot& S = P->slots[sidx];
+   for (long u = S.unit0; u < S.unit0 + S.units; u++) P->h_unit_slot[u] = -1;
+   const int fr = kFree;
+   PK_HIP(hipMemcpyAsync(P->unit_slot.p + S.unit0, P->h_unit_slot.data() + S.unit0, sizeof(int) * S.units,
+                         hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.phase + sidx, &fr, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   P->ua.release(S.unit0, S.units);
+   P->ra.release(S.re0, S.re);
+   P->fa.release(S.fe0, S.fe);
+   P->ba.release(S.bi0, S.bi);
+   S = Slot();
+   P->h_phase[sidx] = kFree;
+   return 0;
+}
+
+// ---- one tick ------------------------------------------------------------------------------
+
+template <class T>
+static Inner<T>& inner_of(pk_batch_s* P);
+template <>
+Inner<double>& inner_of<double>(pk_batch_s* P) {
+   return P->i64;
+}
+template <>
+Inner<float>& inner_of<float>(pk_batch_s* P) {
+   return P->i32;
+}
+
+// Launches one tick on the stream for row units [0, units).  With `prof`,
+// GMRES kernels are bracketed by profiler events and the cycle stops issuing
+// steps once no slot is still iterating (the arithmetic is the same).
+template <class T>
+static int launch_tick(pk_batch_s* P, long units, bool prof) {
+   Ctx& c = ctx();
+   Inner<T>& I = inner_of<T>(P);
+   const BLayout L = P->layout();
+   const BSell Rs = P->rsell(), Fs = P->fsell();
+   const long N = P->max_units * kRows;  // vector stride of the basis
+   const long rows = units * kRows;
+   const int nwg = (int)units, m = P->m;
+   const int ns = P->max_slots;
+   const BOpts o = P->opts;
+   const BState S = P->S;
+   double* part = P->part.p;
+   unsigned* tk = P->tick.p;
+   const bool fp32 = sizeof(T) != sizeof(double);
+   const T* aval = fp32 ? I.aval.p : (const T*)P->a_val.p;
+   brf_kernel<<<nwg, kRows, 0, c.stream>>>(L, Rs, P->r_val.p, Fs, P->f_val.p, P->x.p, P->y1.p, P->y2.p, part, tk,
+                                           S);
+   bresid_kernel<<<nwg, kRows, 0, c.stream>>>(L, P->x.p, P->y1.p, P->y2.p, P->phi.p, P->r.p, o, part, tk, S);
+   bshift_kernel<<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, Rs, P->r_val.p, Fs, P->f_val.p, S,
+                                                          P->a_val.p);
+   if (fp32)
+      to_t_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, S.rebuild, P->a_val.p, I.aval.p, Rs);
+   bblock_inverse_kernel<T><<<grid_for(rows, 64), 64, 0, c.stream>>>(rows, L, Rs, P->a_val.p, S.rebuild,
+                                                                     I.binv.p, P->err.p);
+   bclear_rebuild_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
+   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->r.p, I.rhs.p, I.e.p, I.V.p, o, part, tk, S);
    const double s = sizeof(T);
-   std::vector<int> fl(4 * (size_t)B);
+   std::vector<int> hf, ph;
    for (int j = 0; j < m; j++) {
-      PK_HIP(hipMemcpyAsync(fl.data(), Bt->gs.flags, sizeof(int) * 4 * B, hipMemcpyDeviceToHost, c.stream));
-      PK_HIP(hipStreamSynchronize(c.stream));
-      double n = 0.0, nnz = 0.0, bsn = 0.0;
-      for (int b = 0; b < B; b++)
-         if (!fl[4 * b]) {
-            n += Bt->n[b];
-            nnz += (double)Bt->R[b]->nnz;
-            bsn += (double)Bt->n[b] * Bt->pbs[b];
-         }
-      if (n == 0.0) break;
-      prof_begin("gmres_prec", n * s * 3.0 + bsn * s);
-      bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, G.V.p + (size_t)j * N, G.binv.p, G.z.p, Bt->gs);
-      prof_end();
-      prof_begin("gmres_spmv_dot", nnz * (s + 4.0) + (n + B) * 4.0 + 2.0 * n * s + (j + 1.0) * n * s);
-      bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Bt->Rc.view(), G.aval.p, G.z.p, G.w.p, G.V.p, j,
-                                                       Bt->part.p, Bt->tick.p, Bt->gs);
-      prof_end();
-      prof_begin("gmres_orth", n * s * (j + 3.0));
-      borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, G.w.p, G.V.p, j, Bt->part.p, Bt->tick.p, Bt->gs);
-      prof_end();
+      if (prof) {
+         // issue only steps in which some slot iterates; bytes of those slots
+         hf.resize(4 * (size_t)ns);
+         ph.resize(ns);
+         PK_HIP(hipMemcpyAsync(hf.data(), S.flags, sizeof(int) * 4 * ns, hipMemcpyDeviceToHost, c.stream));
+         PK_HIP(hipMemcpyAsync(ph.data(), S.phase, sizeof(int) * ns, hipMemcpyDeviceToHost, c.stream));
+         PK_HIP(hipStreamSynchronize(c.stream));
+         double n = 0.0, nnz = 0.0, bsn = 0.0;
+         for (int b = 0; b < ns; b++)
+            if (P->slots[b].used && ph[b] == kInner && !hf[4 * b]) {
+               n += P->slots[b].n;
+               nnz += (double)P->slots[b].nnz;
+               bsn += (double)P->slots[b].n * P->slots[b].pbs;
+            }
+         if (n == 0.0) break;
+         // prec reads V[j] and the block inverses, writes z and V[j]; spmv_dot is
+         // an SpMV (value and column of every stored entry, z gathered once, w
+         // written) plus j+1 basis vectors; orth reads w and j+1 basis vectors and
+         // writes V[j+1]
+         prof_begin("gmres_prec", n * s * 3.0 + bsn * s);
+         bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S);
+         prof_end();
+         prof_begin("gmres_spmv_dot", nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s);
+         bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, S);
+         prof_end();
+         prof_begin("gmres_orth", n * s * (j + 3.0));
+         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, j, part, tk, S);
+         prof_end();
+      } else {
+         bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S);
+         bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, S);
+         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, j, part, tk, S);
+      }
    }
-   btrisolve_kernel<<<Bt->B, kRows, 0, c.stream>>>(Bt->B, Bt->gs);
-   bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, G.V.p, G.binv.p, Bt->gs, G.e.p);
-   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Bt->Rc.view(), G.aval.p, G.e.p, G.rhs.p, G.V.p, 0,
-                                                   Bt->reduction, 0.0, Bt->max_inner, Bt->part.p, Bt->tick.p,
-                                                   Bt->gs);
+   btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
+   bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p);
+   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, o, part, tk, S);
+   bfinish_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, P->phi.p, I.e.p, S, P->x.p);
+   bphase_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
    PK_HIP(hipGetLastError());
    return 0;
 }
 
-// GMRES on A e = r (r = Bt->r, FP64) for every active instance
 template <class T>
-static int batch_gmres(pk_batch_s* Bt, std::vector<int>& hflags) {
+static int run_tick(pk_batch_s* P, long units) {
    Ctx& c = ctx();
-   auto& G = gm<T>(Bt);
-   const BLayout L = Bt->layout();
-   const long N = Bt->N;
-   const int B = Bt->B;
-   to_t_kernel<T><<<grid_for(N), kBlock, 0, c.stream>>>(Bt->r.p, G.rhs.p, N);
-   PK_HIP(hipMemsetAsync(G.e.p, 0, sizeof(T) * N, c.stream));
-   brestart_kernel<T><<<Bt->nwg, kRows, 0, c.stream>>>(N, L, Bt->Rc.view(), G.aval.p, G.e.p, G.rhs.p, G.V.p, 1,
-                                                       Bt->reduction, 0.0, Bt->max_inner, Bt->part.p,
-                                                       Bt->tick.p, Bt->gs);
-   PK_HIP(hipGetLastError());
-   if (!G.graph && !profiling()) {
-      // capture one restart cycle; its kernels read every varying quantity
-      // from device memory, so the graph is reused for all cycles and solves
+   if (profiling()) return launch_tick<T>(P, units, true);
+   if (P->graph.exec == nullptr || P->graph.units != units) {
+      // every quantity that varies between ticks is read from device memory,
+      // so one graph serves every tick for this grid size
+      P->graph.clear();
       hipGraph_t g = nullptr;
       PK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      launch_cycle<T>(Bt);
-      PK_HIP(hipStreamEndCapture(c.stream, &g));
-      PK_HIP(hipGraphInstantiate(&G.graph, g, nullptr, nullptr, 0));
+      int rc = launch_tick<T>(P, units, false);
+      hipError_t e = hipStreamEndCapture(c.stream, &g);
+      if (rc) {
+         if (g) (void)hipGraphDestroy(g);
+         return rc;
+      }
+      PK_HIP(e);
+      hipError_t ie = hipGraphInstantiate(&P->graph.exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
+      PK_HIP(ie);
+      P->graph.units = units;
    }
-   hflags.assign(4 * (size_t)B, 0);
-   for (int cyc = 0; cyc < 100000; cyc++) {
-      PK_HIP(hipMemcpyAsync(hflags.data(), Bt->gs.flags, sizeof(int) * 4 * B, hipMemcpyDeviceToHost, c.stream));
-      PK_HIP(hipStreamSynchronize(c.stream));
-      bool all = true;
-      for (int b = 0; b < B; b++) all &= hflags[4 * b] != 0;
-      if (all) break;
-      if (profiling())
-         PK_CALL(launch_cycle_profiled<T>(Bt));
-      else if (G.graph)
-         PK_HIP(hipGraphLaunch(G.graph, c.stream));
-      else
-         launch_cycle<T>(Bt);
-      PK_HIP(hipGetLastError());
-   }
+   PK_HIP(hipGraphLaunch(P->graph.exec, c.stream));
    return 0;
 }
 
-template <class T>
-static int batch_solve_t(pk_batch_s* Bt) {
+// Runs `ticks` ticks (none when nothing is live), then reads every slot's
+// phase.  *live = slots still iterating, *done = DONE slots not yet collected.
+static int pool_step(pk_batch_s* P, int ticks, int* live, int* done) {
    Ctx& c = ctx();
-   const BLayout L = Bt->layout();
-   const long N = Bt->N;
-   const int B = Bt->B;
-   const int gb = grid_for(N);
-   // initial iterates and state
-   bfill_kernel<<<gb, kBlock, 0, c.stream>>>(Bt->x.p, N, L, 1.0);
-   if (Bt->init.p)
-      for (int b = 0; b < B; b++)
-         if (Bt->has_init[b])
-            PK_HIP(hipMemcpyAsync(Bt->x.p + Bt->row_off[b], Bt->init.p + Bt->row_off[b],
-                                  sizeof(double) * Bt->n[b], hipMemcpyDeviceToDevice, c.stream));
-   PK_HIP(hipMemsetAsync(Bt->st_i.p, 0, sizeof(int) * B * 10, c.stream));
-   PK_HIP(hipMemsetAsync(Bt->es.ks, 0, sizeof(double) * B, c.stream));
-   std::vector<int> ones(B, 1);
-   PK_HIP(hipMemcpyAsync(Bt->es.active, ones.data(), sizeof(int) * B, hipMemcpyHostToDevice, c.stream));
-   PK_CALL(batch_rebuild<T>(Bt, 1));
-   Bt->inner.assign(B, 0);
-   std::vector<int> hconv(B), hreb(B), hflags;
-   for (int it = 0; it < Bt->max_outer; it++) {
-      brf_kernel<<<Bt->nwg, kRows, 0, c.stream>>>(N, L, Bt->Rc.view(), Bt->Rc.val.p, Bt->Fc.view(), Bt->Fc.val.p,
-                                                  Bt->x.p, Bt->y1.p, Bt->y2.p, Bt->part.p, Bt->tick.p, Bt->es);
-      bresid_kernel<<<Bt->nwg, kRows, 0, c.stream>>>(N, L, Bt->x.p, Bt->y1.p, Bt->y2.p, Bt->phi.p, Bt->r.p, Bt->tol,
-                                                     Bt->delta, Bt->part.p, Bt->tick.p, Bt->es);
-      PK_HIP(hipGetLastError());
-      PK_HIP(hipMemcpyAsync(hconv.data(), Bt->es.conv, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
-      PK_HIP(hipMemcpyAsync(hreb.data(), Bt->es.rebuild, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
-      PK_HIP(hipStreamSynchronize(c.stream));
-      bool all = true, reb = false;
-      for (int b = 0; b < B; b++) {
-         all &= hconv[b] != 0;
-         reb |= hreb[b] != 0;
+   const long units = P->ua.high_water();
+   int nlive = 0;
+   for (int s = 0; s < P->max_slots; s++)
+      if (P->slots[s].used && (P->h_phase[s] == kOuter || P->h_phase[s] == kInner)) nlive++;
+   auto t0 = std::chrono::steady_clock::now();
+   if (nlive > 0 && units > 0)
+      for (int t = 0; t < ticks; t++) {
+         if (P->precision == PK_PREC_MIXED)
+            PK_CALL(run_tick<float>(P, units));
+         else
+            PK_CALL(run_tick<double>(P, units));
+         P->ticks++;
       }
-      if (all) break;
-      if (reb) PK_CALL(batch_rebuild<T>(Bt, 0));
-      PK_CALL(batch_gmres<T>(Bt, hflags));
-      for (int b = 0; b < B; b++) Bt->inner[b] += hflags[4 * b + 3];
-      badd_kernel<T><<<gb, kBlock, 0, c.stream>>>(N, L, Bt->phi.p, gm<T>(Bt).e.p, Bt->es.active, Bt->x.p);
-      PK_HIP(hipGetLastError());
+   int herr = 0;
+   PK_HIP(hipMemcpyAsync(P->h_phase.data(), P->S.phase, sizeof(int) * P->max_slots, hipMemcpyDeviceToHost,
+                         c.stream));
+   PK_HIP(hipMemcpyAsync(&herr, P->err.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   P->tick_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+   PK_REQUIRE(herr == 0, "singular diagonal block in the block-Jacobi preconditioner");
+   int nl = 0, nd = 0;
+   for (int s = 0; s < P->max_slots; s++) {
+      if (!P->slots[s].used) continue;
+      if (P->h_phase[s] == kDone) {
+         if (!P->slots[s].harvested) nd++;
+      } else {
+         nl++;
+      }
    }
-   // per-instance results
-   std::vector<double> hk(B), hres(B);
-   std::vector<int> hit(B);
-   PK_HIP(hipMemcpy(hk.data(), Bt->es.k, sizeof(double) * B, hipMemcpyDeviceToHost));
-   PK_HIP(hipMemcpy(hres.data(), Bt->es.res, sizeof(double) * B, hipMemcpyDeviceToHost));
-   PK_HIP(hipMemcpy(hit.data(), Bt->es.it, sizeof(int) * B, hipMemcpyDeviceToHost));
-   PK_HIP(hipMemcpy(hconv.data(), Bt->es.conv, sizeof(int) * B, hipMemcpyDeviceToHost));
-   Bt->keff = hk;
-   Bt->resid = hres;
-   Bt->outer.assign(B, 0);
-   for (int b = 0; b < B; b++) Bt->outer[b] = hit[b] > 0 ? hit[b] - 1 : 0;
-   for (int b = 0; b < B; b++)
-      PK_REQUIRE(hres[b] <= Bt->tol, "eigen iteration of batch member " + std::to_string(b) + " did not converge");
+   if (live) *live = nl;
+   if (done) *done = nd;
+   return 0;
+}
+
+struct SlotResult {
+   double k = 0.0, res = 0.0;
+   int it = 0, inner = 0, status = 0;
+};
+
+static int slot_result(pk_batch_s* P, int sidx, SlotResult& R, pk_vec_s* x) {
+   Ctx& c = ctx();
+   PK_REQUIRE(sidx >= 0 && sidx < P->max_slots && P->slots[sidx].used, "no such slot");
+   PK_REQUIRE(P->h_phase[sidx] == kDone, "slot has not finished");
+   PK_HIP(hipMemcpyAsync(&R.k, P->S.k + sidx, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipMemcpyAsync(&R.res, P->S.res + sidx, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipMemcpyAsync(&R.it, P->S.it + sidx, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipMemcpyAsync(&R.inner, P->S.inner + sidx, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipMemcpyAsync(&R.status, P->S.status + sidx, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   if (x) {
+      PK_REQUIRE(x->n == P->slots[sidx].n, "vector size mismatch");
+      PK_HIP(hipMemcpyAsync(x->d.p, P->phi.p + P->slots[sidx].unit0 * kRows, sizeof(double) * x->n,
+                            hipMemcpyDeviceToDevice, c.stream));
+   }
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
+}
+
+// ---- fixed batch: every member admitted at once and run to the end ------------------
+
+static int fixed_solve(pk_batch_s* P) {
+   const int B = (int)P->fixR.size();
+   for (int s = 0; s < P->max_slots; s++)
+      if (P->slots[s].used) PK_CALL(pool_release(P, s));
+   P->fix_slot.assign(B, -1);
+   for (int b = 0; b < B; b++) {
+      int slot = -1;
+      const double* x0 = P->fix_has_init[b] ? P->fix_init[b].p : nullptr;
+      int rc = pool_admit(P, P->fixR[b], P->fixF[b], x0, &slot);
+      PK_REQUIRE(rc != 2, "batch does not fit its pool");
+      PK_CALL(rc);
+      P->fix_slot[b] = slot;
+   }
+   int live = B, done = 0;
+   while (live > 0) PK_CALL(pool_step(P, 1, &live, &done));
    return 0;
 }
 
@@ -959,113 +1163,228 @@ extern "C" {
 int pk_batch_create(int num, const pk_mat* R, const pk_mat* F, pk_batch* out) {
    PK_CALL(ensure_init());
    PK_REQUIRE(num >= 1 && R && F && out, "wrong batch arguments");
-   auto* Bt = new pk_batch_s();
-   Bt->B = num;
+   long units = 0, re = 0, fe = 0, bi = 0;
    for (int b = 0; b < num; b++) {
-      pk_mat_s* r = R[b];
-      pk_mat_s* f = F[b];
-      if (!r || !f || !r->assembled || !f->assembled || r->n != f->n || r->bs != f->bs ||
-          f->nnz != r->n * (long)r->bs) {
-         delete Bt;
-         PK_FAIL("batch member " + std::to_string(b) +
-                 ": R and F must be assembled, of one size, and F must hold one cell block per row");
+      if (check_member(R[b], F[b])) {
+         std::string msg = std::string("batch member ") + std::to_string(b) + ": " + pk_last_error();
+         PK_FAIL(msg);
       }
-      Bt->R.push_back(r);
-      Bt->F.push_back(f);
-      Bt->n.push_back((int)r->n);
-      Bt->fbs.push_back(r->bs);
-      int p = 1;
-      while (p * 2 <= kBMaxBs && r->bs % (p * 2) == 0) p *= 2;
-      if (r->bs <= kBMaxBs && kRows % r->bs == 0) p = r->bs;
-      Bt->pbs.push_back(p);
+      units += units_of(R[b]->n);
+      re += blocks64(R[b]->sell.padded);
+      fe += blocks64(F[b]->sell.padded);
+      bi += blocks64(R[b]->n * preconditioner_block(R[b]->bs));
    }
-   Bt->has_init.assign(num, 0);
-   *out = Bt;
+   PK_REQUIRE(units * kRows < (1L << 31), "batch too large for 32-bit columns");
+   auto* P = new pk_batch_s();
+   P->max_units = units;
+   P->max_rentries = std::max(1L, re);
+   P->max_fentries = std::max(1L, fe);
+   P->max_binv = std::max(1L, bi);
+   P->max_slots = num;
+   for (int b = 0; b < num; b++) {
+      P->fixR.push_back(R[b]);
+      P->fixF.push_back(F[b]);
+   }
+   P->fix_init.resize(num);
+   P->fix_has_init.assign(num, 0);
+   *out = P;
    return 0;
 }
 
-int pk_batch_set_tolerances(pk_batch Bt, double tol, int max_outer) {
-   PK_REQUIRE(Bt && tol > 0.0 && max_outer > 0, "wrong tolerances");
-   Bt->tol = tol;
-   Bt->max_outer = max_outer;
+int pk_batch_set_tolerances(pk_batch P, double tol, int max_outer) {
+   PK_REQUIRE(P && tol > 0.0 && max_outer > 0, "wrong tolerances");
+   P->opts.tol = tol;
+   P->opts.max_outer = max_outer;
+   P->graph.clear();
    return 0;
 }
 
-int pk_batch_set_precision(pk_batch Bt, int precision) {
-   PK_REQUIRE(Bt && (precision == PK_PREC_FP64 || precision == PK_PREC_MIXED), "wrong precision");
-   PK_REQUIRE(!Bt->ready, "precision must be set before the first solve");
-   Bt->precision = precision;
+int pk_batch_set_precision(pk_batch P, int precision) {
+   PK_REQUIRE(P && (precision == PK_PREC_FP64 || precision == PK_PREC_MIXED), "wrong precision");
+   PK_REQUIRE(!P->sd.p, "precision must be set before the first solve");
+   P->precision = precision;
    return 0;
 }
 
-int pk_batch_set_inner(pk_batch Bt, int restart, double reduction, int max_inner) {
-   PK_REQUIRE(Bt && restart >= 1 && restart <= kBMaxM && reduction > 0.0 && reduction < 1.0 && max_inner > 0,
+int pk_batch_set_inner(pk_batch P, int restart, double reduction, int max_inner) {
+   PK_REQUIRE(P && restart >= 1 && restart <= kBMaxM && reduction > 0.0 && reduction < 1.0 && max_inner > 0,
               "wrong inner-solver options");
-   PK_REQUIRE(!Bt->ready, "inner options must be set before the first solve");
-   Bt->restart = restart;
-   Bt->reduction = reduction;
-   Bt->max_inner = max_inner;
+   PK_REQUIRE(!P->sd.p || restart == P->m, "the restart length must be set before the first solve");
+   P->m = restart;
+   P->opts.reduction = reduction;
+   P->opts.max_inner = max_inner;
+   P->graph.clear();
    return 0;
 }
 
-int pk_batch_set_shift(pk_batch Bt, double delta) {
-   PK_REQUIRE(Bt && delta >= 0.0, "wrong shift");
-   Bt->delta = delta;
+int pk_batch_set_shift(pk_batch P, double delta) {
+   PK_REQUIRE(P && delta >= 0.0, "wrong shift");
+   P->opts.delta = delta;
+   P->graph.clear();
    return 0;
 }
 
-int pk_batch_set_initial(pk_batch Bt, int index, pk_vec x) {
-   PK_REQUIRE(Bt && index >= 0 && index < Bt->B && x && x->n == Bt->n[index], "wrong initial vector");
-   PK_CALL(batch_setup(Bt));
-   if (!Bt->init.p) {
-      PK_REQUIRE(!Bt->init.alloc(Bt->N), "device allocation failed");
-      PK_HIP(hipMemsetAsync(Bt->init.p, 0, sizeof(double) * Bt->N, ctx().stream));
-   }
+int pk_batch_set_initial(pk_batch P, int index, pk_vec x) {
+   PK_REQUIRE(P && index >= 0 && index < (int)P->fixR.size() && x && x->n == P->fixR[index]->n,
+              "wrong initial vector");
    // an all-zero vector means "no initial space" (petsc.cxx:136-140)
    double nrm = 0.0;
    PK_CALL(pk_vec_dot(x, x, &nrm));
-   Bt->has_init[index] = nrm > 0.0;
-   if (nrm > 0.0)
-      PK_HIP(hipMemcpyAsync(Bt->init.p + Bt->row_off[index], x->d.p, sizeof(double) * x->n, hipMemcpyDeviceToDevice,
-                            ctx().stream));
-   return 0;
-}
-
-int pk_batch_solve(pk_batch Bt) {
-   PK_REQUIRE(Bt, "null batch");
-   auto t0 = std::chrono::steady_clock::now();
-   PK_CALL(batch_setup(Bt));
-   int rc = Bt->precision == PK_PREC_MIXED ? batch_solve_t<float>(Bt) : batch_solve_t<double>(Bt);
-   Bt->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-   return rc;
-}
-
-int pk_batch_get_eigenpair(pk_batch Bt, int index, double* lambda, pk_vec x) {
-   PK_REQUIRE(Bt && index >= 0 && index < Bt->B && !Bt->keff.empty(), "no solution for this batch member");
-   if (lambda) *lambda = 1.0 / Bt->keff[index];
-   if (x) {
-      PK_REQUIRE(x->n == Bt->n[index], "vector size mismatch");
-      PK_HIP(hipMemcpyAsync(x->d.p, Bt->phi.p + Bt->row_off[index], sizeof(double) * x->n,
-                            hipMemcpyDeviceToDevice, ctx().stream));
-      PK_HIP(hipStreamSynchronize(ctx().stream));
+   P->fix_has_init[index] = nrm > 0.0;
+   if (nrm > 0.0) {
+      DBuf<double>& d = P->fix_init[index];
+      if (!d.p) PK_REQUIRE(!d.alloc(x->n), "device allocation failed");
+      PK_HIP(hipMemcpyAsync(d.p, x->d.p, sizeof(double) * x->n, hipMemcpyDeviceToDevice, ctx().stream));
    }
    return 0;
 }
 
-int pk_batch_get_stats(pk_batch Bt, int index, int* outer, long* inner, double* residual, double* seconds) {
-   PK_REQUIRE(Bt && index >= 0 && index < Bt->B && !Bt->keff.empty(), "no solution for this batch member");
-   if (outer) *outer = Bt->outer[index];
-   if (inner) *inner = Bt->inner[index];
-   if (residual) *residual = Bt->resid[index];
-   if (seconds) *seconds = Bt->seconds;
+int pk_batch_solve(pk_batch P) {
+   PK_REQUIRE(P && !P->fixR.empty(), "not a fixed batch");
+   auto t0 = std::chrono::steady_clock::now();
+   if (!P->sd.p) PK_CALL(pool_alloc(P));
+   int rc = fixed_solve(P);
+   P->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+   PK_CALL(rc);
+   for (int b = 0; b < (int)P->fixR.size(); b++) {
+      SlotResult R;
+      PK_CALL(slot_result(P, P->fix_slot[b], R, nullptr));
+      PK_REQUIRE(R.status == 1, "eigen iteration of batch member " + std::to_string(b) + " did not converge");
+   }
    return 0;
 }
 
-int pk_batch_destroy(pk_batch* Bt) {
-   if (!Bt || !*Bt) return 0;
+int pk_batch_get_eigenpair(pk_batch P, int index, double* lambda, pk_vec x) {
+   PK_REQUIRE(P && index >= 0 && index < (int)P->fix_slot.size() && P->fix_slot[index] >= 0,
+              "no solution for this batch member");
+   SlotResult R;
+   PK_CALL(slot_result(P, P->fix_slot[index], R, x));
+   if (lambda) *lambda = 1.0 / R.k;
+   return 0;
+}
+
+int pk_batch_get_stats(pk_batch P, int index, int* outer, long* inner, double* residual, double* seconds) {
+   PK_REQUIRE(P && index >= 0 && index < (int)P->fix_slot.size() && P->fix_slot[index] >= 0,
+              "no solution for this batch member");
+   SlotResult R;
+   PK_CALL(slot_result(P, P->fix_slot[index], R, nullptr));
+   if (outer) *outer = R.it;
+   if (inner) *inner = R.inner;
+   if (residual) *residual = R.res;
+   if (seconds) *seconds = P->seconds;
+   return 0;
+}
+
+int pk_batch_destroy(pk_batch* P) {
+   if (!P || !*P) return 0;
    (void)hipStreamSynchronize(ctx().stream);
-   delete *Bt;
-   *Bt = nullptr;
+   delete *P;
+   *P = nullptr;
+   return 0;
+}
+
+// ---- pool (continuous batching) ----------------------------------------------------
+
+double pk_pool_bytes(long max_rows, long max_entries, long max_fentries, long max_binv, int max_slots,
+                     int precision, int restart) {
+   return pool_bytes(units_of(max_rows), blocks64(max_entries), blocks64(max_fentries), blocks64(max_binv),
+                     max_slots, restart, precision);
+}
+
+int pk_pool_create(long max_rows, long max_entries, long max_fentries, long max_binv, int max_slots,
+                   int precision, int restart, pk_batch* out) {
+   PK_CALL(ensure_init());
+   PK_REQUIRE(out && max_rows > 0 && max_slots > 0 && restart >= 1 && restart <= kBMaxM &&
+                  (precision == PK_PREC_FP64 || precision == PK_PREC_MIXED),
+              "wrong eigen pool arguments");
+   auto* P = new pk_batch_s();
+   P->max_units = units_of(max_rows);
+   P->max_rentries = std::max(1L, blocks64(max_entries));
+   P->max_fentries = std::max(1L, blocks64(max_fentries));
+   P->max_binv = std::max(1L, blocks64(max_binv));
+   P->max_slots = max_slots;
+   P->precision = precision;
+   P->m = restart;
+   if (P->max_units * kRows >= (1L << 31)) {
+      delete P;
+      PK_FAIL("pool too large for 32-bit columns");
+   }
+   if (pool_alloc(P)) {
+      delete P;
+      return 1;
+   }
+   *out = P;
+   return 0;
+}
+
+int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction, int max_inner, double delta) {
+   PK_REQUIRE(P && tol > 0.0 && max_outer > 0 && reduction > 0.0 && reduction < 1.0 && max_inner > 0 &&
+                  delta >= 0.0,
+              "wrong pool options");
+   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner};
+   P->graph.clear();
+   return 0;
+}
+
+int pk_pool_admit(pk_batch P, pk_mat R, pk_mat F, pk_vec x0, int* slot) {
+   PK_REQUIRE(P && slot && P->sd.p, "wrong pool arguments");
+   const double* x0d = nullptr;
+   if (x0) {
+      PK_REQUIRE(R && x0->n == R->n, "initial vector size mismatch");
+      x0d = x0->d.p;
+   }
+   *slot = -1;
+   return pool_admit(P, R, F, x0d, slot);
+}
+
+int pk_pool_step(pk_batch P, int ticks, int* live, int* done) {
+   PK_REQUIRE(P && P->sd.p && ticks >= 0, "wrong pool arguments");
+   return pool_step(P, ticks, live, done);
+}
+
+int pk_pool_collect(pk_batch P, int max, int* slots, int* count) {
+   PK_REQUIRE(P && slots && count && max >= 0, "wrong pool arguments");
+   // DONE slots in admission order
+   std::vector<std::pair<long, int>> d;
+   for (int s = 0; s < P->max_slots; s++)
+      if (P->slots[s].used && P->h_phase[s] == kDone && !P->slots[s].harvested)
+         d.push_back({P->slots[s].ticket, s});
+   std::sort(d.begin(), d.end());
+   int n = std::min<int>(max, (int)d.size());
+   for (int i = 0; i < n; i++) {
+      slots[i] = d[i].second;
+      P->slots[d[i].second].harvested = true;
+   }
+   *count = n;
+   return 0;
+}
+
+int pk_pool_result(pk_batch P, int slot, double* keff, pk_vec x, int* outer, long* inner, double* residual,
+                   int* converged) {
+   PK_REQUIRE(P, "null pool");
+   SlotResult R;
+   PK_CALL(slot_result(P, slot, R, x));
+   if (keff) *keff = R.k;
+   if (outer) *outer = R.it;
+   if (inner) *inner = R.inner;
+   if (residual) *residual = R.res;
+   if (converged) *converged = R.status == 1;
+   return 0;
+}
+
+int pk_pool_release(pk_batch P, int slot) {
+   PK_REQUIRE(P, "null pool");
+   return pool_release(P, slot);
+}
+
+int pk_pool_capacity(pk_batch P, long* free_rows, long* free_entries, int* free_slots, long* ticks) {
+   PK_REQUIRE(P, "null pool");
+   if (free_rows) *free_rows = P->ua.largest_free() * kRows;
+   if (free_entries) *free_entries = P->ra.largest_free() * 64;
+   int fs = 0;
+   for (auto& s : P->slots) fs += !s.used;
+   if (free_slots) *free_slots = fs;
+   if (ticks) *ticks = P->ticks;
    return 0;
 }
 

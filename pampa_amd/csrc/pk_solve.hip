@@ -1,6 +1,7 @@
 // Krylov solvers of the pk_* layer: restarted GMRES with right cell-block
-// Jacobi preconditioning, the Wielandt-shifted power iteration for
-// R x = lambda F x, and the GMRES linear solver for the transient rows.
+// Jacobi preconditioning for the linear (transient) solves, and the pk_eig
+// interface, which runs one problem in the pooled eigen engine (pk_batch.hip,
+// the same GMRES steps per slot).
 //
 // GMRES step j of a restart cycle is three kernels and no host round trip:
 //   prec     V[j] *= 1/h(j,j-1) (normalise), z = B^-1 V[j]      (B = cell blocks)
@@ -305,13 +306,6 @@ __global__ void convert_kernel(const double* a, T* b, long n) {
    if (i < n) b[i] = (T)a[i];
 }
 
-// x = base + e (FP64 result)
-template <class T>
-__global__ void add_correction_kernel(const double* base, const T* e, double* x, long n) {
-   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (i < n) x[i] = base[i] + (double)e[i];
-}
-
 // ---- GMRES engine ----------------------------------------------------------------
 
 // Solves A e = r (r FP64, e in precision T, e0 = 0) to |r - A e| <= reduction |r|
@@ -457,327 +451,80 @@ struct Gmres {
 }  // namespace pk
 
 // ---- eigen solver -------------------------------------------------------------------
+// One problem in the pooled engine (pk_batch.hip): the same kernels and tick
+// graph that serve batches, with a pool of one slot kept for repeated solves.
 
 using namespace pk;
-
-namespace pk {
-
-// A = R - F/ks on R's SELL pattern (F's entries lie inside R's cell blocks).
-__global__ void shifted_values_kernel(long n, int bs, const long* off, const int* width,
-                                      const int* rlen, const int* col, const double* rval,
-                                      const double* fval_csr, double inv_ks, double* aval) {
-   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (row >= n) return;
-   long s = row >> 6;
-   long base = off[s] + (row & 63);
-   long cell = row / bs;
-   for (int k = 0; k < width[s]; k++) {
-      long p = base + (long)k * 64;
-      int cc = col[p];
-      double v = rval[p];
-      // padding entries (k >= row length) keep their zero value
-      if (k < rlen[row] && cc / bs == cell) v -= fval_csr[row * bs + (cc % bs)] * inv_ks;
-      aval[p] = v;
-   }
-}
-
-__global__ void fill_kernel(double* x, long n, double a) {
-   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (i < n) x[i] = a;
-}
-
-// y1 = R x, y2 = F x; out = {<y2,y1>, <y2,y2>, <x,x>}
-__global__ void __launch_bounds__(kBlock) rf_kernel(long n, const long* roff,
-                                                     const int* rwid, const int* rcol,
-                                                     const double* rval, const long* foff,
-                                                     const int* fwid, const int* fcol,
-                                                     const double* fval, const double* x, double* y1,
-                                                     double* y2, double* partials, unsigned* ticket,
-                                                     double* out) {
-   long row = blockIdx.x * (long)kBlock + threadIdx.x;
-   double a = 0.0, b = 0.0, xv = 0.0;
-   if (row < n) {
-      long s = row >> 6;
-      int lane = threadIdx.x & 63;
-      long base = roff[s] + lane;
-      for (int k = 0; k < rwid[s]; k++) {
-         long p = base + (long)k * 64;
-         a += rval[p] * x[rcol[p]];
-      }
-      base = foff[s] + lane;
-      for (int k = 0; k < fwid[s]; k++) {
-         long p = base + (long)k * 64;
-         b += fval[p] * x[fcol[p]];
-      }
-      y1[row] = a;
-      y2[row] = b;
-      xv = x[row];
-   }
-   grid_reduce<3>(
-       3,
-       [&](int k) { return k == 0 ? b * a : (k == 1 ? b * b : xv * xv); }, partials, ticket, out);
-}
-
-// phi = x/nx, r = (y2/k - y1)/nx ; out = {|r|^2, |y2/k|^2 / nx^2}
-__global__ void __launch_bounds__(kBlock) resid_kernel(long n, const double* x, const double* y1,
-                                                        const double* y2, const double* scal,
-                                                        double* phi, double* r, double* partials,
-                                                        unsigned* ticket, double* out) {
-   long row = blockIdx.x * (long)kBlock + threadIdx.x;
-   double inv_k = scal[0] / scal[1];  // 1/k = <y2,y1>/<y2,y2>
-   double inv_nx = 1.0 / sqrt(scal[2]);
-   double rr = 0.0, qq = 0.0;
-   if (row < n) {
-      double q = y2[row] * inv_k * inv_nx;
-      double v = q - y1[row] * inv_nx;
-      phi[row] = x[row] * inv_nx;
-      r[row] = v;
-      rr = v * v;
-      qq = q * q;
-   }
-   grid_reduce<2>(
-       2, [&](int k) { return k == 0 ? rr : qq; }, partials, ticket, out);
-}
-
-}  // namespace pk
 
 struct pk_eig_s {
    pk_mat_s* R = nullptr;
    pk_mat_s* F = nullptr;
-   long n = 0;
-   int bs = 1;   // cell block of F (groups, or groups x directions)
-   int pbs = 1;  // block of the Jacobi preconditioner (divides bs and 256, <= 16)
-   double tol = 1e-8;
-   int max_outer = 10000;
-   int precision = PK_PREC_FP64;
-   int restart = 30;
-   double reduction = 0.1;
-   int max_inner = 2000;
-   double delta = 0.1;  // Wielandt shift
-   // state
-   DBuf<double> x, phi, y1, y2, r, aval, init;
-   SellData Ash;  // shifted operator (values in aval / val32 of Ash)
-   Gmres<double> g64;
-   Gmres<float> g32;
-   bool has_init = false, ready = false;
-   double ks = 0.0;  // current shift (0 = none)
-   double keff = 1.0, residual = 0.0, seconds = 0.0;
-   int outer = 0;
-   long inner = 0;
+   pk_batch B = nullptr;
+   bool solved = false;
 };
-
-static int eig_build_shift(pk_eig_s* E, double ks) {
-   Ctx& c = ctx();
-   E->ks = ks;
-   shifted_values_kernel<<<grid_for(E->n), kBlock, 0, c.stream>>>(
-       E->n, E->bs, E->R->sell.off.p, E->R->sell.width.p, E->R->sell.rlen.p, E->R->sell.col.p,
-       E->R->sell.val.p, E->F->val.p, ks > 0.0 ? 1.0 / ks : 0.0, E->Ash.val.p);
-   PK_HIP(hipGetLastError());
-   if (E->precision == PK_PREC_MIXED) {
-      convert_kernel<float><<<grid_for(E->Ash.padded), kBlock, 0, c.stream>>>(E->Ash.val.p, E->Ash.val32.p,
-                                                                           E->Ash.padded);
-      PK_CALL(E->g32.factor(E->Ash));
-   } else {
-      PK_CALL(E->g64.factor(E->Ash));
-   }
-   return 0;
-}
-
-static int eig_setup(pk_eig_s* E) {
-   if (E->ready) return 0;
-   long n = E->n;
-   PK_REQUIRE(!E->x.alloc(n) && !E->phi.alloc(n) && !E->y1.alloc(n) && !E->y2.alloc(n) &&
-                  !E->r.alloc(n),
-              "device allocation failed");
-   // shifted operator shares R's pattern arrays; own values
-   SellData& S = E->Ash;
-   S.nslices = E->R->sell.nslices;
-   S.padded = E->R->sell.padded;
-   PK_REQUIRE(!S.val.alloc(S.padded ? S.padded : 1), "device allocation failed");
-   if (E->precision == PK_PREC_MIXED) PK_REQUIRE(!S.val32.alloc(S.padded ? S.padded : 1), "device allocation failed");
-   // the pattern pointers are borrowed from R (not freed by Ash)
-   S.off.p = E->R->sell.off.p;
-   S.width.p = E->R->sell.width.p;
-   S.col.p = E->R->sell.col.p;
-   S.rlen.p = E->R->sell.rlen.p;
-   if (E->precision == PK_PREC_MIXED)
-      PK_CALL(E->g32.setup(&E->Ash, E->Ash.val32.p, n, E->pbs, E->restart));
-   else
-      PK_CALL(E->g64.setup(&E->Ash, E->Ash.val.p, n, E->pbs, E->restart));
-   E->ready = true;
-   return 0;
-}
 
 extern "C" {
 
 int pk_eig_create(pk_mat R, pk_mat F, pk_eig* E) {
    PK_CALL(ensure_init());
-   PK_REQUIRE(R && F && R->assembled && F->assembled, "matrices not assembled");
-   PK_REQUIRE(R->n == F->n, "matrix size mismatch");
-   PK_REQUIRE(R->bs == F->bs, "block size mismatch");
-   // F must be the cell-block fission matrix: G entries per row inside the block
-   PK_REQUIRE(F->nnz == R->n * (long)R->bs, "F must hold one full cell block per row");
+   PK_REQUIRE(E, "null output");
+   pk_batch B = nullptr;
+   PK_CALL(pk_batch_create(1, &R, &F, &B));
    auto* e = new pk_eig_s();
    e->R = R;
    e->F = F;
-   e->n = R->n;
-   e->bs = R->bs;
-   // largest power of two <= kMaxBs dividing the cell block (cells of the
-   // preconditioner must not straddle 256-thread workgroups)
-   e->pbs = 1;
-   while (e->pbs * 2 <= kMaxBs && e->bs % (e->pbs * 2) == 0) e->pbs *= 2;
-   if (e->bs <= kMaxBs && kBlock % e->bs == 0) e->pbs = e->bs;
+   e->B = B;
    *E = e;
    return 0;
 }
 
 int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer) {
-   PK_REQUIRE(E && tol > 0.0 && max_outer > 0, "wrong tolerances");
-   E->tol = tol;
-   E->max_outer = max_outer;
-   return 0;
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_tolerances(E->B, tol, max_outer);
 }
 
 int pk_eig_set_precision(pk_eig E, int precision) {
-   PK_REQUIRE(E && (precision == PK_PREC_FP64 || precision == PK_PREC_MIXED), "wrong precision");
-   PK_REQUIRE(!E->ready, "precision must be set before the first solve");
-   E->precision = precision;
-   return 0;
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_precision(E->B, precision);
 }
 
 int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner) {
-   PK_REQUIRE(E && restart >= 1 && restart <= kMaxRestart && reduction > 0.0 && reduction < 1.0 &&
-                  max_inner > 0,
-              "wrong inner-solver options");
-   PK_REQUIRE(!E->ready, "inner options must be set before the first solve");
-   E->restart = restart;
-   E->reduction = reduction;
-   E->max_inner = max_inner;
-   return 0;
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_inner(E->B, restart, reduction, max_inner);
 }
 
 int pk_eig_set_shift(pk_eig E, double delta) {
-   PK_REQUIRE(E && delta >= 0.0, "wrong shift");
-   E->delta = delta;
-   return 0;
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_shift(E->B, delta);
 }
 
 int pk_eig_set_initial(pk_eig E, pk_vec x) {
-   PK_REQUIRE(E && x && x->n == E->n, "wrong initial vector");
-   PK_REQUIRE(!E->init.alloc(E->n), "device allocation failed");
-   PK_HIP(hipMemcpyAsync(E->init.p, x->d.p, sizeof(double) * E->n, hipMemcpyDeviceToDevice,
-                         ctx().stream));
-   E->has_init = true;
-   return 0;
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_initial(E->B, 0, x);
 }
 
 int pk_eig_solve(pk_eig E) {
    PK_REQUIRE(E, "null eigen solver");
-   Ctx& c = ctx();
-   auto t0 = std::chrono::steady_clock::now();
-   PK_CALL(eig_setup(E));
-   const long n = E->n;
-   const int gb = grid_for(n);
-   PK_CALL(reserve_partials((size_t)gb * 4));
-   // initial guess: the given vector if it is not all zero, else a constant
-   double nrm = 0.0;
-   if (E->has_init) {
-      PK_CALL(reduce_dot(E->init.p, E->init.p, n, c.scalars));
-      PK_HIP(hipMemcpyAsync(c.host_scalars, c.scalars, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-      PK_HIP(hipStreamSynchronize(c.stream));
-      nrm = c.host_scalars[0];
-   }
-   if (nrm > 0.0)
-      PK_HIP(hipMemcpyAsync(E->x.p, E->init.p, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
-   else
-      fill_kernel<<<gb, kBlock, 0, c.stream>>>(E->x.p, n, 1.0);
-   double* dsc = c.scalars + 8;  // {<y2,y1>, <y2,y2>, <x,x>}
-   double* dres = c.scalars + 16;
-   double k_prev = 0.0;
-   E->inner = 0;
-   E->outer = 0;
-   bool shifted = false;
-   PK_CALL(eig_build_shift(E, 0.0));
-   for (int it = 0; it < E->max_outer; it++) {
-      // eigenvalue estimate and residual of the current iterate
-      rf_kernel<<<gb, kBlock, 0, c.stream>>>(n, E->R->sell.off.p, E->R->sell.width.p,
-                                             E->R->sell.col.p, E->R->sell.val.p, E->F->sell.off.p,
-                                             E->F->sell.width.p, E->F->sell.col.p, E->F->sell.val.p,
-                                             E->x.p, E->y1.p, E->y2.p, c.partials, c.ticket, dsc);
-      resid_kernel<<<gb, kBlock, 0, c.stream>>>(n, E->x.p, E->y1.p, E->y2.p, dsc, E->phi.p, E->r.p,
-                                                c.partials, c.ticket, dres);
-      PK_HIP(hipGetLastError());
-      PK_HIP(hipMemcpyAsync(c.host_scalars, dsc, sizeof(double) * 16, hipMemcpyDeviceToHost, c.stream));
-      PK_HIP(hipStreamSynchronize(c.stream));
-      double s_fr = c.host_scalars[0], s_ff = c.host_scalars[1];
-      double rr = c.host_scalars[8], qq = c.host_scalars[9];
-      PK_REQUIRE(s_ff > 0.0 && s_fr != 0.0, "zero fission source in the eigen solve");
-      double k = s_ff / s_fr;
-      double res = sqrt(rr / qq);
-      E->keff = k;
-      E->residual = res;
-      E->outer = it;
-      if (!(res == res)) PK_FAIL("eigen iteration diverged");
-      if (res <= E->tol) break;
-      // Wielandt shift once the eigenvalue estimate has settled
-      if (E->delta > 0.0) {
-         bool settle = it >= 2 && fabs(k - k_prev) < 1e-2 * k;
-         if (settle && (!shifted || fabs(k + E->delta - E->ks) > 0.2 * E->delta)) {
-            PK_CALL(eig_build_shift(E, k + E->delta));
-            shifted = true;
-         }
-      }
-      k_prev = k;
-      // correction: A e = r with A = R - F/ks, x = phi + e
-      if (E->precision == PK_PREC_MIXED) {
-         PK_CALL(E->g32.solve(E->r.p, E->reduction, 0.0, E->max_inner));
-         E->inner += E->g32.last_iters;
-         add_correction_kernel<float><<<gb, kBlock, 0, c.stream>>>(E->phi.p, E->g32.e.p, E->x.p, n);
-      } else {
-         PK_CALL(E->g64.solve(E->r.p, E->reduction, 0.0, E->max_inner));
-         E->inner += E->g64.last_iters;
-         add_correction_kernel<double><<<gb, kBlock, 0, c.stream>>>(E->phi.p, E->g64.e.p, E->x.p, n);
-      }
-      PK_HIP(hipGetLastError());
-      E->outer = it + 1;
-   }
-   PK_HIP(hipStreamSynchronize(c.stream));
-   E->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-   PK_REQUIRE(E->residual <= E->tol, "eigen iteration did not converge");
+   E->solved = false;
+   PK_CALL(pk_batch_solve(E->B));
+   E->solved = true;
    return 0;
 }
 
 int pk_eig_get_eigenpair(pk_eig E, double* lambda, pk_vec x) {
-   PK_REQUIRE(E, "null eigen solver");
-   if (lambda) *lambda = 1.0 / E->keff;
-   if (x) {
-      PK_REQUIRE(x->n == E->n, "vector size mismatch");
-      PK_HIP(hipMemcpyAsync(x->d.p, E->phi.p, sizeof(double) * E->n, hipMemcpyDeviceToDevice, ctx().stream));
-      PK_HIP(hipStreamSynchronize(ctx().stream));
-   }
-   return 0;
+   PK_REQUIRE(E && E->solved, "no eigen solution");
+   return pk_batch_get_eigenpair(E->B, 0, lambda, x);
 }
 
 int pk_eig_get_stats(pk_eig E, int* outer, long* inner, double* residual, double* seconds) {
-   PK_REQUIRE(E, "null eigen solver");
-   if (outer) *outer = E->outer;
-   if (inner) *inner = E->inner;
-   if (residual) *residual = E->residual;
-   if (seconds) *seconds = E->seconds;
-   return 0;
+   PK_REQUIRE(E && E->solved, "no eigen solution");
+   return pk_batch_get_stats(E->B, 0, outer, inner, residual, seconds);
 }
 
 int pk_eig_destroy(pk_eig* E) {
    if (!E || !*E) return 0;
-   (void)hipStreamSynchronize(ctx().stream);
-   pk_eig_s* e = *E;
-   e->Ash.off.p = nullptr;
-   e->Ash.width.p = nullptr;
-   e->Ash.col.p = nullptr;
-   e->Ash.rlen.p = nullptr;
-   e->Ash.val.free();
-   e->Ash.val32.free();
-   delete e;
+   (void)pk_batch_destroy(&(*E)->B);
+   delete *E;
    *E = nullptr;
    return 0;
 }

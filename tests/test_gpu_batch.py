@@ -44,6 +44,9 @@ def members(oracle_mod, tmp_path_factory):
     return out
 
 
+_MIX = {}  # precision -> [(keff, eigenvector)] of the fixed batch
+
+
 def _mats(pk, members):
     Rs, Fs = [], []
     for _, G, ref in members:
@@ -76,6 +79,7 @@ def test_batch_parity_and_alone_vs_mix(pk, members, precision):
         assert dk <= DK_PCM, name
         assert l2 <= FLUX_L2, name
         mix.append((k, xv))
+    _MIX[precision] = mix
     # a second solve of the same batch (graph replays) gives the same bits
     batch.solve()
     for b in range(len(members)):
@@ -116,3 +120,39 @@ def test_batch_initial_vector(pk, members):
     k1, _ = batch.eigenpair(1)
     assert batch.stats(1)["outer"] <= 1
     assert abs(k1 - k0) * 1e5 <= DK_PCM
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pool_continuous_batching(pk, members, precision):
+    """Members arrive in order into a pool that holds only a few at a time;
+    each is admitted when a slot and memory free up, while the others iterate.
+    Every result is bit-identical to the same member's in the fixed batch."""
+    if precision not in _MIX:
+        pytest.skip("needs the fixed-batch results of the same precision")
+    prec = pk.PK_PREC_FP64 if precision == "fp64" else pk.PK_PREC_MIXED
+    Rs, Fs = _mats(pk, members)
+    pool = pk.Pool(max_rows=70000, max_entries=1 << 22, max_fentries=1 << 21, max_binv=1 << 21,
+                   max_slots=3, precision=prec)
+    queue = list(range(len(members)))
+    slot_of, got = {}, {}
+    waits = 0
+    while queue or slot_of:
+        while queue:
+            s = pool.admit(Rs[queue[0]], Fs[queue[0]])
+            if s is None:
+                waits += 1
+                break
+            slot_of[s] = queue.pop(0)
+        live, done = pool.step(1)
+        for s in pool.collect():
+            r = pool.result(s)
+            # the batch interface reports lambda = 1/keff: compare through it
+            got[slot_of.pop(s)] = (1.0 / (1.0 / r["keff"]), r["x"].get(), r["converged"])
+            pool.release(s)
+    assert waits > 0  # admission really waited for room
+    for b, (k, x, conv) in got.items():
+        assert conv, members[b][0]
+        assert k == _MIX[precision][b][0], members[b][0]
+        assert np.array_equal(x, _MIX[precision][b][1]), members[b][0]
+    print(f"   pool: {len(got)} solves, {waits} admission waits, ticks={pool.capacity()['ticks']}",
+          flush=True)
