@@ -48,6 +48,7 @@ int pk_initialize(int device);
 int pk_finalize(void);
 int pk_synchronize(void);
 const char* pk_last_error(void);
+int pk_device_memory(double* free_bytes, double* total_bytes); /* hipMemGetInfo of the device */
 
 /* ---- vectors (petsc::create(Vec&), destroy, set, copy; petsc.cxx:113-140,
  *      236-345) ------------------------------------------------------------ */
@@ -94,11 +95,18 @@ int pk_mesh_destroy(pk_mesh* mesh);
 int pk_xs_create(int num_mat, int G, const double* sigt, const double* sigs, const double* nusf,
                  const double* kapf, const double* chi, const double* D, pk_xs* xs);
 int pk_xs_destroy(pk_xs* xs);
+/* new values for the same table sizes (a perturbed copy of a material set) */
+int pk_xs_set(pk_xs xs, const double* sigt, const double* sigs, const double* nusf, const double* kapf,
+              const double* chi, const double* D);
 /* R and F of DiffusionSolver::buildMatrices (DiffusionSolver.cxx:88-320, n = 0)
  * assembled on the device; bc_type/bc_value are indexed by bc id =
  * boundary_index + 1 (size num_bc).  Replaces the MatSetValues loop. */
 int pk_diffusion_assemble(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
                           const double* bc_value, pk_mat* R, pk_mat* F);
+/* The values of R and F (from pk_diffusion_assemble on this mesh) recomputed for
+ * new cross sections: same mesh and boundary table, so the same pattern and
+ * device layout; nothing is allocated. */
+int pk_diffusion_refill(pk_mesh mesh, pk_xs xs, pk_mat R, pk_mat F);
 /* Transient rows (DiffusionSolver.cxx:134-148,168-175): R gains
  * V/(v dt) on the diagonal and the prompt fission -(1-beta) chi_p nuSf V / keff;
  * b = chi_d S + V/(v dt) phi0.  velocity/chip/chid are [num_mat][G], beta is
@@ -166,6 +174,10 @@ int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction,
                         double delta);
 /* copies R, F (and x0 when not NULL) into the pool; the caller may destroy them after */
 int pk_pool_admit(pk_batch P, pk_mat R, pk_mat F, pk_vec x0, int* slot);
+/* *fits = 1 when pk_pool_admit(R, F) would find room now */
+int pk_pool_fits(pk_batch P, pk_mat R, pk_mat F, int* fits);
+/* device bytes one problem (R, F) holds in a pool of this precision */
+int pk_pool_footprint(pk_mat R, pk_mat F, int precision, int restart, double* bytes);
 int pk_pool_step(pk_batch P, int ticks, int* live, int* done);
 int pk_pool_collect(pk_batch P, int max, int* slots, int* count); /* finished, admission order */
 int pk_pool_result(pk_batch P, int slot, double* keff, pk_vec x, int* outer, long* inner,

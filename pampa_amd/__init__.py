@@ -102,6 +102,16 @@ def _declare(L):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = _I
+    _ULLP = ct.POINTER(ct.c_ulonglong)
+    drop.update({
+        "pampa_serve_create": [_D, _I, _I, _PP, _IP], "pampa_serve_set_options": [_P, _D, _D, _D, _IP],
+        "pampa_serve_add_case": [_P, ct.c_char_p, _IP, _P, _IP, _IP],
+        "pampa_serve_case_tables": [_P, _I, ct.c_ulonglong, _D, _P, _P, _P, _P, _IP],
+        "pampa_serve_footprint": [_P, _I, _P, _P, _IP],
+        "pampa_serve_run": [_P, _I, _P, _P, _ULLP, _D, _I, _P, _P, _P, _P, _P, _P, _IP],
+        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
+        "pampa_serve_destroy": [_PP, _IP],
+    })
     for name, args in drop.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -413,6 +423,87 @@ class Pool(Handle):
         fr, fe, fs, t = ct.c_long(), ct.c_long(), ct.c_int(), ct.c_long()
         check(lib().pk_pool_capacity(self.h, ct.byref(fr), ct.byref(fe), ct.byref(fs), ct.byref(t)))
         return dict(free_rows=fr.value, free_entries=fe.value, free_slots=fs.value, ticks=t.value)
+
+
+class Server:
+    """Serving over the pampa_serve_* C-ABI: resident base cases, requests
+    continuously batched into one device eigen pool."""
+
+    def __init__(self, budget_bytes=0.0, max_slots=1024, precision=PK_PREC_MIXED):
+        self.L = lib()
+        self.h = ct.c_void_p()
+        err = ct.c_int(0)
+        self.L.pampa_serve_create(budget_bytes, max_slots, precision, ct.byref(self.h), ct.byref(err))
+        if err.value:
+            raise PkError("pampa_serve_create failed")
+        self.rows = []
+
+    def _call(self, name, *args):
+        err = ct.c_int(0)
+        getattr(self.L, name)(*args, ct.byref(err))
+        if err.value:
+            raise PkError(f"{name} failed")
+
+    def options(self, tol=1e-8, reduction=0.1, shift=0.1):
+        self._call("pampa_serve_set_options", self.h, tol, reduction, shift)
+
+    def add_case(self, input_path: str):
+        cid, rows, groups = ct.c_int(), ct.c_long(), ct.c_int()
+        self._call("pampa_serve_add_case", self.h, input_path.encode(), ct.byref(cid), ct.byref(rows),
+                   ct.byref(groups))
+        self.rows.append(rows.value)
+        return cid.value, rows.value, groups.value
+
+    def case_tables(self, case_id, nm, G, seed, rel):
+        st, ss = np.zeros((nm, G)), np.zeros((nm, G, G))
+        D, nf = np.zeros((nm, G)), np.zeros((nm, G))
+        self._call("pampa_serve_case_tables", self.h, case_id, seed, rel, ptr(st), ptr(ss), ptr(D), ptr(nf))
+        return st, ss, D, nf
+
+    def footprint(self, case_id):
+        b, pb = ct.c_double(), ct.c_double()
+        self._call("pampa_serve_footprint", self.h, case_id, ct.byref(b), ct.byref(pb))
+        return b.value, pb.value
+
+    def run(self, arrival, case_id, seed, rel=0.02, warm=True, keep_flux=None):
+        """Open-loop stream; returns per-request arrays (keff, latency, outer,
+        inner, converged) and the fluxes of the requests in keep_flux."""
+        n = len(arrival)
+        arr = np.ascontiguousarray(arrival, dtype=np.float64)
+        cid = np.ascontiguousarray(case_id, dtype=np.int32)
+        sd = np.ascontiguousarray(seed, dtype=np.uint64)
+        keff, lat = np.zeros(n), np.zeros(n)
+        outer, conv = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        inner = np.zeros(n, np.int64)
+        fl = {}
+        fp = (ct.c_void_p * max(n, 1))()
+        for r in (keep_flux or []):
+            fl[r] = np.zeros(self.rows[cid[r]])
+            fp[r] = fl[r].ctypes.data
+        self._call("pampa_serve_run", self.h, n, ptr(arr), ptr(cid),
+                   sd.ctypes.data_as(ct.POINTER(ct.c_ulonglong)), rel, int(warm), ptr(keff), ptr(lat),
+                   ptr(outer), ptr(inner), ptr(conv), fp)
+        return dict(keff=keff, latency=lat, outer=outer, inner=inner, converged=conv.astype(bool), flux=fl)
+
+    def forget_warm(self):
+        self._call("pampa_serve_forget_warm", self.h)
+
+    def stats(self):
+        s, t, m, pb = ct.c_double(), ct.c_long(), ct.c_int(), ct.c_double()
+        self._call("pampa_serve_stats", self.h, ct.byref(s), ct.byref(t), ct.byref(m), ct.byref(pb))
+        return dict(seconds=s.value, ticks=t.value, max_live=m.value, pool_bytes=pb.value)
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            err = ct.c_int(0)
+            self.L.pampa_serve_destroy(ct.byref(self.h), ct.byref(err))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DropIn:

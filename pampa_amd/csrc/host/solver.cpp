@@ -89,23 +89,31 @@ int NeutronicSolver::boundaryTable(double t) {
    return 0;
 }
 
-int NeutronicSolver::uploadMaterials(bool diffusion) {
+int NeutronicSolver::materialTables(bool diffusion, XSTables& t) const {
    const int nm = (int)materials.size();
    PAMPA_CHECK(nm == 0, "no materials defined");
-   std::vector<double> st, ss, nf, kf, ch, D;
+   t = XSTables();
+   t.nm = nm;
+   t.G = G;
    for (const auto& m : materials) {
       PAMPA_CHECK(m.check(G, diffusion, transient), "wrong nuclear data in material '" + m.name + "'");
-      st.insert(st.end(), m.sigma_total.begin(), m.sigma_total.end());
-      ss.insert(ss.end(), m.sigma_scattering.begin(), m.sigma_scattering.end());
-      nf.insert(nf.end(), m.nu_sigma_fission.begin(), m.nu_sigma_fission.end());
-      kf.insert(kf.end(), m.kappa_sigma_fission.begin(), m.kappa_sigma_fission.end());
-      ch.insert(ch.end(), m.chi_effective.begin(), m.chi_effective.end());
-      if (diffusion) D.insert(D.end(), m.diffusion_coefficient.begin(), m.diffusion_coefficient.end());
+      t.st.insert(t.st.end(), m.sigma_total.begin(), m.sigma_total.end());
+      t.ss.insert(t.ss.end(), m.sigma_scattering.begin(), m.sigma_scattering.end());
+      t.nf.insert(t.nf.end(), m.nu_sigma_fission.begin(), m.nu_sigma_fission.end());
+      t.kf.insert(t.kf.end(), m.kappa_sigma_fission.begin(), m.kappa_sigma_fission.end());
+      t.ch.insert(t.ch.end(), m.chi_effective.begin(), m.chi_effective.end());
+      if (diffusion) t.D.insert(t.D.end(), m.diffusion_coefficient.begin(), m.diffusion_coefficient.end());
    }
+   return 0;
+}
+
+int NeutronicSolver::uploadMaterials(bool diffusion) {
+   XSTables t;
+   PAMPA_CHECK(materialTables(diffusion, t), "wrong material data");
    for (int i = 0; i < mesh->num_cells; i++)
-      PAMPA_CHECK(mesh->mat[i] < 0 || mesh->mat[i] >= nm, "wrong material index in the mesh");
-   PK_CHECK(pk_xs_create(nm, G, st.data(), ss.data(), nf.data(), kf.data(), ch.data(),
-                         diffusion ? D.data() : nullptr, &dxs));
+      PAMPA_CHECK(mesh->mat[i] < 0 || mesh->mat[i] >= t.nm, "wrong material index in the mesh");
+   PK_CHECK(pk_xs_create(t.nm, G, t.st.data(), t.ss.data(), t.nf.data(), t.kf.data(), t.ch.data(),
+                         diffusion ? t.D.data() : nullptr, &dxs));
    return 0;
 }
 
@@ -164,6 +172,12 @@ int NeutronicSolver::getStats(int* outer, long* inner, double* residual, double*
 }
 
 // ---- DiffusionSolver ------------------------------------------------------------------
+
+int DiffusionSolver::assembleBase(pk_mat* Rb, pk_mat* Fb) {
+   PAMPA_CHECK(boundaryTable(0.0), "wrong boundary conditions");
+   PK_CHECK(pk_diffusion_assemble(dmesh, dxs, (int)bc_type.size(), bc_type.data(), bc_value.data(), Rb, Fb));
+   return 0;
+}
 
 int DiffusionSolver::build() {
    PAMPA_CHECK(uploadMaterials(true), "wrong material data");

@@ -26,6 +26,12 @@ struct SolverOptions {
 };
 extern SolverOptions options;
 
+// Per-material tables in the layout of pk_xs_create: [nm][G] and sigs [nm][G][G].
+struct XSTables {
+   int nm = 0, G = 0;
+   std::vector<double> st, ss, nf, kf, ch, D;
+};
+
 struct Field {
    std::string name;
    pk_vec* vec = nullptr;
@@ -77,6 +83,10 @@ class NeutronicSolver : public Solver {
       if (directions) *directions = numDirections();
    }
    virtual int numDirections() const { return 1; }
+   int materialTables(bool diffusion, XSTables& t) const;
+   pk_mesh deviceMesh() const { return dmesh; }
+   long numCells() const { return mesh->num_cells; }
+   int groups() const { return G; }
 
   protected:
    virtual int readKeyword(const std::vector<std::string>& line, std::istream& in);
@@ -111,6 +121,8 @@ class DiffusionSolver : public NeutronicSolver {
    int getOperator(const std::string& which, long* n, long* nnz, int* ptr, int* col,
                    double* val) const override;
    int timeKernel(int reps, int fp32, double* seconds, double* bytes) override;
+   // R and F of the input as given (steady state, t = 0), owned by the caller
+   int assembleBase(pk_mat* Rb, pk_mat* Fb);
 
   protected:
    int build() override;

@@ -440,6 +440,10 @@ static int diffusion_assemble_impl(pk_mesh mesh, pk_xs xs, int num_bc, const int
    }
    PK_CALL(finalize_device_csr(r));
    if (f) PK_CALL(finalize_device_csr(f));
+   if (!vel) {
+      r->asm_bct = std::move(dbct);
+      r->asm_bcv = std::move(dbcv);
+   }
    *R = r;
    if (F) *F = f;
    return 0;
@@ -449,6 +453,53 @@ int pk_diffusion_assemble(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type
                           const double* bc_value, pk_mat* R, pk_mat* F) {
    return diffusion_assemble_impl(mesh, xs, num_bc, bc_type, bc_value, nullptr, 0.0, nullptr,
                                   nullptr, 1.0, R, F);
+}
+
+int pk_diffusion_refill(pk_mesh mesh, pk_xs xs, pk_mat R, pk_mat F) {
+   PK_REQUIRE(mesh && xs && R && F && R->assembled && F->assembled, "null or unassembled argument");
+   PK_REQUIRE(R->asm_bct.p, "R does not come from a steady diffusion assembly");
+   const int G = xs->G;
+   const long N = (long)mesh->C * G;
+   PK_REQUIRE(R->n == N && F->n == N && R->bs == G && F->nnz == N * G,
+              "mesh or group count differ from the assembly of R and F");
+   Ctx& c = ctx();
+   DBuf<int>& derr = R->asm_err;
+   if (!derr.p) PK_REQUIRE(!derr.alloc(1), "device allocation failed");
+   PK_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), c.stream));
+   // same mesh and boundary table: the pattern (and SELL layout) is unchanged
+   diff_fill_kernel<<<grid_for(N), kBlock, 0, c.stream>>>(view(mesh), view(xs), R->asm_bct.p, R->asm_bcv.p,
+                                                          R->ptr.p, R->col.p, R->val.p, F->col.p, F->val.p,
+                                                          nullptr, 0.0, nullptr, nullptr, 1.0, derr.p);
+   PK_HIP(hipGetLastError());
+   PK_CALL(refresh_sell_values(R));
+   PK_CALL(refresh_sell_values(F));
+   int herr = 0;
+   PK_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   PK_REQUIRE(herr == 0, herr == 1 ? "boundary condition not implemented" : "too many neighbours per cell");
+   return 0;
+}
+
+int pk_xs_set(pk_xs xs, const double* sigt, const double* sigs, const double* nusf, const double* kapf,
+              const double* chi, const double* D) {
+   PK_REQUIRE(xs && sigt && sigs && nusf && kapf && chi, "null argument");
+   Ctx& c = ctx();
+   const size_t n1 = (size_t)xs->nm * xs->G, n2 = n1 * xs->G;
+   xs->h_sigt.assign(sigt, sigt + n1);
+   xs->h_sigs.assign(sigs, sigs + n2);
+   xs->h_nusf.assign(nusf, nusf + n1);
+   xs->h_kapf.assign(kapf, kapf + n1);
+   xs->h_chi.assign(chi, chi + n1);
+   if (D) xs->h_D.assign(D, D + n1);
+   // ordered on the library stream after any kernel still reading the tables
+   PK_HIP(hipMemcpyAsync(xs->sigt.p, xs->h_sigt.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->sigs.p, xs->h_sigs.data(), sizeof(double) * n2, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->nusf.p, xs->h_nusf.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->kapf.p, xs->h_kapf.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->chi.p, xs->h_chi.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->D.p, xs->h_D.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
 }
 
 int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,

@@ -1337,6 +1337,28 @@ int pk_pool_admit(pk_batch P, pk_mat R, pk_mat F, pk_vec x0, int* slot) {
    return pool_admit(P, R, F, x0d, slot);
 }
 
+int pk_pool_fits(pk_batch P, pk_mat R, pk_mat F, int* fits) {
+   PK_REQUIRE(P && fits && P->sd.p, "wrong pool arguments");
+   PK_CALL(check_member(R, F));
+   const long units = units_of(R->n);
+   const long re = blocks64(R->sell.padded), fe = blocks64(F->sell.padded);
+   const long bi = blocks64(R->n * preconditioner_block(R->bs));
+   bool slot = false;
+   for (auto& s : P->slots) slot |= !s.used;
+   *fits = slot && P->ua.largest_free() >= units && P->ra.largest_free() >= re && P->fa.largest_free() >= fe &&
+           P->ba.largest_free() >= bi;
+   return 0;
+}
+
+int pk_pool_footprint(pk_mat R, pk_mat F, int precision, int restart, double* bytes) {
+   PK_REQUIRE(bytes && restart >= 1 && restart <= kBMaxM, "wrong arguments");
+   PK_CALL(check_member(R, F));
+   const long units = units_of(R->n);
+   *bytes = pool_bytes(units, blocks64(R->sell.padded), blocks64(F->sell.padded),
+                       blocks64(R->n * preconditioner_block(R->bs)), 1, restart, precision);
+   return 0;
+}
+
 int pk_pool_step(pk_batch P, int ticks, int* live, int* done) {
    PK_REQUIRE(P && P->sd.p && ticks >= 0, "wrong pool arguments");
    return pool_step(P, ticks, live, done);

@@ -87,6 +87,16 @@ struct DBuf {
       o.p = nullptr;
       o.n = 0;
    }
+   DBuf& operator=(DBuf&& o) noexcept {
+      if (this != &o) {
+         free();
+         p = o.p;
+         n = o.n;
+         o.p = nullptr;
+         o.n = 0;
+      }
+      return *this;
+   }
    ~DBuf() { free(); }
    int alloc(size_t count) {
       free();
@@ -144,6 +154,10 @@ struct pk_mat_s {
    pk::DBuf<int> ptr, col;
    pk::DBuf<double> val;
    SellData sell;  // SpMV copy; FP32 values built on demand for mixed precision
+   // boundary table of a steady diffusion assembly (kept for value refills)
+   pk::DBuf<int> asm_bct;
+   pk::DBuf<double> asm_bcv;
+   pk::DBuf<int> asm_err;
    int ensure_fp32();
 };
 
@@ -169,5 +183,6 @@ namespace pk {
 int spmv(pk_mat_s* A, const double* x, double* y);
 int spmv_f(pk_mat_s* A, const float* x, float* y);
 int build_sell(pk_mat_s* A);
+int refresh_sell_values(pk_mat_s* A);
 int finalize_device_csr(pk_mat_s* A);
 }  // namespace pk

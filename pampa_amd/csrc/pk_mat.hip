@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <numeric>
 
+#include <hipcub/hipcub.hpp>
+
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
 
@@ -19,59 +21,101 @@ namespace pk {
 
 constexpr int kSlice = 64;
 
-int build_sell(pk_mat_s* A) {
-   // host pattern -> SELL-64
-   long n = A->n;
-   std::vector<int> hptr(n + 1), hcol(A->nnz);
-   std::vector<double> hval(A->nnz);
-   PK_HIP(hipMemcpy(hptr.data(), A->ptr.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
-   if (A->nnz) {
-      PK_HIP(hipMemcpy(hcol.data(), A->col.p, sizeof(int) * A->nnz, hipMemcpyDeviceToHost));
-      PK_HIP(hipMemcpy(hval.data(), A->val.p, sizeof(double) * A->nnz, hipMemcpyDeviceToHost));
+__global__ void to_f32_kernel(const double* a, float* b, long n) {
+   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) b[i] = (float)a[i];
+}
+
+// width of slice s = longest row among its 64 rows, as entries x 64
+__global__ void sell_width_kernel(long n, long ns, const int* ptr, int* width, long* wl) {
+   long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (s > ns) return;
+   if (s == ns) {
+      wl[ns] = 0;
+      return;
    }
-   long ns = (n + kSlice - 1) / kSlice;
-   std::vector<long> off(ns + 1, 0);
-   std::vector<int> width(ns, 0);
-   for (long s = 0; s < ns; s++) {
-      int w = 0;
-      for (long r = s * kSlice; r < std::min(n, (s + 1) * kSlice); r++) w = std::max(w, hptr[r + 1] - hptr[r]);
-      width[s] = w;
-      off[s + 1] = off[s] + (long)w * kSlice;
-   }
-   long tot = off[ns];
-   std::vector<int> scol(tot);
-   std::vector<double> sval(tot, 0.0);
-   for (long s = 0; s < ns; s++)
-      for (int r = 0; r < kSlice; r++) {
-         long row = s * kSlice + r;
-         for (int k = 0; k < width[s]; k++) {
-            long p = off[s] + (long)k * kSlice + r;
-            if (row < n && k < hptr[row + 1] - hptr[row]) {
-               scol[p] = hcol[hptr[row] + k];
-               sval[p] = hval[hptr[row] + k];
-            } else {
-               scol[p] = (int)(row < n ? row : 0);
-               sval[p] = 0.0;
-            }
-         }
+   int w = 0;
+   const long r1 = min(n, (s + 1) * (long)kSlice);
+   for (long r = s * kSlice; r < r1; r++) w = max(w, ptr[r + 1] - ptr[r]);
+   width[s] = w;
+   wl[s] = (long)w * kSlice;
+}
+
+// one thread per row: its entries (or padding) at off[s] + 64 k + lane;
+// with cols == false only the values are written (same pattern)
+__global__ void sell_fill_kernel(long n, long nrows_pad, const int* ptr, const int* col, const double* val,
+                                 const long* off, const int* width, int* scol, double* sval, int* rlen,
+                                 bool cols) {
+   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (row >= nrows_pad) return;
+   const long s = row / kSlice;
+   const int lane = (int)(row % kSlice);
+   const int w = width[s];
+   const long a = row < n ? ptr[row] : 0;
+   const int len = row < n ? ptr[row + 1] - ptr[row] : 0;
+   for (int k = 0; k < w; k++) {
+      const long p = off[s] + (long)k * kSlice + lane;
+      if (k < len) {
+         if (cols) scol[p] = col[a + k];
+         sval[p] = val[a + k];
+      } else {
+         if (cols) scol[p] = (int)(row < n ? row : 0);
+         sval[p] = 0.0;
       }
+   }
+   if (cols && row < n) rlen[row] = len;
+}
+
+int build_sell(pk_mat_s* A) {
+   // device CSR -> SELL-64: slice widths, offsets by a scan, then the fill
+   Ctx& c = ctx();
+   const long n = A->n;
+   const long ns = (n + kSlice - 1) / kSlice;
    SellData& S = A->sell;
    S.nslices = ns;
-   S.padded = tot;
    S.nnz = A->nnz;
-   PK_REQUIRE(!S.off.alloc(ns + 1) && !S.width.alloc(ns) && !S.col.alloc(tot ? tot : 1) &&
-                  !S.val.alloc(tot ? tot : 1) && !S.rlen.alloc(n ? n : 1),
+   DBuf<long> wl;
+   PK_REQUIRE(!S.off.alloc(ns + 1) && !S.width.alloc(ns ? ns : 1) && !S.rlen.alloc(n ? n : 1) &&
+                  !wl.alloc(ns + 1),
               "device allocation failed");
-   std::vector<int> rl(n);
-   for (long r = 0; r < n; r++) rl[r] = hptr[r + 1] - hptr[r];
-   if (n) PK_HIP(hipMemcpy(S.rlen.p, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-   PK_HIP(hipMemcpy(S.off.p, off.data(), sizeof(long) * (ns + 1), hipMemcpyHostToDevice));
-   PK_HIP(hipMemcpy(S.width.p, width.data(), sizeof(int) * ns, hipMemcpyHostToDevice));
-   if (tot) {
-      PK_HIP(hipMemcpy(S.col.p, scol.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
-      PK_HIP(hipMemcpy(S.val.p, sval.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+   sell_width_kernel<<<grid_for(ns + 1), kBlock, 0, c.stream>>>(n, ns, A->ptr.p, S.width.p, wl.p);
+   PK_HIP(hipGetLastError());
+   size_t tmp_bytes = 0;
+   PK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, wl.p, S.off.p, (int)(ns + 1), c.stream));
+   DBuf<char> tmp;
+   PK_REQUIRE(!tmp.alloc(tmp_bytes ? tmp_bytes : 1), "device allocation failed");
+   PK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, wl.p, S.off.p, (int)(ns + 1), c.stream));
+   long tot = 0;
+   PK_HIP(hipMemcpyAsync(&tot, S.off.p + ns, sizeof(long), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   S.padded = tot;
+   PK_REQUIRE(!S.col.alloc(tot ? tot : 1) && !S.val.alloc(tot ? tot : 1), "device allocation failed");
+   if (n) {
+      sell_fill_kernel<<<grid_for(ns * kSlice), kBlock, 0, c.stream>>>(n, ns * kSlice, A->ptr.p, A->col.p,
+                                                                     A->val.p, S.off.p, S.width.p, S.col.p,
+                                                                     S.val.p, S.rlen.p, true);
+      PK_HIP(hipGetLastError());
    }
+   PK_HIP(hipStreamSynchronize(c.stream));
    S.val32.free();
+   return 0;
+}
+
+// new CSR values on the same pattern: refresh the SELL values (and drop the FP32 copy)
+int refresh_sell_values(pk_mat_s* A) {
+   Ctx& c = ctx();
+   SellData& S = A->sell;
+   if (A->n) {
+      sell_fill_kernel<<<grid_for(S.nslices * kSlice), kBlock, 0, c.stream>>>(
+          A->n, S.nslices * kSlice, A->ptr.p, A->col.p, A->val.p, S.off.p, S.width.p, nullptr, S.val.p, nullptr,
+          false);
+      PK_HIP(hipGetLastError());
+   }
+   if (S.val32.p) {
+      long tot = S.padded ? S.padded : 1;
+      to_f32_kernel<<<grid_for(tot), kBlock, 0, c.stream>>>(S.val.p, S.val32.p, S.padded);
+      PK_HIP(hipGetLastError());
+   }
    return 0;
 }
 
@@ -97,10 +141,6 @@ __global__ void __launch_bounds__(256) sell_spmv_kernel(long n, const long* __re
    y[row] = acc;
 }
 
-__global__ void to_f32_kernel(const double* a, float* b, long n) {
-   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (i < n) b[i] = (float)a[i];
-}
 
 // algorithmic bytes of a CSR SpMV: nnz (value + column) + row pointers + x, y
 static double spmv_bytes(const pk_mat_s* A, double s) {

@@ -173,6 +173,15 @@ int pk_initialize(int device) {
    return 0;
 }
 
+int pk_device_memory(double* free_bytes, double* total_bytes) {
+   PK_CALL(ensure_init());
+   size_t f = 0, t = 0;
+   PK_HIP(hipMemGetInfo(&f, &t));
+   if (free_bytes) *free_bytes = (double)f;
+   if (total_bytes) *total_bytes = (double)t;
+   return 0;
+}
+
 int pk_finalize(void) {
    Ctx& c = ctx();
    if (!c.stream) return 0;

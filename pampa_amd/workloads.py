@@ -130,6 +130,41 @@ def perturb(m: dict, rng: SplitMix64, rel: float):
     return out
 
 
+def perturb_tables(sigt, sigs, D, nusf, seed: int, rel: float):
+    """Cross-section tables of a served request: every total, scattering,
+    diffusion and nu-fission value of every material scaled by its own
+    U[1-rel, 1+rel] factor, drawn material by material in the order sigt[g],
+    sigs[g][g'] (row-major), D[g], nusf[g]; then sigt[g] = max(sigt[g],
+    sum_g' sigs[g][g']).  Seed 0 leaves the tables as they are.  Arrays are
+    (nm, G) and (nm, G, G); returns new arrays."""
+    import numpy as np
+    sigt, sigs = np.array(sigt, dtype=float), np.array(sigs, dtype=float)
+    D = None if D is None else np.array(D, dtype=float)
+    nusf = np.array(nusf, dtype=float)
+    if seed == 0 or rel == 0.0:
+        return sigt, sigs, D, nusf
+    rng = SplitMix64(seed)
+    lo, hi = 1.0 - rel, 1.0 + rel
+    nm, G = sigt.shape
+    for m in range(nm):
+        for g in range(G):
+            sigt[m, g] = float(sigt[m, g]) * rng.uniform(lo, hi)
+        for g in range(G):
+            for h in range(G):
+                sigs[m, g, h] = float(sigs[m, g, h]) * rng.uniform(lo, hi)
+        if D is not None:
+            for g in range(G):
+                D[m, g] = float(D[m, g]) * rng.uniform(lo, hi)
+        for g in range(G):
+            nusf[m, g] = float(nusf[m, g]) * rng.uniform(lo, hi)
+        for g in range(G):
+            tot = 0.0
+            for h in range(G):
+                tot += float(sigs[m, g, h])
+            sigt[m, g] = max(float(sigt[m, g]), tot)
+    return sigt, sigs, D, nusf
+
+
 def _write_input(d: str, names, solver_lines):
     lines = ["mesh {} mesh.pmp".format(solver_lines[0])]
     for n in names:
