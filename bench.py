@@ -112,6 +112,148 @@ def sweep_numbers(pa, d):
         os.chdir(cwd)
 
 
+def _shape(v):
+    v = np.asarray(v, dtype=float)
+    return v / v.sum()
+
+
+def _oracle_request(oracle, W, path, seed, rel):
+    """FP64 oracle solve of a served request (perturbed tables of a base case)."""
+    case = oracle.pmp.read_input(path)
+    mesh = oracle.build_mesh(case)
+    s = case.solver
+    xs = oracle.xs_tables(case.materials, s.G, diffusion=True)
+    xs.sigt, xs.sigs, xs.D, xs.nusf = W.perturb_tables(xs.sigt, xs.sigs, xs.D, xs.nusf, seed, rel)
+    bct, bcv = oracle.bc_table(mesh, s.bcs)
+    R, F = oracle.assemble_diffusion(mesh, xs, bct, bcv)
+    return oracle.eigen_sinvert(R, F)
+
+
+def _accuracy(oracle, W, res, paths, cid, seeds, rel, sample):
+    out = []
+    for r in sample:
+        k_o, x_o = _oracle_request(oracle, W, paths[cid[r]], int(seeds[r]), rel)
+        out.append({"request": int(r), "dk_pcm": abs(res["keff"][r] - k_o) * 1e5,
+                    "flux_l2": float(np.linalg.norm(_shape(res["flux"][r]) - _shape(x_o)) /
+                                     np.linalg.norm(_shape(x_o)))})
+    return {"max_dk_pcm": max(a["dk_pcm"] for a in out), "max_flux_l2": max(a["flux_l2"] for a in out),
+            "requests": out}
+
+
+def _serve(pa, paths, precision, slots, arrival, cid, seeds, rel, warm, keep=(), budget=64e9):
+    srv = pa.Server(budget_bytes=budget, max_slots=slots, precision=precision)
+    t0 = time.perf_counter()
+    for p in paths:
+        srv.add_case(p)
+    log(f"[bench]   {len(paths)} cases resident ({time.perf_counter() - t0:.1f} s)")
+    if warm:
+        # one unperturbed solve per base case first: the stream's warm starts
+        srv.run(np.zeros(len(paths)), np.arange(len(paths)), np.zeros(len(paths), np.uint64), rel=0.0,
+                warm=False)
+    res = srv.run(arrival, cid, seeds, rel=rel, warm=warm, keep_flux=list(keep))
+    st = srv.stats()
+    log(f"[bench]   {len(arrival)} requests done, makespan {float((np.asarray(arrival) + res['latency']).max()):.2f} s")
+    srv.close()
+    return res, st
+
+
+def _family(W, tmp, n):
+    """Instances 0..n-1 of the batched family: (paths, rows per instance)."""
+    paths, rows = [], []
+    for i in range(n):
+        d = os.path.join(tmp, f"inst-{i}")
+        side, G = W.batched_instance(d, i)
+        paths.append(os.path.join(d, "input.pmp"))
+        rows.append(side * side * G)
+    return paths, np.array(rows)
+
+
+def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
+    """Config-3 family: n_inst independent instances (sides and group counts
+    drawn by seed, cross sections perturbed +-5% per instance), all admitted at
+    once into one mixed-precision pool; the plain FP64 drop-in (one solve at a
+    time) on the first fp64_sample of them, and the pool on the same subset."""
+    paths, rows = _family(W, tmp, n_inst)
+    n = n_inst
+    cid = np.arange(n)
+    seeds = np.zeros(n, np.uint64)
+    small = [int(i) for i in np.argsort(rows, kind="stable")[:4]]
+    log(f"[bench] batched: {n} instances ({rows.sum()} rows) in one mixed pool, profiled pass")
+    pa.profile(True)
+    _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), cid, seeds, 0.0, False)
+    prof = {}
+    for name in ("gmres_prec", "gmres_spmv_dot", "gmres_orth"):
+        cnt, sec, byt = pa.profile_get(name)
+        if cnt:
+            prof[name] = dict(launches=cnt, seconds=sec, bytes=byt)
+    pa.profile(False)
+    log("[bench] batched: timed pass")
+    res, st = _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), cid, seeds, 0.0, False, keep=small)
+    spmv = {}
+    for name, prec in (("fp32", pa.PK_PREC_MIXED), ("fp64", pa.PK_PREC_FP64)):
+        srv = pa.Server(budget_bytes=64e9, max_slots=n, precision=prec)
+        for p in paths:
+            srv.add_case(p)
+        sec, byt, nrows = srv.time_spmv(20)
+        srv.close()
+        spmv[name] = {"us": sec * 1e6, "bytes": byt, "GBs": byt / sec / 1e9,
+                      "frac_hbm": byt / sec / 1e9 / HBM_PEAK_GBS, "rows": nrows}
+        log(f"[bench]   batched SpMV {name}: {byt / sec / 1e9:.0f} GB/s")
+    mixed_all = n / res["latency"].max()
+    acc = _accuracy(oracle, W, res, paths, cid, seeds, 0.0, small)
+    k = fp64_sample
+    log(f"[bench] batched: FP64 drop-in, one at a time, on {k} of them")
+    r64, _ = _serve(pa, paths[:k], pa.PK_PREC_FP64, 1, np.zeros(k), np.arange(k), np.zeros(k, np.uint64), 0.0,
+                    False)
+    rmx, _ = _serve(pa, paths[:k], pa.PK_PREC_MIXED, k, np.zeros(k), np.arange(k), np.zeros(k, np.uint64),
+                    0.0, False)
+    fp64_rate = k / r64["latency"].max()
+    mixed_rate = k / rmx["latency"].max()
+    roof = None
+    if prof:
+        name, dv = max(prof.items(), key=lambda kv: kv[1]["seconds"])
+        ach = dv["bytes"] / dv["seconds"] / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "avg_launch_us": dv["seconds"] / dv["launches"] * 1e6,
+                "bytes_per_launch": dv["bytes"] / dv["launches"], "launches": dv["launches"]}
+    return {"workload": f"{n} instances of the batched family (sides {list(W.BATCH_SIDES)}, groups "
+                        f"{list(W.BATCH_GROUPS)}, +-5% cross sections), {int(rows.sum())} rows, all "
+                        f"admitted at once",
+            "mixed_solves_per_s_all": mixed_all, "makespan_s_all": float(res["latency"].max()),
+            "ticks_all": st["ticks"], "pool_bytes": st["pool_bytes"],
+            "subset": k, "subset_rows": int(rows[:k].sum()), "fp64_dropin_solves_per_s": fp64_rate,
+            "mixed_batched_solves_per_s": mixed_rate, "mixed_over_fp64": mixed_rate / fp64_rate,
+            "accuracy_vs_fp64_oracle": acc, "outer_mean": float(res["outer"].mean()),
+            "inner_mean": float(res["inner"].mean()), "profiled_kernels": prof, "roofline": roof,
+            "spmv_all_instances": spmv}
+
+
+def serving_numbers(pa, W, oracle, tmp, n_base=16, n_req=300, rate=200.0, rel=0.02, seed=11):
+    """Open-loop Poisson stream of seeded +-2% perturbations of n_base
+    previously seen base cases (the batched family), warm-started: the mixed
+    pool with continuous batching vs the plain FP64 drop-in (one at a time) on
+    the same stream."""
+    paths, rows = _family(W, tmp, n_base)
+    rng = np.random.default_rng(seed)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    arrival -= arrival[0]
+    cid = rng.integers(0, n_base, n_req)
+    seeds = np.arange(1, n_req + 1, dtype=np.uint64)
+    small = [r for r in range(n_req) if rows[cid[r]] <= 20000][:4]
+    out = {"workload": f"{n_req} requests, Poisson {rate:g} req/s, {n_base} base cases of the batched "
+                       f"family ({int(rows.sum())} rows), +-{rel:g} seeded perturbations, warm start"}
+    for name, prec, slots in (("mixed", pa.PK_PREC_MIXED, 256), ("fp64_dropin", pa.PK_PREC_FP64, 1)):
+        log(f"[bench] serving: {name}")
+        res, st = _serve(pa, paths, prec, slots, arrival, cid, seeds, rel, True, keep=small)
+        span = float((arrival + res["latency"]).max())
+        out[name] = {"solves_per_s": n_req / span, "p50_ms": float(np.percentile(res["latency"], 50) * 1e3),
+                     "p99_ms": float(np.percentile(res["latency"], 99) * 1e3), "makespan_s": span,
+                     "max_in_flight": st["max_live"], "converged": int(res["converged"].sum()),
+                     "accuracy_vs_fp64_oracle": _accuracy(oracle, W, res, paths, cid, seeds, rel, small)}
+    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64_dropin"]["solves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +261,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sweep", action="store_true", help="skip the S_N sweep numbers")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-family numbers")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving numbers")
+    ap.add_argument("--batch", type=int, default=256, help="instances in the batched run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -189,6 +334,12 @@ def main():
         sweep = sweep_numbers(pa, d2)
         sweep["workload"] = "hex lattice 331 columns x 24 triangles x 40 layers, G=4, S6"
 
+    batched = serving = None
+    if not args.no_batched:
+        batched = batched_numbers(pa, W, oracle, os.path.join(tmp, "family"), n_inst=args.batch)
+    if not args.no_serving:
+        serving = serving_numbers(pa, W, oracle, os.path.join(tmp, "family"))
+
     out = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 / value, "higher_is_better": True,
@@ -199,7 +350,8 @@ def main():
                    "unknowns": 57800, "nnz_R": 345440},
         "fp64_value": fp64_value, "mixed_over_fp64": value / fp64_value,
         "latency": lat, "accuracy_vs_fp64_oracle": acc,
-        "roofline": roofline, "spmv": spmv, "sweep": sweep, "cpu_baseline": cpu,
+        "roofline": roofline, "spmv": spmv, "sweep": sweep, "batched": batched, "serving": serving,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

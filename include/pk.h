@@ -184,6 +184,10 @@ int pk_pool_result(pk_batch P, int slot, double* keff, pk_vec x, int* outer, lon
                    double* residual, int* converged);
 int pk_pool_release(pk_batch P, int slot);
 int pk_pool_capacity(pk_batch P, long* free_rows, long* free_entries, int* free_slots, long* ticks);
+/* Time the SpMV of every live problem's inner operator in one launch (HIP
+ * events on the library stream): average seconds per launch over `reps`, and
+ * the algorithmic CSR bytes of one launch (unpadded nonzeros). */
+int pk_pool_time_spmv(pk_batch P, int reps, double* avg_seconds, double* bytes);
 
 /* ---- linear solver (petsc::create(KSP&) petsc.cxx:147-172, petsc::solve(KSP)
  *      petsc.cxx:382-425): GMRES + cell-block Jacobi, nonzero initial guess. */

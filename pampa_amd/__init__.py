@@ -109,7 +109,7 @@ def _declare(L):
         "pampa_serve_case_tables": [_P, _I, ct.c_ulonglong, _D, _P, _P, _P, _P, _IP],
         "pampa_serve_footprint": [_P, _I, _P, _P, _IP],
         "pampa_serve_run": [_P, _I, _P, _P, _ULLP, _D, _I, _P, _P, _P, _P, _P, _P, _IP],
-        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
+        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_time_spmv": [_P, _I, _P, _P, _P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
         "pampa_serve_destroy": [_PP, _IP],
     })
     for name, args in drop.items():
@@ -484,6 +484,12 @@ class Server:
                    sd.ctypes.data_as(ct.POINTER(ct.c_ulonglong)), rel, int(warm), ptr(keff), ptr(lat),
                    ptr(outer), ptr(inner), ptr(conv), fp)
         return dict(keff=keff, latency=lat, outer=outer, inner=inner, converged=conv.astype(bool), flux=fl)
+
+    def time_spmv(self, reps=20):
+        """(seconds per launch, algorithmic bytes, rows) of one SpMV over every case."""
+        t, b, r = ct.c_double(), ct.c_double(), ct.c_double()
+        self._call("pampa_serve_time_spmv", self.h, reps, ct.byref(t), ct.byref(b), ct.byref(r))
+        return t.value, b.value, r.value
 
     def forget_warm(self):
         self._call("pampa_serve_forget_warm", self.h)

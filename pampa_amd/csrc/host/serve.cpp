@@ -316,6 +316,39 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
    *error = 0;
 }
 
+void pampa_serve_time_spmv(pampa_server S, int reps, double* seconds, double* bytes, double* rows, int* error) {
+   *error = 1;
+   if (!S || reps < 1 || S->cases.empty()) return;
+   // every case once (unperturbed) in the pool, one tick to build the inner
+   // operators, the timed SpMV launch over all of them, then release
+   std::vector<int> slots;
+   double nrows = 0.0;
+   auto cleanup = [&] {
+      for (int sl : slots) pk_pool_release(S->pool, sl);
+   };
+   for (auto& cp : S->cases) {
+      Case& c = *cp;
+      int slot = -1;
+      if (S->requestOperators(c, 0, 0.0)) return cleanup();
+      int rc = pk_pool_admit(S->pool, c.R, c.F, nullptr, &slot);
+      if (rc) {
+         report_error(__FILE__, __LINE__, __func__, rc == PK_POOL_FULL ? "cases do not fit the pool together"
+                                                                     : pk_last_error());
+         return cleanup();
+      }
+      slots.push_back(slot);
+      nrows += (double)c.n;
+   }
+   int live = 0, done = 0;
+   if (pk_pool_step(S->pool, 1, &live, &done) || pk_pool_time_spmv(S->pool, reps, seconds, bytes)) {
+      report_error(__FILE__, __LINE__, __func__, pk_last_error());
+      return cleanup();
+   }
+   if (rows) *rows = nrows;
+   cleanup();
+   *error = 0;
+}
+
 void pampa_serve_forget_warm(pampa_server S, int* error) {
    *error = S ? 0 : 1;
    if (S)

@@ -31,6 +31,7 @@
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
+#include "pk_sell.hpp"
 
 namespace pk {
 
@@ -94,6 +95,8 @@ struct BOpts {
 __device__ __forceinline__ int slot_of_wg(const BLayout& L) { return L.unit_slot[blockIdx.x]; }
 
 __device__ __forceinline__ int slot_of_row(const BLayout& L, long row) { return L.unit_slot[row / kRows]; }
+
+
 
 // ---- admission: place a problem's SELL arrays in the pool --------------------------
 
@@ -252,17 +255,8 @@ __global__ void __launch_bounds__(kRows) brf_kernel(BLayout L, BSell R, const do
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const int lane = threadIdx.x & 63;
-   double a = 0.0, c = 0.0;
-   long base = R.off[s] + lane;
-   for (int k = 0; k < R.width[s]; k++) {
-      long p = base + (long)k * 64;
-      a += rval[p] * x[R.col[p]];
-   }
-   base = F.off[s] + lane;
-   for (int k = 0; k < F.width[s]; k++) {
-      long p = base + (long)k * 64;
-      c += fval[p] * x[F.col[p]];
-   }
+   const double a = sell_row_dot<double, double>(R.col, rval, x, R.off[s] + lane, R.width[s], 0.0);
+   const double c = sell_row_dot<double, double>(F.col, fval, x, F.off[s] + lane, F.width[s], 0.0);
    y1[row] = a;
    y2[row] = c;
    const double xv = x[row];
@@ -390,12 +384,7 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const long base = A.off[s] + (threadIdx.x & 63);
-   const int wd = A.width[s];
-   T acc = 0;
-   for (int k = 0; k < wd; k++) {
-      long p = base + (long)k * 64;
-      acc += aval[p] * z[A.col[p]];
-   }
+   const T acc = sell_row_dot<T, T>(A.col, aval, z, base, A.width[s], (T)0);
    w[row] = acc;
    const double wr = (double)acc;
    const int nv = j + 1;
@@ -527,12 +516,7 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const long base = A.off[s] + (threadIdx.x & 63);
-   const int wd = A.width[s];
-   T acc = 0;
-   for (int k = 0; k < wd; k++) {
-      long p = base + (long)k * 64;
-      acc += aval[p] * e[A.col[p]];
-   }
+   const T acc = sell_row_dot<T, T>(A.col, aval, e, base, A.width[s], (T)0);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
    const double u = (double)rv;
@@ -559,6 +543,17 @@ __global__ void bfinish_kernel(long N, BLayout L, const double* phi, const T* e,
    const int b = slot_of_row(L, row);
    if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b]) return;
    x[row] = phi[row] + (double)e[row];
+}
+
+// y = A x over the units of live slots (timing of the pool's operator)
+template <class T>
+__global__ void __launch_bounds__(kRows) bspmv_kernel(BLayout L, BSell A, const T* aval, const T* x, T* y) {
+   const int b = slot_of_wg(L);
+   if (b < 0) return;
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const long s = row >> 6;
+   const long base = A.off[s] + (threadIdx.x & 63);
+   y[row] = sell_row_dot<T, T>(A.col, aval, x, base, A.width[s], (T)0);
 }
 
 __global__ void bphase_kernel(int nslots, BState S) {
@@ -1158,6 +1153,30 @@ static int fixed_solve(pk_batch_s* P) {
    return 0;
 }
 
+template <class T>
+static int time_pool_spmv(pk_batch_s* P, int reps, double* avg_seconds) {
+   Ctx& c = ctx();
+   Inner<T>& I = inner_of<T>(P);
+   const T* aval = sizeof(T) == sizeof(double) ? (const T*)P->a_val.p : I.aval.p;
+   const long units = P->ua.high_water();
+   hipEvent_t e0, e1;
+   PK_HIP(hipEventCreate(&e0));
+   PK_HIP(hipEventCreate(&e1));
+   // one untimed launch, then `reps` between the events
+   bspmv_kernel<T><<<(int)units, kRows, 0, c.stream>>>(P->layout(), P->rsell(), aval, I.z.p, I.w.p);
+   PK_HIP(hipEventRecord(e0, c.stream));
+   for (int r = 0; r < reps; r++)
+      bspmv_kernel<T><<<(int)units, kRows, 0, c.stream>>>(P->layout(), P->rsell(), aval, I.z.p, I.w.p);
+   PK_HIP(hipEventRecord(e1, c.stream));
+   PK_HIP(hipEventSynchronize(e1));
+   float ms = 0.f;
+   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   *avg_seconds = ms * 1e-3 / reps;
+   return 0;
+}
+
 extern "C" {
 
 int pk_batch_create(int num, const pk_mat* R, const pk_mat* F, pk_batch* out) {
@@ -1397,6 +1416,23 @@ int pk_pool_result(pk_batch P, int slot, double* keff, pk_vec x, int* outer, lon
 int pk_pool_release(pk_batch P, int slot) {
    PK_REQUIRE(P, "null pool");
    return pool_release(P, slot);
+}
+
+int pk_pool_time_spmv(pk_batch P, int reps, double* avg_seconds, double* bytes) {
+   PK_REQUIRE(P && P->sd.p && reps > 0 && avg_seconds && bytes, "wrong pool arguments");
+   // every live slot's shifted operator must exist: run after a tick
+   double n = 0.0, nnz = 0.0;
+   for (auto& sl : P->slots)
+      if (sl.used) {
+         n += sl.n;
+         nnz += (double)sl.nnz;
+      }
+   PK_REQUIRE(n > 0.0, "empty pool");
+   const double s = P->precision == PK_PREC_MIXED ? 4.0 : 8.0;
+   // algorithmic bytes of the CSR SpMV of every live problem
+   *bytes = nnz * (s + 4.0) + (n + 1.0) * 4.0 + 2.0 * n * s;
+   if (P->precision == PK_PREC_MIXED) return time_pool_spmv<float>(P, reps, avg_seconds);
+   return time_pool_spmv<double>(P, reps, avg_seconds);
 }
 
 int pk_pool_capacity(pk_batch P, long* free_rows, long* free_entries, int* free_slots, long* ticks) {

@@ -16,6 +16,7 @@
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
+#include "pk_sell.hpp"
 
 namespace pk {
 
@@ -131,14 +132,7 @@ __global__ void __launch_bounds__(256) sell_spmv_kernel(long n, const long* __re
    long s = row >> 6;
    int r = threadIdx.x & 63;
    if (row >= n) return;
-   long base = off[s] + r;
-   int w = width[s];
-   T acc = 0;
-   for (int k = 0; k < w; k++) {
-      long p = base + (long)k * kSlice;
-      acc += val[p] * x[col[p]];
-   }
-   y[row] = acc;
+   y[row] = sell_row_dot<T, T>(col, val, x, off[s] + r, width[s], (T)0);
 }
 
 
