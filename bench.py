@@ -72,6 +72,29 @@ def run_solves(pa, d, precision, steps, warmup):
         os.chdir(cwd)
 
 
+def roofline_pass(pa, d, solves=2):
+    """Profiled cold solves only (mixed precision): per-kernel launches, device
+    time from HIP events on the library stream, algorithmic bytes."""
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        with pa.DropIn("input.pmp", "-silent", "-pk_precision", "mixed") as run:
+            C, G, _ = run.sizes()
+            pa.profile(True)
+            for _ in range(solves):
+                run.set_field("scalar-flux", np.zeros(C * G))
+                run.solve()
+            out = {}
+            for name in ("gmres_prec", "gmres_spmv_dot", "gmres_orth"):
+                n, sec, byt = pa.profile_get(name)
+                out[name] = {"launches": n, "avg_us": sec / max(n, 1) * 1e6,
+                             "bytes_per_launch": byt / max(n, 1), "GBs": byt / sec / 1e9 if sec else 0.0}
+            pa.profile(False)
+        return {"roofline_pass": out, "solves": solves}
+    finally:
+        os.chdir(cwd)
+
+
 def cpu_baseline(d, target_seconds=15.0):
     """The oracle (FP64 CPU restatement: reference-form assembly, exact sparse LU
     + restarted Arnoldi shift-invert, power normalisation) timed on a bounded
@@ -168,6 +191,24 @@ def _family(W, tmp, n):
     return paths, np.array(rows)
 
 
+def batched_spmv(pa, paths, reps=20):
+    """One SpMV over the inner operators of every instance at once (all
+    resident in one pool), FP32 and FP64: HIP events around `reps`
+    back-to-back launches on the library stream."""
+    spmv = {}
+    for name, prec in (("fp32", pa.PK_PREC_MIXED), ("fp64", pa.PK_PREC_FP64)):
+        srv = pa.Server(budget_bytes=64e9, max_slots=len(paths), precision=prec)
+        for p in paths:
+            srv.add_case(p)
+        sec, byt, nrows = srv.time_spmv(reps)
+        srv.close()
+        spmv[name] = {"kernel": f"bspmv_kernel<{'float' if name == 'fp32' else 'double'}>", "us": sec * 1e6,
+                      "bytes": byt, "GBs": byt / sec / 1e9, "frac_hbm": byt / sec / 1e9 / HBM_PEAK_GBS,
+                      "rows": nrows}
+        log(f"[bench]   batched SpMV {name}: {byt / sec / 1e9:.0f} GB/s")
+    return spmv
+
+
 def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
     """Config-3 family: n_inst independent instances (sides and group counts
     drawn by seed, cross sections perturbed +-5% per instance), all admitted at
@@ -189,16 +230,7 @@ def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
     pa.profile(False)
     log("[bench] batched: timed pass")
     res, st = _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), cid, seeds, 0.0, False, keep=small)
-    spmv = {}
-    for name, prec in (("fp32", pa.PK_PREC_MIXED), ("fp64", pa.PK_PREC_FP64)):
-        srv = pa.Server(budget_bytes=64e9, max_slots=n, precision=prec)
-        for p in paths:
-            srv.add_case(p)
-        sec, byt, nrows = srv.time_spmv(20)
-        srv.close()
-        spmv[name] = {"us": sec * 1e6, "bytes": byt, "GBs": byt / sec / 1e9,
-                      "frac_hbm": byt / sec / 1e9 / HBM_PEAK_GBS, "rows": nrows}
-        log(f"[bench]   batched SpMV {name}: {byt / sec / 1e9:.0f} GB/s")
+    spmv = batched_spmv(pa, paths)
     mixed_all = n / res["latency"].max()
     acc = _accuracy(oracle, W, res, paths, cid, seeds, 0.0, small)
     k = fp64_sample
@@ -264,6 +296,11 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-family numbers")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving numbers")
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched run")
+    ap.add_argument("--spmv-batched-only", action="store_true",
+                    help="only the SpMV over every instance of the batched family (FP32 and FP64)")
+    ap.add_argument("--roofline-pass", action="store_true",
+                    help="only the profiled solves of the headline workload (for a kernel trace whose "
+                         "launches are exactly the ones the roofline figure averages)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -276,6 +313,13 @@ def main():
     tmp = tempfile.mkdtemp(prefix="pampa_bench_")
     d1 = W.quarter_core_2d(os.path.join(tmp, "quarter-core-170"))
     log(f"[bench] workload quarter core 170x170 G=2 at {d1}")
+    if args.roofline_pass:
+        print(json.dumps(roofline_pass(pa, d1)))
+        return
+    if args.spmv_batched_only:
+        paths, rows = _family(W, os.path.join(tmp, "family"), args.batch)
+        print(json.dumps(batched_spmv(pa, paths)))
+        return
 
     results = {}
     for prec in ("mixed", "fp64"):

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 
 #include "pk_internal.hpp"
@@ -352,7 +353,7 @@ __device__ __forceinline__ bool gmres_on(const BState& S, int b) {
 
 // z = B^-1 (V[j] * scale), V[j] *= scale (normalises the new basis vector)
 template <class T>
-__global__ void __launch_bounds__(kRows) bprec_kernel(BLayout L, T* Vj, const T* binv, T* z, BState S) {
+__device__ __forceinline__ void prec_step(const BLayout& L, T* Vj, const T* binv, T* z, const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
@@ -374,11 +375,16 @@ __global__ void __launch_bounds__(kRows) bprec_kernel(BLayout L, T* Vj, const T*
    if (real) Vj[row] = own;
 }
 
+template <class T>
+__global__ void __launch_bounds__(kRows) bprec_kernel(BLayout L, T* Vj, const T* binv, T* z, BState S) {
+   prec_step<T>(L, Vj, binv, z, S);
+}
+
 // w = A z and h(i,j) = <V[i], w>, i <= j
 template <class T>
-__global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSell A, const T* aval,
-                                                          const T* z, T* w, const T* V, int j,
-                                                          double* partials, unsigned* tickets, BState S) {
+__device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
+                                              T* w, const T* V, int j, double* partials, unsigned* tickets,
+                                              const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
@@ -394,10 +400,61 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
        L.wg_count[b]);
 }
 
+template <class T>
+__global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSell A, const T* aval,
+                                                          const T* z, T* w, const T* V, int j,
+                                                          double* partials, unsigned* tickets, BState S) {
+   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, S);
+}
+
+// The GMRES steps with their execution span recorded for timing: every
+// workgroup folds its first and last device wall-clock reading into clk[0]
+// (earliest start) and clk[1] (latest end).  Same arithmetic as the untimed
+// kernels.
+__device__ __forceinline__ void record_span(unsigned long long t0, unsigned long long* clk) {
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      atomicMin(&clk[0], t0);
+      atomicMax(&clk[1], (unsigned long long)wall_clock64());
+   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRows) bprec_timed_kernel(BLayout L, T* Vj, const T* binv, T* z, BState S,
+                                                            unsigned long long* clk) {
+   const unsigned long long t0 = wall_clock64();
+   prec_step<T>(L, Vj, binv, z, S);
+   record_span(t0, clk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRows) bspmv_dot_timed_kernel(long N, BLayout L, BSell A, const T* aval,
+                                                                const T* z, T* w, const T* V, int j,
+                                                                double* partials, unsigned* tickets, BState S,
+                                                                unsigned long long* clk) {
+   const unsigned long long t0 = wall_clock64();
+   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, S);
+   record_span(t0, clk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, const T* w, T* V, int j,
+                                                            double* partials, unsigned* tickets, BState S,
+                                                            unsigned long long* clk) {
+   const unsigned long long t0 = wall_clock64();
+   orth_step<T>(N, L, w, V, j, partials, tickets, S);
+   record_span(t0, clk);
+}
+
+__global__ void clk_reset_kernel(unsigned long long* clk, int n) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) clk[i] = (i & 1) ? 0ull : ~0ull;
+}
+
 // V[j+1] = w - sum_i h(i,j) V[i], h(j+1,j) = |V[j+1]|, Givens update, stop test
 template <class T>
-__global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, int j,
-                                                      double* partials, unsigned* tickets, BState S) {
+__device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, T* V, int j, double* partials,
+                                          unsigned* tickets, const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
    const int m = S.m;
@@ -448,6 +505,12 @@ __global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T
    if (breakdown) fl[2] = 1;
    if (res <= S.resid[2 * b + 1] || breakdown || j + 1 == m) fl[0] = 1;
    S.scale[b] = hn > 0.0 ? 1.0 / hn : 0.0;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, int j,
+                                                      double* partials, unsigned* tickets, BState S) {
+   orth_step<T>(N, L, w, V, j, partials, tickets, S);
 }
 
 // y = H^-1 g, one workgroup per slot that took steps this cycle
@@ -677,6 +740,8 @@ struct pk_batch_s {
    std::vector<int> h_phase;
    long admitted = 0;
    Graph graph;
+   DBuf<unsigned long long> clk;  // execution spans of a profiled step's kernels
+   double clock_hz = 0.0;           // device wall clock
    double tick_seconds = 0.0;
    long ticks = 0;
    // fixed-batch use (pk_batch_create): members and optional initial vectors
@@ -1022,15 +1087,21 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          // an SpMV (value and column of every stored entry, z gathered once, w
          // written) plus j+1 basis vectors; orth reads w and j+1 basis vectors and
          // writes V[j+1]
-         prof_begin("gmres_prec", n * s * 3.0 + bsn * s);
-         bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S);
-         prof_end();
-         prof_begin("gmres_spmv_dot", nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s);
-         bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, S);
-         prof_end();
-         prof_begin("gmres_orth", n * s * (j + 3.0));
-         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, j, part, tk, S);
-         prof_end();
+         // the step's kernels record their execution span on the device clock
+         // (prec, spmv_dot, orth: clk[0..5])
+         clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
+         bprec_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S,
+                                                             P->clk.p);
+         bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk,
+                                                                S, P->clk.p + 2);
+         borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, j, part, tk, S, P->clk.p + 4);
+         unsigned long long h[6];
+         PK_HIP(hipMemcpyAsync(h, P->clk.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+         PK_HIP(hipStreamSynchronize(c.stream));
+         prof_add("gmres_prec", 1, (double)(h[1] - h[0]) / P->clock_hz, n * s * 3.0 + bsn * s);
+         prof_add("gmres_spmv_dot", 1, (double)(h[3] - h[2]) / P->clock_hz,
+                  nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s);
+         prof_add("gmres_orth", 1, (double)(h[5] - h[4]) / P->clock_hz, n * s * (j + 3.0));
       } else {
          bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S);
          bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, S);
@@ -1046,10 +1117,30 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    return 0;
 }
 
+// PK_NO_GRAPH=1 in the environment launches the tick's kernels one by one
+// instead of through the captured graph (same kernels and arguments), for
+// tools that trace individual launches.
+static bool graphs_disabled() {
+   static const bool off = [] {
+      const char* e = getenv("PK_NO_GRAPH");
+      return e && *e && *e != '0';
+   }();
+   return off;
+}
+
 template <class T>
 static int run_tick(pk_batch_s* P, long units) {
    Ctx& c = ctx();
-   if (profiling()) return launch_tick<T>(P, units, true);
+   if (profiling()) {
+      if (!P->clk.p) {
+         int khz = 0;
+         PK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device));
+         PK_REQUIRE(khz > 0 && !P->clk.alloc(6), "no device clock for profiling");
+         P->clock_hz = khz * 1e3;
+      }
+      return launch_tick<T>(P, units, true);
+   }
+   if (graphs_disabled()) return launch_tick<T>(P, units, false);
    if (P->graph.exec == nullptr || P->graph.units != units) {
       // every quantity that varies between ticks is read from device memory,
       // so one graph serves every tick for this grid size

@@ -1,6 +1,7 @@
 // Internal declarations of the pk_* numerics library (HIP, gfx950).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -58,12 +59,27 @@ struct Ctx {
 Ctx& ctx();
 int ensure_init();
 
-// Kernel profiler: with profiling on, instrumented launches are bracketed by
-// a pair of HIP events on the library stream; durations are resolved lazily.
-// `bytes` is the launch's algorithmic traffic.
+// Kernel profiler: with profiling on, instrumented launches carry a pair of
+// HIP events on the library stream; durations are resolved lazily.  `bytes`
+// is the launch's algorithmic traffic.
 bool profiling();
-void prof_begin(const char* name, double bytes);
-void prof_end();
+// With profiling on: a pending record (name, algorithmic bytes) whose start and
+// stop events the launch itself records (hipExtLaunchKernelGGL: the events
+// take the kernel's own start and end, not those of separate markers).
+bool prof_events(const char* name, double bytes, hipEvent_t* e0, hipEvent_t* e1);
+// Add measured launches directly (profiling on only).
+void prof_add(const char* name, long launches, double seconds, double bytes);
+
+// kernel<<<grid, block, 0, stream>>>(args...), timed when profiling is on
+#define PK_LAUNCH(name, bytes, kernel, grid, block, stream, ...)                                        \
+   do {                                                                                                 \
+      hipEvent_t e0_ = nullptr, e1_ = nullptr;                                                          \
+      if (::pk::prof_events((name), (bytes), &e0_, &e1_))                                               \
+         hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, (stream), e0_, e1_, 0, __VA_ARGS__); \
+      else                                                                                              \
+         kernel<<<(grid), (block), 0, (stream)>>>(__VA_ARGS__);                                         \
+   } while (0)
+
 // Grow the reduction scratch to at least `count` doubles (setup time only).
 int reserve_partials(size_t count);
 

@@ -144,10 +144,8 @@ static double spmv_bytes(const pk_mat_s* A, double s) {
 int spmv(pk_mat_s* A, const double* x, double* y) {
    if (A->n == 0) return 0;
    SellData& S = A->sell;
-   prof_begin("spmv", spmv_bytes(A, 8.0));
-   sell_spmv_kernel<double><<<grid_for(A->n, 256), 256, 0, ctx().stream>>>(
-       A->n, S.off.p, S.width.p, S.col.p, S.val.p, x, y);
-   prof_end();
+   PK_LAUNCH("spmv", spmv_bytes(A, 8.0), sell_spmv_kernel<double>, grid_for(A->n, 256), 256, ctx().stream,
+             A->n, S.off.p, S.width.p, S.col.p, S.val.p, x, y);
    PK_HIP(hipGetLastError());
    return 0;
 }
@@ -156,10 +154,8 @@ int spmv_f(pk_mat_s* A, const float* x, float* y) {
    if (A->n == 0) return 0;
    PK_CALL(A->ensure_fp32());
    SellData& S = A->sell;
-   prof_begin("spmv_fp32", spmv_bytes(A, 4.0));
-   sell_spmv_kernel<float><<<grid_for(A->n, 256), 256, 0, ctx().stream>>>(
-       A->n, S.off.p, S.width.p, S.col.p, S.val32.p, x, y);
-   prof_end();
+   PK_LAUNCH("spmv_fp32", spmv_bytes(A, 4.0), sell_spmv_kernel<float>, grid_for(A->n, 256), 256, ctx().stream,
+             A->n, S.off.p, S.width.p, S.col.p, S.val32.p, x, y);
    PK_HIP(hipGetLastError());
    return 0;
 }

@@ -38,16 +38,27 @@ static hipEvent_t take_event() {
 
 bool profiling() { return g_prof; }
 
-void prof_begin(const char* name, double bytes) {
-   if (!g_prof) return;
+bool prof_events(const char* name, double bytes, hipEvent_t* e0, hipEvent_t* e1) {
+   if (!g_prof) return false;
    ProfRec r{name, bytes, take_event(), take_event()};
-   (void)hipEventRecord(r.e0, g_ctx.stream);
    g_pending.push_back(r);
+   *e0 = r.e0;
+   *e1 = r.e1;
+   return true;
 }
 
-void prof_end() {
-   if (!g_prof || g_pending.empty()) return;
-   (void)hipEventRecord(g_pending.back().e1, g_ctx.stream);
+void prof_add(const char* name, long launches, double seconds, double bytes) {
+   if (!g_prof) return;
+   ProfAcc* acc = nullptr;
+   for (auto& p : g_prof_acc)
+      if (p.first == name) acc = &p.second;
+   if (!acc) {
+      g_prof_acc.push_back({name, ProfAcc()});
+      acc = &g_prof_acc.back().second;
+   }
+   acc->launches += launches;
+   acc->seconds += seconds;
+   acc->bytes += bytes;
 }
 
 static void prof_resolve() {

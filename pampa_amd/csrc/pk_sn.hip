@@ -200,10 +200,8 @@ static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
    const double per_item = profiling() ? sweep_bytes(S, sizeof(T)) / (double)S->step_ptr[S->nsteps] : 0.0;
    for (int s = 0; s < S->nsteps; s++) {
       long a = S->step_ptr[s], n = S->step_ptr[s + 1] - a;
-      prof_begin("sn_sweep_step", per_item * n);
-      sn_sweep_step_kernel<T><<<grid_for(n, 256), 256, 0, c.stream>>>(S->items.p + a, n, S->C, S->G, g,
-                                                                     sigt, q, psi);
-      prof_end();
+      PK_LAUNCH("sn_sweep_step", per_item * n, sn_sweep_step_kernel<T>, grid_for(n, 256), 256, c.stream,
+                S->items.p + a, n, S->C, S->G, g, sigt, q, psi);
    }
    PK_HIP(hipGetLastError());
    return 0;

@@ -405,18 +405,14 @@ struct Gmres {
             // algorithmic bytes: prec reads V[j], the block inverses and writes z, V[j];
             // spmv_dot is a CSR SpMV plus the j+1 basis vectors; orth reads w and
             // j+1 basis vectors and writes V[j+1]
-            prof_begin("gmres_prec", (double)n * s * (3.0 + bs));
-            prec_kernel<T><<<gb, kBlock, 0, c.stream>>>(n, bs, V.p + (size_t)j * n, binv.p, z.p, S);
-            prof_end();
-            prof_begin("gmres_spmv_dot", (double)A->nnz * (s + 4.0) + (double)(n + 1) * 4.0 +
-                                             2.0 * n * s + (double)(j + 1) * n * s);
-            spmv_dot_kernel<T><<<gb, kBlock, 0, c.stream>>>(n, A->off.p, A->width.p, A->col.p, Aval,
-                                                            z.p, w.p, V.p, j, m, c.partials,
-                                                            c.ticket, S);
-            prof_end();
-            prof_begin("gmres_orth", (double)n * s * (j + 3.0));
-            orth_kernel<T><<<gb, kBlock, 0, c.stream>>>(n, w.p, V.p, j, m, c.partials, c.ticket, ww, S);
-            prof_end();
+            PK_LAUNCH("lin_prec", (double)n * s * (3.0 + bs), prec_kernel<T>, gb, kBlock, c.stream, n, bs,
+                      V.p + (size_t)j * n, binv.p, z.p, S);
+            PK_LAUNCH("lin_spmv_dot",
+                      (double)A->nnz * (s + 4.0) + (double)(n + 1) * 4.0 + 2.0 * n * s + (double)(j + 1) * n * s,
+                      spmv_dot_kernel<T>, gb, kBlock, c.stream, n, A->off.p, A->width.p, A->col.p, Aval, z.p,
+                      w.p, V.p, j, m, c.partials, c.ticket, S);
+            PK_LAUNCH("lin_orth", (double)n * s * (j + 3.0), orth_kernel<T>, gb, kBlock, c.stream, n, w.p, V.p,
+                      j, m, c.partials, c.ticket, ww, S);
             if (prof) {
                // profiling: issue only steps that do work
                int f0 = 0;
