@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
@@ -72,6 +73,89 @@ __global__ void __launch_bounds__(256) sn_sweep_step_kernel(const int* __restric
    const int mt = geo.mat[i];
    const T V = (T)geo.vol[i];
    T* out = psi + ((long)m * C + i) * G;
+   const T* qi = q + (long)i * G;
+   for (int g = 0; g < G; g++) out[g] = (qi[g] - inflow[g]) / (sigt[mt * G + g] * V + lhs0);
+}
+
+// Per-item records: what a sweep step needs of (direction m, cell i), taken
+// from the geometry once per precision in the step kernel's own arithmetic
+// (same operations in the same order, so the same bits): the flat index m C + i,
+// the outgoing part of the left-hand side lhs0 = sum_out coef w A + sum_vac,out w A,
+// and up to kMaxIn incoming interior faces as (flat upwind index, coef w A) in
+// face order.  Stored structure-of-arrays in step order, so a step reads its
+// records with coalesced loads and has one dependent gather (the upwind values)
+// left, instead of the chain cell -> faces -> normals -> neighbours -> values.
+constexpr int kMaxIn = 4;
+
+template <class T>
+struct SnRec {
+   long n;
+   int* code;
+   int* nin;
+   T* lhs;
+   int* up;  // [kMaxIn][n]
+   T* cw;    // [kMaxIn][n]
+};
+
+template <class T>
+__global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, SnGeom geo, SnRec<T> R, int* overflow) {
+#pragma clang fp contract(off)
+   long it = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (it >= n) return;
+   const int code = items[it];
+   const int m = code / C, i = code - m * C;
+   const double o0 = geo.dirs[3 * m], o1 = geo.dirs[3 * m + 1], o2 = geo.dirs[3 * m + 2];
+   T lhs0 = 0;
+   int k = 0;
+   for (int f = geo.fptr[i]; f < geo.fptr[i + 1]; f++) {
+      const double* nn = geo.fnorm + 3L * f;
+      T w = (T)(o0 * nn[0] + o1 * nn[1] + o2 * nn[2]);
+      const int i2 = geo.fnb[f];
+      const T A = (T)geo.farea[f];
+      if (i2 < 0) {
+         if (w > (T)0) lhs0 += w * A;
+      } else if (w > (T)0) {
+         lhs0 += (T)geo.coef[f] * w * A;
+      } else {
+         if (k < kMaxIn) {
+            R.up[(long)k * n + it] = m * C + i2;
+            R.cw[(long)k * n + it] = (T)geo.coef[f] * w * A;
+         } else {
+            *overflow = 1;
+         }
+         k++;
+      }
+   }
+   R.code[it] = code;
+   R.nin[it] = k < kMaxIn ? k : kMaxIn;
+   R.lhs[it] = lhs0;
+}
+
+// one step from the records: items a .. a+count-1
+template <class T>
+__global__ void __launch_bounds__(256) sn_sweep_rec_kernel(long a, long count, int C, int G, SnRec<T> R,
+                                                           const int* __restrict__ mat,
+                                                           const double* __restrict__ vol,
+                                                           const T* __restrict__ sigt, const T* __restrict__ q,
+                                                           T* psi) {
+#pragma clang fp contract(off)
+   const long t = blockIdx.x * 256L + threadIdx.x;
+   if (t >= count) return;
+   const long it = a + t;
+   const int code = R.code[it];
+   const int nin = R.nin[it];
+   const T lhs0 = R.lhs[it];
+   const int i = code % C;
+   T inflow[kMaxSnGroups];
+   for (int g = 0; g < G; g++) inflow[g] = 0;
+   for (int k = 0; k < nin; k++) {
+      const T c = R.cw[(long)k * R.n + it];
+      const T* up = psi + (long)R.up[(long)k * R.n + it] * G;
+      for (int g = 0; g < G; g++) inflow[g] += c * up[g];
+   }
+   const int mt = mat[i];
+   const T V = (T)vol[i];
+   T* out = psi + (long)code * G;
    const T* qi = q + (long)i * G;
    for (int g = 0; g < G; g++) out[g] = (qi[g] - inflow[g]) / (sigt[mt * G + g] * V + lhs0);
 }
@@ -171,6 +255,18 @@ struct pk_sn_s {
    std::vector<long> step_ptr;    // nsteps + 1
    int nsteps = 0;
    DBuf<int> items;
+   // per-item records (one set per precision, built on first use) and the
+   // captured step sequences; geometry = true sweeps from the geometry instead
+   bool geometry = false;
+   DBuf<int> rec_code, rec_nin, rec_up, rec_err;
+   DBuf<double> rec_lhs64, rec_cw64;
+   DBuf<float> rec_lhs32, rec_cw32;
+   hipGraphExec_t graph32 = nullptr, graph64 = nullptr;
+   const void* graph_ptrs[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+   ~pk_sn_s() {
+      if (graph32) (void)hipGraphExecDestroy(graph32);
+      if (graph64) (void)hipGraphExecDestroy(graph64);
+   }
    DBuf<double> dirs, wts, coef;
    DBuf<double> psi64, q64, sigt64;
    DBuf<float> psi32, q32, sigt32;
@@ -194,16 +290,99 @@ static SnGeom geom(pk_sn_s* S) {
 static double sweep_bytes(const pk_sn_s* S, double s);
 
 template <class T>
+static SnRec<T> recs(pk_sn_s* S);
+template <>
+SnRec<double> recs<double>(pk_sn_s* S) {
+   const long n = S->step_ptr[S->nsteps];
+   return SnRec<double>{n, S->rec_code.p, S->rec_nin.p, S->rec_lhs64.p, S->rec_up.p, S->rec_cw64.p};
+}
+template <>
+SnRec<float> recs<float>(pk_sn_s* S) {
+   const long n = S->step_ptr[S->nsteps];
+   return SnRec<float>{n, S->rec_code.p, S->rec_nin.p, S->rec_lhs32.p, S->rec_up.p, S->rec_cw32.p};
+}
+
+// records of precision T, built once
+template <class T>
+static int sn_records(pk_sn_s* S) {
+   Ctx& c = ctx();
+   const long n = S->step_ptr[S->nsteps];
+   const bool f32 = sizeof(T) == sizeof(float);
+   if (f32 ? S->rec_lhs32.p != nullptr : S->rec_lhs64.p != nullptr) return 0;
+   if (!S->rec_code.p)
+      PK_REQUIRE(!S->rec_code.alloc(n) && !S->rec_nin.alloc(n) && !S->rec_up.alloc((size_t)kMaxIn * n) &&
+                     !S->rec_err.alloc(1),
+                 "device allocation failed");
+   if (f32)
+      PK_REQUIRE(!S->rec_lhs32.alloc(n) && !S->rec_cw32.alloc((size_t)kMaxIn * n), "device allocation failed");
+   else
+      PK_REQUIRE(!S->rec_lhs64.alloc(n) && !S->rec_cw64.alloc((size_t)kMaxIn * n), "device allocation failed");
+   PK_HIP(hipMemsetAsync(S->rec_err.p, 0, sizeof(int), c.stream));
+   sn_rec_kernel<T><<<grid_for(n), kBlock, 0, c.stream>>>(S->items.p, n, S->C, geom(S), recs<T>(S), S->rec_err.p);
+   PK_HIP(hipGetLastError());
+   int h = 0;
+   PK_HIP(hipMemcpyAsync(&h, S->rec_err.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   PK_REQUIRE(h == 0, "more incoming faces per cell than the sweep records hold");
+   return 0;
+}
+
+template <class T>
+static void sn_launch_steps(pk_sn_s* S, const T* sigt, const T* q, T* psi, bool prof) {
+   Ctx& c = ctx();
+   const SnRec<T> R = recs<T>(S);
+   const double per_item = prof ? sweep_bytes(S, sizeof(T)) / (double)S->step_ptr[S->nsteps] : 0.0;
+   for (int s = 0; s < S->nsteps; s++) {
+      const long a = S->step_ptr[s], n = S->step_ptr[s + 1] - a;
+      PK_LAUNCH("sn_sweep_step", per_item * n, sn_sweep_rec_kernel<T>, grid_for(n, 256), 256, c.stream, a, n,
+                S->C, S->G, R, S->mesh->mat.p, S->mesh->vol.p, sigt, q, psi);
+   }
+}
+
+// One sweep psi = L^-1 q.  The step launches are captured once per precision
+// as a HIP graph (the sweep's buffers are fixed per handle and precision) and
+// relaunched; with the profiler on they are launched one by one.
+template <class T>
 static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
    Ctx& c = ctx();
-   SnGeom g = geom(S);
-   const double per_item = profiling() ? sweep_bytes(S, sizeof(T)) / (double)S->step_ptr[S->nsteps] : 0.0;
-   for (int s = 0; s < S->nsteps; s++) {
-      long a = S->step_ptr[s], n = S->step_ptr[s + 1] - a;
-      PK_LAUNCH("sn_sweep_step", per_item * n, sn_sweep_step_kernel<T>, grid_for(n, 256), 256, c.stream,
-                S->items.p + a, n, S->C, S->G, g, sigt, q, psi);
+   if (S->geometry) {
+      SnGeom g = geom(S);
+      const double per_item = profiling() ? sweep_bytes(S, sizeof(T)) / (double)S->step_ptr[S->nsteps] : 0.0;
+      for (int s = 0; s < S->nsteps; s++) {
+         long a = S->step_ptr[s], n = S->step_ptr[s + 1] - a;
+         PK_LAUNCH("sn_sweep_step", per_item * n, sn_sweep_step_kernel<T>, grid_for(n, 256), 256, c.stream,
+                   S->items.p + a, n, S->C, S->G, g, sigt, q, psi);
+      }
+      PK_HIP(hipGetLastError());
+      return 0;
    }
-   PK_HIP(hipGetLastError());
+   PK_CALL(sn_records<T>(S));
+   if (profiling()) {
+      sn_launch_steps<T>(S, sigt, q, psi, true);
+      PK_HIP(hipGetLastError());
+      return 0;
+   }
+   const int pi = sizeof(T) == sizeof(float) ? 0 : 1;
+   hipGraphExec_t& G = pi == 0 ? S->graph32 : S->graph64;
+   const void** gp = S->graph_ptrs[pi];
+   if (G && (gp[0] != sigt || gp[1] != q || gp[2] != psi)) {
+      (void)hipGraphExecDestroy(G);
+      G = nullptr;
+   }
+   if (!G) {
+      gp[0] = sigt;
+      gp[1] = q;
+      gp[2] = psi;
+      hipGraph_t g = nullptr;
+      PK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      sn_launch_steps<T>(S, sigt, q, psi, false);
+      hipError_t e = hipStreamEndCapture(c.stream, &g);
+      PK_HIP(e);
+      hipError_t ie = hipGraphInstantiate(&G, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      PK_HIP(ie);
+   }
+   PK_HIP(hipGraphLaunch(G, c.stream));
    return 0;
 }
 
@@ -337,6 +516,8 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
          S->step_ptr[s + 1] = (long)items.size();
       }
    }
+   const char* ge = getenv("PK_SWEEP_GEOMETRY");
+   S->geometry = ge && *ge && *ge != '0';
    const int G = S->G;
    S->value_units = (double)((long)C * G * M + nin_total * G) + 2.0 * C * G;
    int e = 0;
@@ -348,6 +529,7 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
       PK_FAIL("device allocation failed");
    }
    PK_HIP(hipMemcpy(S->items.p, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
+
    PK_HIP(hipMemcpy(S->dirs.p, dirs, sizeof(double) * 3 * M, hipMemcpyHostToDevice));
    PK_HIP(hipMemcpy(S->wts.p, wts, sizeof(double) * M, hipMemcpyHostToDevice));
    if (mesh->F) PK_HIP(hipMemcpy(S->coef.p, coef.data(), sizeof(double) * mesh->F, hipMemcpyHostToDevice));
@@ -469,7 +651,7 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
       PK_HIP(hipGetLastError());
       PK_HIP(hipMemcpyAsync(c.host_scalars, st, sizeof(double) * 5, hipMemcpyDeviceToHost, c.stream));
       PK_HIP(hipStreamSynchronize(c.stream));
-      double prod_old = c.host_scalars[1], prod_new = c.host_scalars[2];
+         double prod_old = c.host_scalars[1], prod_new = c.host_scalars[2];
       PK_REQUIRE(prod_new > 0.0 && prod_old > 0.0, "zero fission source in the S_N eigen solve");
       double knew = k * prod_new / prod_old;
       last_scale = prod_old / prod_new;
