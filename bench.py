@@ -61,7 +61,7 @@ def run_solves(pa, d, precision, steps, warmup):
             run.set_field("scalar-flux", zero)
             run.solve()
             prof = {}
-            for name in ("gmres_prec", "gmres_spmv_dot", "gmres_orth", "spmv", "spmv_fp32"):
+            for name in ("gmres_spmv_dot", "gmres_orth", "spmv", "spmv_fp32"):
                 n, s, b = pa.profile_get(name)
                 if n:
                     prof[name] = dict(launches=n, seconds=s, bytes=b)
@@ -85,7 +85,7 @@ def roofline_pass(pa, d, solves=2):
                 run.set_field("scalar-flux", np.zeros(C * G))
                 run.solve()
             out = {}
-            for name in ("gmres_prec", "gmres_spmv_dot", "gmres_orth"):
+            for name in ("gmres_spmv_dot", "gmres_orth"):
                 n, sec, byt = pa.profile_get(name)
                 out[name] = {"launches": n, "avg_us": sec / max(n, 1) * 1e6,
                              "bytes_per_launch": byt / max(n, 1), "GBs": byt / sec / 1e9 if sec else 0.0}
@@ -223,7 +223,7 @@ def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
     pa.profile(True)
     _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), cid, seeds, 0.0, False)
     prof = {}
-    for name in ("gmres_prec", "gmres_spmv_dot", "gmres_orth"):
+    for name in ("gmres_spmv_dot", "gmres_orth"):
         cnt, sec, byt = pa.profile_get(name)
         if cnt:
             prof[name] = dict(launches=cnt, seconds=sec, bytes=byt)

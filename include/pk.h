@@ -130,6 +130,9 @@ int pk_normalize_flux(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double k
  * Wielandt-shifted power iteration with restarted GMRES and cell-block Jacobi
  * inner solves.  tol is the relative residual |R x - F x / k| / |F x / k|. */
 int pk_eig_create(pk_mat R, pk_mat F, pk_eig* E);
+/* new R and F of the same sizes (the next steady solve of the same problem):
+ * the solver keeps its device memory and captured graphs */
+int pk_eig_set_operators(pk_eig E, pk_mat R, pk_mat F);
 int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer);
 int pk_eig_set_precision(pk_eig E, int precision);
 int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner);
@@ -147,6 +150,7 @@ int pk_eig_destroy(pk_eig* E);
  *      Members may differ in size and block size; F_b must hold one cell block
  *      per row (the diffusion fission operator).  Options as for pk_eig. */
 int pk_batch_create(int num, const pk_mat* R, const pk_mat* F, pk_batch* B);
+int pk_batch_set_member(pk_batch B, int index, pk_mat R, pk_mat F); /* must fit the pool */
 int pk_batch_set_tolerances(pk_batch B, double tol, int max_outer);
 int pk_batch_set_precision(pk_batch B, int precision);
 int pk_batch_set_inner(pk_batch B, int restart, double reduction, int max_inner);

@@ -294,7 +294,7 @@ def diffusion_assemble(mesh: DeviceMesh, xs: DeviceXS, bc_type, bc_value):
 
 
 class Eigen(Handle):
-    def __init__(self, R: Mat, F: Mat, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=30,
+    def __init__(self, R: Mat, F: Mat, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=60,
                  reduction=0.1, max_inner=2000, max_outer=20000):
         L = lib()
         h = ct.c_void_p()
@@ -327,7 +327,7 @@ class Eigen(Handle):
 class Batch(Handle):
     """Batched eigen solver over pairs (R_b, F_b): one launch per step for all."""
 
-    def __init__(self, Rs, Fs, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=30,
+    def __init__(self, Rs, Fs, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=60,
                  reduction=0.1, max_inner=2000, max_outer=20000):
         L = lib()
         n = len(Rs)
@@ -370,7 +370,7 @@ class Pool(Handle):
     others iterate, step, collect finished slots, release them."""
 
     def __init__(self, max_rows, max_entries, max_fentries, max_binv, max_slots,
-                 precision=PK_PREC_FP64, restart=30):
+                 precision=PK_PREC_FP64, restart=60):
         h = ct.c_void_p()
         check(lib().pk_pool_create(max_rows, max_entries, max_fentries, max_binv, max_slots, precision,
                                    restart, ct.byref(h)))
@@ -379,7 +379,7 @@ class Pool(Handle):
 
     @staticmethod
     def bytes(max_rows, max_entries, max_fentries, max_binv, max_slots, precision=PK_PREC_FP64,
-              restart=30):
+              restart=60):
         return lib().pk_pool_bytes(max_rows, max_entries, max_fentries, max_binv, max_slots, precision,
                                    restart)
 

@@ -93,7 +93,7 @@ struct Case {
 
 struct Server {
    pk_batch pool = nullptr;
-   int precision = PK_PREC_FP64, restart = 30, max_slots = 0;
+   int precision = PK_PREC_FP64, restart = 60, max_slots = 0;
    double pool_bytes = 0.0;
    double tol = 1e-8, reduction = 0.1, shift = 0.1;
    std::vector<std::unique_ptr<Case>> cases;
@@ -135,7 +135,7 @@ void pampa_serve_create(double budget_bytes, int max_slots, int precision, pampa
    // capacity from the budget: per row of the row space, room for 24 stored
    // entries of R (and A), 16 of F and 16 block-inverse values (16-group 3-D
    // operators); rows scale down until the pool fits the budget
-   const int restart = 30;
+   const int restart = 60;
    long rows = 1L << 30;
    while (rows > 256 && pk_pool_bytes(rows, rows * 24, rows * 16, rows * 16, max_slots, precision, restart) >
                             budget_bytes)

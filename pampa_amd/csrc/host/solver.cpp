@@ -210,8 +210,11 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       PK_CHECK(pk_mat_destroy(&R));
       PK_CHECK(pk_mat_destroy(&F));
       PK_CHECK(pk_diffusion_assemble(dmesh, dxs, nb, bc_type.data(), bc_value.data(), &R, &F));
-      pk_eig eps = nullptr;
-      PK_CHECK(pk_eig_create(R, F, &eps));
+      // the reference recreates the EPS for every steady solve
+      // (NeutronicSolver.cxx:14-19); here the solver keeps its device memory and
+      // captured graphs across solves of the same problem and takes the new R, F
+      if (eps && pk_eig_set_operators(eps, R, F)) PK_CHECK(pk_eig_destroy(&eps));
+      if (!eps) PK_CHECK(pk_eig_create(R, F, &eps));
       PK_CHECK(pk_eig_set_tolerances(eps, options.eps_tol, options.eps_max_it));
       PK_CHECK(pk_eig_set_precision(eps, options.precision));
       PK_CHECK(pk_eig_set_inner(eps, options.gmres_restart, options.inner_reduction, 2000));
@@ -226,7 +229,6 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       double lambda = 0.0;
       PK_CHECK(pk_eig_get_eigenpair(eps, &lambda, phi));
       PK_CHECK(pk_eig_get_stats(eps, &stat_outer, &stat_inner, &stat_residual, &stat_seconds));
-      PK_CHECK(pk_eig_destroy(&eps));
       keff = 1.0 / lambda;
       PAMPA_CHECK(powerAndProduction(true), "unable to normalize the scalar flux");
    } else {
@@ -273,6 +275,7 @@ int DiffusionSolver::writeVTK(const std::string& path, int n) {
 }
 
 int DiffusionSolver::finalize() {
+   PK_CHECK(pk_eig_destroy(&eps));
    PK_CHECK(pk_mat_destroy(&R));
    PK_CHECK(pk_mat_destroy(&F));
    PK_CHECK(pk_vec_destroy(&b));

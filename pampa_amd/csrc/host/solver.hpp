@@ -15,7 +15,7 @@ struct SolverOptions {
    double eps_tol = 1e-8;     // eigen residual |R x - F x/k| / |F x/k|
    int eps_max_it = 20000;
    int precision = PK_PREC_FP64;
-   int gmres_restart = 30;
+   int gmres_restart = 60;
    double inner_reduction = 0.1;
    double wielandt_shift = 0.1;
    double ksp_rtol = 1e-5, ksp_atol = 1e-50;  // PETSc KSP defaults
@@ -131,6 +131,7 @@ class DiffusionSolver : public NeutronicSolver {
   private:
    pk_mat R = nullptr, F = nullptr;
    pk_vec b = nullptr;
+   pk_eig eps = nullptr;  // kept across steady solves of this problem
 };
 
 class SNSolver : public NeutronicSolver {

@@ -51,6 +51,7 @@ struct BLayout {
    const int* pbs;         // [S] preconditioner block
    const int* fbs;         // [S] cell block of F
    const long* binv_off;   // [S]
+   const int* mlen;        // [S] restart length of the slot (<= the pool's)
 };
 
 struct BSell {
@@ -68,7 +69,7 @@ struct BState {  // per-slot state (arrays over slots)
    double* sn;     // [S][m]
    double* g;      // [S][m+1]
    double* y;      // [S][m]
-   double* scale;  // [S]
+   double* vs;     // [S][m+1] 1/norm of each stored (unnormalised) basis vector
    double* resid;  // [S][2]: estimate, target
    double* ww;     // [S]
    int* flags;     // [S][4]: stop, steps in cycle, breakdown, steps in this solve
@@ -249,7 +250,7 @@ __global__ void bclear_rebuild_kernel(int nslots, BState S) {
 // y1 = R x, y2 = F x and <y2,y1>, <y2,y2>, <x,x> per slot
 __global__ void __launch_bounds__(kRows) brf_kernel(BLayout L, BSell R, const double* rval, BSell F,
                                                      const double* fval, const double* x, double* y1,
-                                                     double* y2, double* partials, unsigned* tickets,
+                                                     double* y2, double* partials, unsigned* tickets, unsigned* gtickets,
                                                      BState S) {
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kOuter) return;
@@ -263,14 +264,14 @@ __global__ void __launch_bounds__(kRows) brf_kernel(BLayout L, BSell R, const do
    const double xv = x[row];
    seg_reduce<3>(
        3, [&](int k) { return k == 0 ? c * a : (k == 1 ? c * c : xv * xv); },
-       partials + (long)L.wg_first[b] * 3, tickets + b, S.dots + 3 * b, (int)blockIdx.x - L.wg_first[b],
+       partials + (long)L.wg_first[b] * 3, tickets + b, gtickets + L.wg_first[b], S.dots + 3 * b, (int)blockIdx.x - L.wg_first[b],
        L.wg_count[b]);
 }
 
 // phi = x/|x|, r = (y2/k - y1)/|x|; residual; convergence, shift and phase decisions
 __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* x, const double* y1,
                                                         const double* y2, double* phi, double* r,
-                                                        BOpts o, double* partials, unsigned* tickets,
+                                                        BOpts o, double* partials, unsigned* tickets, unsigned* gtickets,
                                                         BState S) {
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kOuter) return;
@@ -284,7 +285,7 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
    r[row] = v;
    const double rr = v * v, qq = q * q;
    bool last = seg_reduce<2>(
-       2, [&](int k) { return k == 0 ? rr : qq; }, partials + (long)L.wg_first[b] * 2, tickets + b,
+       2, [&](int k) { return k == 0 ? rr : qq; }, partials + (long)L.wg_first[b] * 2, tickets + b, gtickets + L.wg_first[b],
        S.rq + 2 * b, (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    if (!last || threadIdx.x != 0) return;
    const double k = d[1] / d[0];
@@ -317,21 +318,49 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
 
 // ---- GMRES ------------------------------------------------------------------------------
 
-// V0 = rhs = r (in T), e = 0, beta = |V0|, target = reduction beta
+__device__ __forceinline__ bool gmres_on(const BState& S, int b) {
+   return b >= 0 && S.phase[b] == kInner && !S.flags[4 * b];
+}
+
+// z = B^-1 t for this workgroup's rows of slot b: t of the whole workgroup is
+// staged in LDS (a cell block never straddles a workgroup); every thread calls
+// it (it holds a barrier), padding rows get 0
 template <class T>
-__global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* r, T* rhs, T* e, T* V0,
-                                                        BOpts o, double* partials, unsigned* tickets,
-                                                        BState S) {
+__device__ __forceinline__ T cell_block_apply(const BLayout& L, int b, const T* binv, T t, T* tl) {
+   tl[threadIdx.x] = t;
+   __syncthreads();
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const long local = row - L.row_off[b];
+   T acc = 0;
+   if (local < L.nrows[b]) {
+      const int bs = L.pbs[b];
+      const long cell = local / bs;
+      const int gr = (int)(local - cell * bs);
+      const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
+      const int first = (int)(threadIdx.x - gr);
+      for (int q = 0; q < bs; q++) acc += bi[q] * tl[first + q];
+   }
+   return acc;
+}
+
+// V0 = rhs = r (in T), z = B^-1 V0, e = 0, beta = |V0|, target = reduction beta.
+// The basis is stored unnormalised: vs[i] = 1/|V_i| scales it where it is used.
+template <class T>
+__global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* r, T* rhs, T* e, T* V0, T* z,
+                                                        const T* binv, BOpts o, double* partials,
+                                                        unsigned* tickets, unsigned* gtickets, BState S) {
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kInner || !S.start[b]) return;
+   __shared__ T tl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const T v = (T)r[row];
    rhs[row] = v;
    e[row] = 0;
    V0[row] = v;
+   z[row] = cell_block_apply<T>(L, b, binv, v, tl);
    const double u = (double)v;
    bool last = seg_reduce<1>(
-       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, S.ww + b,
+       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
        (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    if (!last || threadIdx.x != 0) return;
    const double beta = sqrt(S.ww[b]);
@@ -339,7 +368,7 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* 
    S.resid[2 * b + 1] = o.reduction * beta;
    S.resid[2 * b] = beta;
    S.g[(long)b * (S.m + 1)] = beta;
-   S.scale[b] = beta > 0.0 ? 1.0 / beta : 0.0;
+   S.vs[(long)b * (S.m + 1)] = beta > 0.0 ? 1.0 / beta : 0.0;
    fl[0] = beta > 0.0 ? 0 : 1;
    fl[1] = 0;
    fl[2] = 0;
@@ -347,126 +376,55 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* 
    S.start[b] = 0;
 }
 
-__device__ __forceinline__ bool gmres_on(const BState& S, int b) {
-   return b >= 0 && S.phase[b] == kInner && !S.flags[4 * b];
-}
-
-// z = B^-1 (V[j] * scale), V[j] *= scale (normalises the new basis vector)
-template <class T>
-__device__ __forceinline__ void prec_step(const BLayout& L, T* Vj, const T* binv, T* z, const BState& S) {
-   const int b = slot_of_wg(L);
-   if (!gmres_on(S, b)) return;
-   const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const long local = row - L.row_off[b];
-   const int bs = L.pbs[b];
-   const T sc = (T)S.scale[b];
-   T own = 0, acc = 0;
-   const bool real = local < L.nrows[b];
-   if (real) {
-      const long cell = local / bs;
-      const int gr = (int)(local - cell * bs);
-      const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
-      const T* v = Vj + L.row_off[b] + cell * bs;
-      for (int q = 0; q < bs; q++) acc += bi[q] * (v[q] * sc);
-      own = Vj[row] * sc;
-   }
-   __syncthreads();
-   z[row] = acc;
-   if (real) Vj[row] = own;
-}
-
-template <class T>
-__global__ void __launch_bounds__(kRows) bprec_kernel(BLayout L, T* Vj, const T* binv, T* z, BState S) {
-   prec_step<T>(L, Vj, binv, z, S);
-}
-
-// w = A z and h(i,j) = <V[i], w>, i <= j
+// w = A (vs_j z) and h(i,j) = <vs_i V_i, w>, i <= j  (z = B^-1 V_j, unscaled)
 template <class T>
 __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
-                                              T* w, const T* V, int j, double* partials, unsigned* tickets,
+                                              T* w, const T* V, int j, double* partials, unsigned* tickets, unsigned* gtickets,
                                               const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
+   const double* vs = S.vs + (long)b * (S.m + 1);
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const long base = A.off[s] + (threadIdx.x & 63);
-   const T acc = sell_row_dot<T, T>(A.col, aval, z, base, A.width[s], (T)0);
+   const T acc = (T)vs[j] * sell_row_dot<T, T>(A.col, aval, z, base, A.width[s], (T)0);
    w[row] = acc;
    const double wr = (double)acc;
    const int nv = j + 1;
    seg_reduce<kBMaxM>(
-       nv, [&](int k) { return (double)V[(long)k * N + row] * wr; }, partials + (long)L.wg_first[b] * nv,
-       tickets + b, S.H + (long)b * (S.m + 1) * S.m + (long)j * (S.m + 1), (int)blockIdx.x - L.wg_first[b],
+       nv, [&](int k) { return (double)V[(long)k * N + row] * vs[k] * wr; }, partials + (long)L.wg_first[b] * nv,
+       tickets + b, gtickets + L.wg_first[b], S.H + (long)b * (S.m + 1) * S.m + (long)j * (S.m + 1), (int)blockIdx.x - L.wg_first[b],
        L.wg_count[b]);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSell A, const T* aval,
                                                           const T* z, T* w, const T* V, int j,
-                                                          double* partials, unsigned* tickets, BState S) {
-   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, S);
+                                                          double* partials, unsigned* tickets, unsigned* gtickets, BState S) {
+   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, gtickets, S);
 }
 
-// The GMRES steps with their execution span recorded for timing: every
-// workgroup folds its first and last device wall-clock reading into clk[0]
-// (earliest start) and clk[1] (latest end).  Same arithmetic as the untimed
-// kernels.
-__device__ __forceinline__ void record_span(unsigned long long t0, unsigned long long* clk) {
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      atomicMin(&clk[0], t0);
-      atomicMax(&clk[1], (unsigned long long)wall_clock64());
-   }
-}
 
+// V[j+1] = w - sum_i h(i,j) vs_i V_i (stored unnormalised), z = B^-1 V[j+1],
+// h(j+1,j) = |V[j+1]|, vs_{j+1} = 1/h(j+1,j), Givens update, stop test
 template <class T>
-__global__ void __launch_bounds__(kRows) bprec_timed_kernel(BLayout L, T* Vj, const T* binv, T* z, BState S,
-                                                            unsigned long long* clk) {
-   const unsigned long long t0 = wall_clock64();
-   prec_step<T>(L, Vj, binv, z, S);
-   record_span(t0, clk);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kRows) bspmv_dot_timed_kernel(long N, BLayout L, BSell A, const T* aval,
-                                                                const T* z, T* w, const T* V, int j,
-                                                                double* partials, unsigned* tickets, BState S,
-                                                                unsigned long long* clk) {
-   const unsigned long long t0 = wall_clock64();
-   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, S);
-   record_span(t0, clk);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, const T* w, T* V, int j,
-                                                            double* partials, unsigned* tickets, BState S,
-                                                            unsigned long long* clk) {
-   const unsigned long long t0 = wall_clock64();
-   orth_step<T>(N, L, w, V, j, partials, tickets, S);
-   record_span(t0, clk);
-}
-
-__global__ void clk_reset_kernel(unsigned long long* clk, int n) {
-   int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i < n) clk[i] = (i & 1) ? 0ull : ~0ull;
-}
-
-// V[j+1] = w - sum_i h(i,j) V[i], h(j+1,j) = |V[j+1]|, Givens update, stop test
-template <class T>
-__device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, T* V, int j, double* partials,
-                                          unsigned* tickets, const BState& S) {
+__device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, T* V, T* z, const T* binv, int j,
+                                          double* partials, unsigned* tickets, unsigned* gtickets, const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
+   __shared__ T tl[kRows];
    const int m = S.m;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    double* Hb = S.H + (long)b * (m + 1) * m;
    const double* h = Hb + (long)j * (m + 1);
+   const double* vs = S.vs + (long)b * (m + 1);
    T t = w[row];
-   for (int i = 0; i <= j; i++) t -= (T)h[i] * V[(long)i * N + row];
+   for (int i = 0; i <= j; i++) t -= (T)(h[i] * vs[i]) * V[(long)i * N + row];
    V[(long)(j + 1) * N + row] = t;
+   z[row] = cell_block_apply<T>(L, b, binv, t, tl);
    const double u = (double)t;
    bool last = seg_reduce<1>(
-       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, S.ww + b,
+       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
        (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    if (!last) return;
    __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], gj;
@@ -503,14 +461,50 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, 
    fl[1] = j + 1;
    const bool breakdown = !(hn > 1e-300) || !(rr > 0.0);
    if (breakdown) fl[2] = 1;
-   if (res <= S.resid[2 * b + 1] || breakdown || j + 1 == m) fl[0] = 1;
-   S.scale[b] = hn > 0.0 ? 1.0 / hn : 0.0;
+   if (res <= S.resid[2 * b + 1] || breakdown || j + 1 == L.mlen[b]) fl[0] = 1;
+   S.vs[(long)b * (m + 1) + j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
 }
 
 template <class T>
-__global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, int j,
-                                                      double* partials, unsigned* tickets, BState S) {
-   orth_step<T>(N, L, w, V, j, partials, tickets, S);
+__global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, T* z, const T* binv,
+                                                      int j, double* partials, unsigned* tickets, unsigned* gtickets, BState S) {
+   orth_step<T>(N, L, w, V, z, binv, j, partials, tickets, gtickets, S);
+}
+
+// The GMRES steps with their execution span recorded for timing: every
+// workgroup folds its first and last device wall-clock reading into clk[0]
+// (earliest start) and clk[1] (latest end).  Same arithmetic as the untimed
+// kernels.
+__device__ __forceinline__ void record_span(unsigned long long t0, unsigned long long* clk) {
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      atomicMin(&clk[0], t0);
+      atomicMax(&clk[1], (unsigned long long)wall_clock64());
+   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRows) bspmv_dot_timed_kernel(long N, BLayout L, BSell A, const T* aval,
+                                                                const T* z, T* w, const T* V, int j,
+                                                                double* partials, unsigned* tickets, unsigned* gtickets, BState S,
+                                                                unsigned long long* clk) {
+   const unsigned long long t0 = wall_clock64();
+   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, gtickets, S);
+   record_span(t0, clk);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, const T* w, T* V, T* z,
+                                                            const T* binv, int j, double* partials,
+                                                            unsigned* tickets, unsigned* gtickets, BState S, unsigned long long* clk) {
+   const unsigned long long t0 = wall_clock64();
+   orth_step<T>(N, L, w, V, z, binv, j, partials, tickets, gtickets, S);
+   record_span(t0, clk);
+}
+
+__global__ void clk_reset_kernel(unsigned long long* clk, int n) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) clk[i] = (i & 1) ? 0ull : ~0ull;
 }
 
 // y = H^-1 g, one workgroup per slot that took steps this cycle
@@ -553,7 +547,8 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const
    const int bs = L.pbs[b];
    __shared__ T u[kRows];
    T acc = 0;
-   for (int i = 0; i < k; i++) acc += (T)S.y[(long)b * S.m + i] * V[(long)i * N + row];
+   const double* vs = S.vs + (long)b * (S.m + 1);
+   for (int i = 0; i < k; i++) acc += (T)(S.y[(long)b * S.m + i] * vs[i]) * V[(long)i * N + row];
    u[threadIdx.x] = acc;
    __syncthreads();
    if (local < L.nrows[b]) {
@@ -567,24 +562,26 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const
    }
 }
 
-// true residual V[0] = rhs - A e after the cycle, and the stop decision
+// true residual V[0] = rhs - A e after the cycle, z = B^-1 V[0], and the stop decision
 template <class T>
 __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, const T* aval, const T* e,
-                                                         const T* rhs, T* V0, BOpts o, double* partials,
-                                                         unsigned* tickets, BState S) {
+                                                         const T* rhs, T* V0, T* z, const T* binv, BOpts o,
+                                                         double* partials, unsigned* tickets, unsigned* gtickets, BState S) {
    const int b = slot_of_wg(L);
    // slots not in the inner phase, or stopped before this cycle took a step,
    // keep their flags
    if (b < 0 || S.phase[b] != kInner || (S.flags[4 * b] && S.flags[4 * b + 1] == 0)) return;
+   __shared__ T tl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const long base = A.off[s] + (threadIdx.x & 63);
    const T acc = sell_row_dot<T, T>(A.col, aval, e, base, A.width[s], (T)0);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
+   z[row] = cell_block_apply<T>(L, b, binv, rv, tl);
    const double u = (double)rv;
    bool last = seg_reduce<1>(
-       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, S.ww + b,
+       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
        (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    if (!last || threadIdx.x != 0) return;
    const double beta = sqrt(S.ww[b]);
@@ -592,7 +589,7 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    fl[3] += fl[1];
    S.g[(long)b * (S.m + 1)] = beta;
    S.resid[2 * b] = beta;
-   S.scale[b] = beta > 0.0 ? 1.0 / beta : 0.0;
+   S.vs[(long)b * (S.m + 1)] = beta > 0.0 ? 1.0 / beta : 0.0;
    fl[0] = (beta <= S.resid[2 * b + 1] || !(beta > 0.0) || fl[3] >= o.max_inner) ? 1 : 0;
    fl[1] = 0;
    fl[2] = 0;
@@ -715,11 +712,11 @@ struct pk_batch_s {
    // capacity
    long max_units = 0, max_rentries = 0, max_fentries = 0, max_binv = 0;
    int max_slots = 0;
-   int m = 30;
+   int m = 60;
    int precision = PK_PREC_FP64;
    BOpts opts{1e-8, 0.1, 0.1, 10000, 2000};
    // device arrays
-   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs;
+   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen;
    DBuf<long> row_off, binv_off;
    DBuf<long> r_off, f_off;
    DBuf<int> r_width, f_width, r_col, f_col, rlen;
@@ -729,6 +726,7 @@ struct pk_batch_s {
    DBuf<int> si;
    DBuf<double> part;
    DBuf<unsigned> tick;
+   DBuf<unsigned> gtick;  // group counters of the two-level reductions, one per row unit
    DBuf<int> err;
    Inner<double> i64;
    Inner<float> i32;
@@ -752,7 +750,7 @@ struct pk_batch_s {
    double seconds = 0.0;
 
    BLayout layout() const {
-      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p};
+      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p};
    }
    BSell rsell() const { return BSell{r_off.p, r_width.p, r_col.p, rlen.p}; }
    BSell fsell() const { return BSell{f_off.p, f_width.p, f_col.p, rlen.p}; }
@@ -792,12 +790,14 @@ static int pool_alloc(pk_batch_s* P) {
    const int S = P->max_slots, m = P->m;
    const long rent = P->max_rentries * 64, fent = P->max_fentries * 64, binv = P->max_binv * 64;
    bool bad = P->unit_slot.alloc(units) || P->wg_first.alloc(S) || P->wg_count.alloc(S) || P->nrows.alloc(S) ||
+              P->mlen.alloc(S) ||
               P->pbs.alloc(S) || P->fbs.alloc(S) || P->row_off.alloc(S) || P->binv_off.alloc(S) ||
               P->r_off.alloc(sl + 1) || P->f_off.alloc(sl + 1) || P->r_width.alloc(sl) || P->f_width.alloc(sl) ||
               P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
               P->f_col.alloc(fent) || P->f_val.alloc(fent) || P->x.alloc(rows) || P->phi.alloc(rows) ||
               P->y1.alloc(rows) || P->y2.alloc(rows) || P->r.alloc(rows) ||
-              P->part.alloc((size_t)units * (m + 4)) || P->tick.alloc(S) || P->err.alloc(1);
+              P->part.alloc((size_t)units * (m + 4)) || P->tick.alloc(S) || P->gtick.alloc(units) ||
+              P->err.alloc(1);
    PK_REQUIRE(!bad, "device allocation failed for the eigen pool");
    auto alloc_inner = [&](auto& I, bool own_a) {
       return I.V.alloc((size_t)(m + 1) * rows) || I.z.alloc(rows) || I.w.alloc(rows) || I.e.alloc(rows) ||
@@ -808,7 +808,7 @@ static int pool_alloc(pk_batch_s* P) {
    else
       PK_REQUIRE(!alloc_inner(P->i64, false), "device allocation failed for the eigen pool");
    // per-slot state
-   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 4 * (size_t)m + 24);
+   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 5 * (size_t)m + 24);
    const size_t ni = (size_t)S * 12;
    PK_REQUIRE(!P->sd.alloc(nd) && !P->si.alloc(ni), "device allocation failed for the eigen pool");
    PK_HIP(hipMemsetAsync(P->sd.p, 0, sizeof(double) * nd, c.stream));
@@ -826,7 +826,7 @@ static int pool_alloc(pk_batch_s* P) {
    B.sn = take((size_t)S * m);
    B.g = take((size_t)S * (m + 1));
    B.y = take((size_t)S * m);
-   B.scale = take(S);
+   B.vs = take((size_t)S * (m + 1));
    B.resid = take(2 * (size_t)S);
    B.ww = take(S);
    B.dots = take(3 * (size_t)S);
@@ -861,6 +861,7 @@ static int pool_alloc(pk_batch_s* P) {
    PK_HIP(hipMemsetAsync(P->x.p, 0, sizeof(double) * rows, c.stream));
    PK_HIP(hipMemsetAsync(P->phi.p, 0, sizeof(double) * rows, c.stream));
    PK_HIP(hipMemsetAsync(P->tick.p, 0, sizeof(unsigned) * S, c.stream));
+   PK_HIP(hipMemsetAsync(P->gtick.p, 0, sizeof(unsigned) * units, c.stream));
    PK_HIP(hipMemsetAsync(P->err.p, 0, sizeof(int), c.stream));
    auto zero_inner = [&](auto& I, size_t t) {
       return hipMemsetAsync(I.V.p, 0, t * (m + 1) * rows, c.stream) ||
@@ -970,12 +971,17 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    PK_HIP(hipGetLastError());
    // slot tables
    const int i_first = (int)u0, i_count = (int)units, i_n = (int)n, i_bs = bs, i_pbs = pbs;
+   // restart length: the pool's, except 30 for cell blocks of 16 groups and
+   // more, whose stronger block-Jacobi blocks gain too few iterations from a
+   // longer basis to pay for it (measured on the 256 x 256 instances)
+   const int i_mlen = bs >= 16 ? std::min(P->m, 30) : P->m;
    const long l_row0 = row0, l_b0 = b0 * 64;
    PK_HIP(hipMemcpyAsync(P->wg_first.p + sidx, &i_first, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->wg_count.p + sidx, &i_count, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->nrows.p + sidx, &i_n, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->fbs.p + sidx, &i_bs, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->pbs.p + sidx, &i_pbs, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->mlen.p + sidx, &i_mlen, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->row_off.p + sidx, &l_row0, sizeof(long), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->binv_off.p + sidx, &l_b0, sizeof(long), hipMemcpyHostToDevice, c.stream));
    // state: OUTER, rebuild (A = R, block inverses) on the first tick
@@ -1052,11 +1058,12 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    const BState S = P->S;
    double* part = P->part.p;
    unsigned* tk = P->tick.p;
+   unsigned* gtk = P->gtick.p;
    const bool fp32 = sizeof(T) != sizeof(double);
    const T* aval = fp32 ? I.aval.p : (const T*)P->a_val.p;
-   brf_kernel<<<nwg, kRows, 0, c.stream>>>(L, Rs, P->r_val.p, Fs, P->f_val.p, P->x.p, P->y1.p, P->y2.p, part, tk,
+   brf_kernel<<<nwg, kRows, 0, c.stream>>>(L, Rs, P->r_val.p, Fs, P->f_val.p, P->x.p, P->y1.p, P->y2.p, part, tk, gtk,
                                            S);
-   bresid_kernel<<<nwg, kRows, 0, c.stream>>>(L, P->x.p, P->y1.p, P->y2.p, P->phi.p, P->r.p, o, part, tk, S);
+   bresid_kernel<<<nwg, kRows, 0, c.stream>>>(L, P->x.p, P->y1.p, P->y2.p, P->phi.p, P->r.p, o, part, tk, gtk, S);
    bshift_kernel<<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, Rs, P->r_val.p, Fs, P->f_val.p, S,
                                                           P->a_val.p);
    if (fp32)
@@ -1064,7 +1071,8 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    bblock_inverse_kernel<T><<<grid_for(rows, 64), 64, 0, c.stream>>>(rows, L, Rs, P->a_val.p, S.rebuild,
                                                                      I.binv.p, P->err.p);
    bclear_rebuild_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
-   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->r.p, I.rhs.p, I.e.p, I.V.p, o, part, tk, S);
+   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->r.p, I.rhs.p, I.e.p, I.V.p, I.z.p, I.binv.p, o, part, tk, gtk,
+                                                 S);
    const double s = sizeof(T);
    std::vector<int> hf, ph;
    for (int j = 0; j < m; j++) {
@@ -1083,34 +1091,31 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
                bsn += (double)P->slots[b].n * P->slots[b].pbs;
             }
          if (n == 0.0) break;
-         // prec reads V[j] and the block inverses, writes z and V[j]; spmv_dot is
-         // an SpMV (value and column of every stored entry, z gathered once, w
-         // written) plus j+1 basis vectors; orth reads w and j+1 basis vectors and
-         // writes V[j+1]
-         // the step's kernels record their execution span on the device clock
-         // (prec, spmv_dot, orth: clk[0..5])
-         clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
-         bprec_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S,
-                                                             P->clk.p);
-         bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk,
-                                                                S, P->clk.p + 2);
-         borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, j, part, tk, S, P->clk.p + 4);
-         unsigned long long h[6];
+         // algorithmic bytes: spmv_dot is an SpMV (value and column of every
+         // stored entry, z gathered once, w written) plus j+1 basis vectors; orth
+         // reads w and j+1 basis vectors, writes V[j+1] and z and reads the block
+         // inverses.  The two kernels record their execution span on the device
+         // clock (clk[0..3]).
+         clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 4);
+         bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk,
+                                                                S, P->clk.p);
+         borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, I.z.p, I.binv.p, j, part, tk, gtk, S,
+                                                            P->clk.p + 2);
+         unsigned long long h[4];
          PK_HIP(hipMemcpyAsync(h, P->clk.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
-         prof_add("gmres_prec", 1, (double)(h[1] - h[0]) / P->clock_hz, n * s * 3.0 + bsn * s);
-         prof_add("gmres_spmv_dot", 1, (double)(h[3] - h[2]) / P->clock_hz,
+         prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
                   nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s);
-         prof_add("gmres_orth", 1, (double)(h[5] - h[4]) / P->clock_hz, n * s * (j + 3.0));
+         prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz, n * s * (j + 4.0) + bsn * s);
       } else {
-         bprec_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, I.V.p + (size_t)j * N, I.binv.p, I.z.p, S);
-         bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, S);
-         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, j, part, tk, S);
+         bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S);
+         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, I.z.p, I.binv.p, j, part, tk, gtk, S);
       }
    }
    btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
    bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p);
-   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, o, part, tk, S);
+   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, I.z.p, I.binv.p, o, part,
+                                                   tk, gtk, S);
    bfinish_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, P->phi.p, I.e.p, S, P->x.p);
    bphase_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
    PK_HIP(hipGetLastError());
@@ -1135,7 +1140,7 @@ static int run_tick(pk_batch_s* P, long units) {
       if (!P->clk.p) {
          int khz = 0;
          PK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device));
-         PK_REQUIRE(khz > 0 && !P->clk.alloc(6), "no device clock for profiling");
+         PK_REQUIRE(khz > 0 && !P->clk.alloc(4), "no device clock for profiling");
          P->clock_hz = khz * 1e3;
       }
       return launch_tick<T>(P, units, true);
@@ -1301,6 +1306,25 @@ int pk_batch_create(int num, const pk_mat* R, const pk_mat* F, pk_batch* out) {
    return 0;
 }
 
+int pk_batch_set_member(pk_batch P, int index, pk_mat R, pk_mat F) {
+   PK_REQUIRE(P && index >= 0 && index < (int)P->fixR.size(), "no such batch member");
+   PK_CALL(check_member(R, F));
+   long units = 0, re = 0, fe = 0, bi = 0;
+   for (int b = 0; b < (int)P->fixR.size(); b++) {
+      pk_mat_s* r = b == index ? R : P->fixR[b];
+      pk_mat_s* f = b == index ? F : P->fixF[b];
+      units += units_of(r->n);
+      re += blocks64(r->sell.padded);
+      fe += blocks64(f->sell.padded);
+      bi += blocks64(r->n * preconditioner_block(r->bs));
+   }
+   PK_REQUIRE(units <= P->max_units && re <= P->max_rentries && fe <= P->max_fentries && bi <= P->max_binv,
+              "the new member does not fit the batch's pool");
+   P->fixR[index] = R;
+   P->fixF[index] = F;
+   return 0;
+}
+
 int pk_batch_set_tolerances(pk_batch P, double tol, int max_outer) {
    PK_REQUIRE(P && tol > 0.0 && max_outer > 0, "wrong tolerances");
    P->opts.tol = tol;
@@ -1311,7 +1335,7 @@ int pk_batch_set_tolerances(pk_batch P, double tol, int max_outer) {
 
 int pk_batch_set_precision(pk_batch P, int precision) {
    PK_REQUIRE(P && (precision == PK_PREC_FP64 || precision == PK_PREC_MIXED), "wrong precision");
-   PK_REQUIRE(!P->sd.p, "precision must be set before the first solve");
+   PK_REQUIRE(!P->sd.p || precision == P->precision, "precision must be set before the first solve");
    P->precision = precision;
    return 0;
 }

@@ -107,12 +107,49 @@ __device__ bool grid_reduce(int nv, Get get, double* partials, unsigned* ticket,
    return true;
 }
 
+// Sum of partial rows first .. first+count-1 (entry k of row e at
+// partials[e * stride * nv + k]) by the whole workgroup, L lanes per value,
+// fixed order; lanes sub == 0 return the sums of their value k.
+__device__ __forceinline__ double sum_partials(const double* partials, int count, int stride, int nv, int& k,
+                                               int& sub) {
+   int L = 64;
+   while (L > 1 && L * nv > kBlock) L >>= 1;
+   k = threadIdx.x / L;
+   sub = threadIdx.x % L;
+   auto ld = [&](long p) {
+      return __hip_atomic_load(&partials[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   };
+   double s = 0.0;
+   if (k < nv) {
+      int b = sub;
+      for (; b + 3 * L < count; b += 4 * L) {
+         double a0 = ld((long)b * stride * nv + k);
+         double a1 = ld((long)(b + L) * stride * nv + k);
+         double a2 = ld((long)(b + 2 * L) * stride * nv + k);
+         double a3 = ld((long)(b + 3 * L) * stride * nv + k);
+         s += a0;
+         s += a1;
+         s += a2;
+         s += a3;
+      }
+      for (; b < count; b += L) s += ld((long)b * stride * nv + k);
+   }
+   for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+   return s;
+}
+
 // Segmented form: the workgroups of one segment (one batched problem) reduce
 // among themselves.  `seg_block` is this workgroup's index inside its segment,
 // `seg_blocks` the segment's workgroup count, `partials` the segment's base
-// (seg_blocks * nv doubles) and `ticket` the segment's counter.
+// (seg_blocks * nv doubles), `ticket` the segment's counter and `gticket` the
+// segment's group counters (one per workgroup).  Segments of more than
+// kSegGroup workgroups reduce in two levels: the last workgroup of each group
+// of kSegGroup sums its group's partials into the group's first row, and the
+// last group to finish sums the group rows; the order of every sum is fixed.
+constexpr int kSegGroup = 32;
+
 template <int MAXV, class Get>
-__device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, double* out,
+__device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, unsigned* gticket, double* out,
                            int seg_block, int seg_blocks) {
    __shared__ double sh[MAXV][kBlock / 64];
    __shared__ int is_last;
@@ -134,33 +171,32 @@ __device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, 
    }
    __builtin_amdgcn_s_waitcnt(0);
    __syncthreads();
+   int count = seg_blocks, stride = 1;
+   if (seg_blocks > kSegGroup) {
+      const int g0 = seg_block / kSegGroup * kSegGroup;
+      const int gs = min(kSegGroup, seg_blocks - g0);
+      if (threadIdx.x == 0)
+         is_last = (__hip_atomic_fetch_add(&gticket[g0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                    (unsigned)gs - 1);
+      __syncthreads();
+      if (!is_last) return false;
+      int k, sub;
+      const double s = sum_partials(partials + (long)g0 * nv, gs, 1, nv, k, sub);
+      if (k < nv && sub == 0)
+         __hip_atomic_store(&partials[(long)g0 * nv + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (threadIdx.x == 0) __hip_atomic_store(&gticket[g0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      count = (seg_blocks + kSegGroup - 1) / kSegGroup;
+      stride = kSegGroup;
+   }
    if (threadIdx.x == 0)
       is_last = (__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-                 (unsigned)seg_blocks - 1);
+                 (unsigned)count - 1);
    __syncthreads();
    if (!is_last) return false;
-   int L = 64;
-   while (L > 1 && L * nv > kBlock) L >>= 1;
-   const int k = threadIdx.x / L, sub = threadIdx.x % L;
-   auto ld = [&](long p) {
-      return __hip_atomic_load(&partials[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-   };
-   double s = 0.0;
-   if (k < nv) {
-      int b = sub;
-      for (; b + 3 * L < seg_blocks; b += 4 * L) {
-         double a0 = ld((long)b * nv + k);
-         double a1 = ld((long)(b + L) * nv + k);
-         double a2 = ld((long)(b + 2 * L) * nv + k);
-         double a3 = ld((long)(b + 3 * L) * nv + k);
-         s += a0;
-         s += a1;
-         s += a2;
-         s += a3;
-      }
-      for (; b < seg_blocks; b += L) s += ld((long)b * nv + k);
-   }
-   for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+   int k, sub;
+   const double s = sum_partials(partials, count, stride, nv, k, sub);
    if (k < nv && sub == 0) out[k] = s;
    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
    __threadfence_block();

@@ -474,6 +474,15 @@ int pk_eig_create(pk_mat R, pk_mat F, pk_eig* E) {
    return 0;
 }
 
+int pk_eig_set_operators(pk_eig E, pk_mat R, pk_mat F) {
+   PK_REQUIRE(E, "null eigen solver");
+   PK_CALL(pk_batch_set_member(E->B, 0, R, F));
+   E->R = R;
+   E->F = F;
+   E->solved = false;
+   return 0;
+}
+
 int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer) {
    PK_REQUIRE(E, "null eigen solver");
    return pk_batch_set_tolerances(E->B, tol, max_outer);

@@ -92,3 +92,26 @@ def test_linear_solver(pk):
     x = X.get()
     assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
     lin.close()
+
+
+def test_eigen_solver_reused_across_solves(pk, tmp_path):
+    """The drop-in keeps its eigen solver between steady solves of one problem:
+    repeated cold solves give the same result every time."""
+    import os
+    import numpy as np
+    from pampa_amd import workloads as W
+    d = W.slab_1d(str(tmp_path / "slab"))
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        for prec in ("fp64", "mixed"):
+            with pk.DropIn("input.pmp", "-silent", "-pk_precision", prec) as run:
+                C, G, _ = run.sizes()
+                ks = []
+                for _ in range(3):
+                    run.set_field("scalar-flux", np.zeros(C * G))
+                    run.solve()
+                    ks.append(run.result()[0])
+                assert ks[0] == ks[1] == ks[2]
+    finally:
+        os.chdir(cwd)
