@@ -169,6 +169,7 @@ def _serve(pa, paths, precision, slots, arrival, cid, seeds, rel, warm, keep=(),
     for p in paths:
         srv.add_case(p)
     log(f"[bench]   {len(paths)} cases resident ({time.perf_counter() - t0:.1f} s)")
+    srv.progress(30.0)
     if warm:
         # one unperturbed solve per base case first: the stream's warm starts
         srv.run(np.zeros(len(paths)), np.arange(len(paths)), np.zeros(len(paths), np.uint64), rel=0.0,

@@ -89,6 +89,8 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
                      const unsigned long long* seed, double rel, int warm, double* keff, double* latency,
                      int* outer, long* inner, int* converged, double* const* flux, int* error);
 void pampa_serve_forget_warm(pampa_server S, int* error);
+/* seconds > 0: pampa_serve_run prints a progress line on stderr that often */
+void pampa_serve_set_progress(pampa_server S, double seconds, int* error);
 /* Every case once in the pool, and the SpMV of all their inner operators in
  * one launch timed (average seconds over reps, algorithmic CSR bytes, rows). */
 void pampa_serve_time_spmv(pampa_server S, int reps, double* seconds, double* bytes, double* rows, int* error);

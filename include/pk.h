@@ -137,6 +137,9 @@ int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer);
 int pk_eig_set_precision(pk_eig E, int precision);
 int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner);
 int pk_eig_set_shift(pk_eig E, double delta); /* Wielandt shift k_s = k + delta (0: none) */
+/* inner preconditioner: 0 = cell-block Jacobi B^-1; 1 (default) = B^-1 (2 I - A B^-1),
+ * one Richardson correction (one more SpMV per GMRES step, about half the steps) */
+int pk_eig_set_preconditioner(pk_eig E, int poly);
 int pk_eig_set_initial(pk_eig E, pk_vec x);
 int pk_eig_solve(pk_eig E);
 int pk_eig_get_eigenpair(pk_eig E, double* lambda, pk_vec x); /* lambda = 1/keff */
@@ -155,6 +158,7 @@ int pk_batch_set_tolerances(pk_batch B, double tol, int max_outer);
 int pk_batch_set_precision(pk_batch B, int precision);
 int pk_batch_set_inner(pk_batch B, int restart, double reduction, int max_inner);
 int pk_batch_set_shift(pk_batch B, double delta);
+int pk_batch_set_preconditioner(pk_batch B, int poly); /* as pk_eig_set_preconditioner */
 int pk_batch_set_initial(pk_batch B, int index, pk_vec x); /* all-zero x: no initial space */
 int pk_batch_solve(pk_batch B);
 int pk_batch_get_eigenpair(pk_batch B, int index, double* lambda, pk_vec x);

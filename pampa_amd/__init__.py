@@ -109,7 +109,7 @@ def _declare(L):
         "pampa_serve_case_tables": [_P, _I, ct.c_ulonglong, _D, _P, _P, _P, _P, _IP],
         "pampa_serve_footprint": [_P, _I, _P, _P, _IP],
         "pampa_serve_run": [_P, _I, _P, _P, _ULLP, _D, _I, _P, _P, _P, _P, _P, _P, _IP],
-        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_time_spmv": [_P, _I, _P, _P, _P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
+        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_set_progress": [_P, _D, _IP], "pampa_serve_time_spmv": [_P, _I, _P, _P, _P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
         "pampa_serve_destroy": [_PP, _IP],
     })
     for name, args in drop.items():
@@ -490,6 +490,10 @@ class Server:
         t, b, r = ct.c_double(), ct.c_double(), ct.c_double()
         self._call("pampa_serve_time_spmv", self.h, reps, ct.byref(t), ct.byref(b), ct.byref(r))
         return t.value, b.value, r.value
+
+    def progress(self, seconds: float):
+        """Print a progress line on stderr every `seconds` during run()."""
+        self._call("pampa_serve_set_progress", self.h, seconds)
 
     def forget_warm(self):
         self._call("pampa_serve_forget_warm", self.h)

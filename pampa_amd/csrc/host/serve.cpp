@@ -12,6 +12,7 @@
 // case remembers the flux of its latest converged request and can start the
 // next one from it (warm start).
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -100,6 +101,7 @@ struct Server {
    double seconds = 0.0;
    long ticks = 0;
    int max_live = 0;
+   double progress = 0.0;  // > 0: a progress line on stderr every `progress` seconds of a run
    ~Server() { pk_batch_destroy(&pool); }
 
    int requestOperators(Case& c, unsigned long long seed, double rel) {
@@ -246,8 +248,17 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
       if (rc) report_error(__FILE__, __LINE__, "pampa_serve_run", pk_last_error());
       return rc != 0;
    };
+   double next_report = S->progress;
    while (done < n) {
       const double t = now();
+      if (S->progress > 0.0 && t >= next_report) {
+         long tk = 0;
+         pk_pool_capacity(S->pool, nullptr, nullptr, nullptr, &tk);
+         fprintf(stderr, "[serve] %.1f s: %d of %d done, %d in flight, %zu waiting, %ld ticks\n", t, done, n, live,
+                 pending.size(), tk - ticks0);
+         fflush(stderr);
+         next_report = t + S->progress;
+      }
       while (next < n && arrival[next] <= t) pending.push_back(next++);
       // admission in arrival order while the pool has room
       while (!pending.empty()) {
@@ -347,6 +358,11 @@ void pampa_serve_time_spmv(pampa_server S, int reps, double* seconds, double* by
    if (rows) *rows = nrows;
    cleanup();
    *error = 0;
+}
+
+void pampa_serve_set_progress(pampa_server S, double seconds, int* error) {
+   *error = S ? 0 : 1;
+   if (S) S->progress = seconds;
 }
 
 void pampa_serve_forget_warm(pampa_server S, int* error) {

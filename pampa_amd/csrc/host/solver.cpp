@@ -219,6 +219,7 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       PK_CHECK(pk_eig_set_precision(eps, options.precision));
       PK_CHECK(pk_eig_set_inner(eps, options.gmres_restart, options.inner_reduction, 2000));
       PK_CHECK(pk_eig_set_shift(eps, options.wielandt_shift));
+      PK_CHECK(pk_eig_set_preconditioner(eps, options.preconditioner));
       PK_CHECK(pk_eig_set_initial(eps, phi));
       int rc = pk_eig_solve(eps);
       if (rc) {

@@ -18,6 +18,7 @@ struct SolverOptions {
    int gmres_restart = 60;
    double inner_reduction = 0.1;
    double wielandt_shift = 0.1;
+   int preconditioner = 1;  // 0 cell-block Jacobi, 1 with one Richardson correction
    double ksp_rtol = 1e-5, ksp_atol = 1e-50;  // PETSc KSP defaults
    int ksp_max_it = 10000;
    double sn_tol_k = 1e-9, sn_tol_phi = 1e-8;

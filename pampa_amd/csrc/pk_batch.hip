@@ -80,6 +80,7 @@ struct BState {  // per-slot state (arrays over slots)
    double* kprev;  // [S]
    double* ks;     // [S] 0 = no shift
    double* res;    // [S]
+   double* dl;     // [S] shift distance of the slot (k_s = k + dl)
    int* phase;     // [S]
    int* start;     // [S] 1: the inner solve starts this tick
    int* rebuild;   // [S] 1: rebuild A and its block inverses this tick
@@ -87,11 +88,13 @@ struct BState {  // per-slot state (arrays over slots)
    int* it;        // [S] corrections done
    int* inner;     // [S] GMRES steps in all corrections
    int* status;    // [S] 0 running, 1 converged, 2 failed
+   int* slow;      // [S] consecutive shifted outer updates that reduced the residual by less than 0.8
 };
 
 struct BOpts {
    double tol, delta, reduction;
    int max_outer, max_inner;
+   int poly;  // 1: preconditioner B^-1 (2 I - A B^-1) (one Richardson correction), 0: B^-1
 };
 
 __device__ __forceinline__ int slot_of_wg(const BLayout& L) { return L.unit_slot[blockIdx.x]; }
@@ -290,6 +293,7 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
    if (!last || threadIdx.x != 0) return;
    const double k = d[1] / d[0];
    const double res = sqrt(S.rq[2 * b] / S.rq[2 * b + 1]);
+   const double rprev = S.res[b];  // 0 before the first update
    S.k[b] = k;
    S.res[b] = res;
    if (res <= o.tol) {
@@ -303,9 +307,25 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
       return;
    }
    if (o.delta > 0.0) {
+      // Adaptive Wielandt shift: the outer residual falls by about
+      // (1/k - 1/k_s) / (1/k_2 - 1/k_s) per update, which tends to 1 when the
+      // dominance ratio k_2/k does; three shifted updates in a row that reduce
+      // the residual by less than 0.8 halve the shift distance (down to 0.004),
+      // at the price of harder inner solves
+      double dl = S.dl[b];
+      if (S.shifted[b] && rprev > 0.0) {
+         S.slow[b] = res > 0.8 * rprev ? S.slow[b] + 1 : 0;
+         if (S.slow[b] >= 3 && dl > 0.004) {
+            dl *= 0.5;
+            S.dl[b] = dl;
+            S.slow[b] = 0;
+            S.ks[b] = k + dl;
+            S.rebuild[b] = 1;
+         }
+      }
       const bool settle = S.it[b] >= 2 && fabs(k - S.kprev[b]) < 1e-2 * k;
-      if (settle && (!S.shifted[b] || fabs(k + o.delta - S.ks[b]) > 0.2 * o.delta)) {
-         S.ks[b] = k + o.delta;
+      if (settle && (!S.shifted[b] || fabs(k + dl - S.ks[b]) > 0.2 * dl)) {
+         S.ks[b] = k + dl;
          S.shifted[b] = 1;
          S.rebuild[b] = 1;
       }
@@ -494,6 +514,23 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_timed_kernel(long N, BLayout 
 }
 
 template <class T>
+__global__ void __launch_bounds__(kRows) bpoly_timed_kernel(BLayout L, BSell A, const T* aval, const T* v,
+                                                            const T* z0, const T* binv, T* z, BState S,
+                                                            unsigned long long* clk) {
+   const unsigned long long t0 = wall_clock64();
+   const int b = slot_of_wg(L);
+   if (gmres_on(S, b)) {
+      __shared__ T tl[kRows];
+      const long row = blockIdx.x * (long)kRows + threadIdx.x;
+      const long s = row >> 6;
+      const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
+      const T c = cell_block_apply<T>(L, b, binv, v[row] - az, tl);
+      z[row] = z0[row] + c;
+   }
+   record_span(t0, clk);
+}
+
+template <class T>
 __global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, const T* w, T* V, T* z,
                                                             const T* binv, int j, double* partials,
                                                             unsigned* tickets, unsigned* gtickets, BState S, unsigned long long* clk) {
@@ -534,10 +571,27 @@ __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) 
    if ((int)threadIdx.x < k) S.y[(long)b * m + threadIdx.x] = yy[threadIdx.x];
 }
 
-// e += B^-1 (sum_i y_i V[i])
+// Second term of the preconditioner M^-1 = B^-1 (2 I - A B^-1):
+// z = z0 + B^-1 (v - A z0), z0 = B^-1 v given.  mode 0: slots iterating in
+// this GMRES step; mode 1: slots whose cycle took steps (the update).
+template <class T>
+__global__ void __launch_bounds__(kRows) bpoly_kernel(BLayout L, BSell A, const T* aval, const T* v, const T* z0,
+                                                      const T* binv, T* z, BState S, int mode) {
+   const int b = slot_of_wg(L);
+   if (mode == 0 ? !gmres_on(S, b) : (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0)) return;
+   __shared__ T tl[kRows];
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const long s = row >> 6;
+   const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
+   const T c = cell_block_apply<T>(L, b, binv, v[row] - az, tl);
+   z[row] = z0[row] + c;
+}
+
+// u = sum_i y_i vs_i V[i]; accumulate: e += B^-1 u, else u and B^-1 u are
+// written out for the second preconditioner term (e += M^-1 u in two kernels)
 template <class T>
 __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const T* V, const T* binv,
-                                                        BState S, T* e) {
+                                                        BState S, T* e, int accumulate, T* uout, T* zout) {
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kInner) return;
    const int k = S.flags[4 * b + 1];
@@ -551,15 +605,30 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const
    for (int i = 0; i < k; i++) acc += (T)(S.y[(long)b * S.m + i] * vs[i]) * V[(long)i * N + row];
    u[threadIdx.x] = acc;
    __syncthreads();
+   T s = 0;
    if (local < L.nrows[b]) {
       const long cell = local / bs;
       const int gr = (int)(local - cell * bs);
       const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
       const int first = (int)(threadIdx.x - gr);  // a cell never straddles a workgroup
-      T s = 0;
       for (int q = 0; q < bs; q++) s += bi[q] * u[first + q];
-      e[row] += s;
    }
+   if (accumulate) {
+      if (local < L.nrows[b]) e[row] += s;
+   } else {
+      uout[row] = acc;
+      zout[row] = s;
+   }
+}
+
+// e += z for slots whose cycle took steps
+template <class T>
+__global__ void badd_kernel(long N, BLayout L, const T* z, BState S, T* e) {
+   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (row >= N) return;
+   const int b = slot_of_row(L, row);
+   if (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0) return;
+   e[row] += z[row];
 }
 
 // true residual V[0] = rhs - A e after the cycle, z = B^-1 V[0], and the stop decision
@@ -692,7 +761,7 @@ struct Slot {
 
 template <class T>
 struct Inner {
-   DBuf<T> V, z, w, e, rhs, binv, aval;
+   DBuf<T> V, z, z0, w, e, rhs, binv, aval;
 };
 
 struct Graph {
@@ -714,7 +783,7 @@ struct pk_batch_s {
    int max_slots = 0;
    int m = 60;
    int precision = PK_PREC_FP64;
-   BOpts opts{1e-8, 0.1, 0.1, 10000, 2000};
+   BOpts opts{1e-8, 0.1, 0.1, 10000, 2000, 1};
    // device arrays
    DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen;
    DBuf<long> row_off, binv_off;
@@ -777,7 +846,7 @@ static double pool_bytes(long units, long rent, long fent, long binv, int slots,
    b += units * 4.0 + sl * 2 * (8.0 + 4.0) + rows * 4.0;        // unit map, slice tables, rlen
    b += rent * 64.0 * (4.0 + 8.0 + 8.0) + fent * 64.0 * 12.0;    // R cols+vals, A (FP64), F
    b += rows * 8.0 * 5;                                          // x, phi, y1, y2, r
-   b += rows * t * (m + 1 + 4) + binv * 64.0 * t;                // basis, z w e rhs, block inverses
+   b += rows * t * (m + 1 + 5) + binv * 64.0 * t;                // basis, z z0 w e rhs, block inverses
    if (precision == PK_PREC_MIXED) b += rent * 64.0 * t;         // A copy in FP32
    b += (double)units * (m + 4) * 8.0;                           // reduction partials
    b += (double)slots * ((m + 1.0) * m + 4.0 * m + 24.0) * 8.0;  // per-slot state
@@ -800,7 +869,8 @@ static int pool_alloc(pk_batch_s* P) {
               P->err.alloc(1);
    PK_REQUIRE(!bad, "device allocation failed for the eigen pool");
    auto alloc_inner = [&](auto& I, bool own_a) {
-      return I.V.alloc((size_t)(m + 1) * rows) || I.z.alloc(rows) || I.w.alloc(rows) || I.e.alloc(rows) ||
+      return I.V.alloc((size_t)(m + 1) * rows) || I.z.alloc(rows) || I.z0.alloc(rows) || I.w.alloc(rows) ||
+             I.e.alloc(rows) ||
              I.rhs.alloc(rows) || I.binv.alloc(binv) || (own_a && I.aval.alloc(rent));
    };
    if (P->precision == PK_PREC_MIXED)
@@ -808,8 +878,8 @@ static int pool_alloc(pk_batch_s* P) {
    else
       PK_REQUIRE(!alloc_inner(P->i64, false), "device allocation failed for the eigen pool");
    // per-slot state
-   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 5 * (size_t)m + 24);
-   const size_t ni = (size_t)S * 12;
+   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 5 * (size_t)m + 26);
+   const size_t ni = (size_t)S * 13;
    PK_REQUIRE(!P->sd.alloc(nd) && !P->si.alloc(ni), "device allocation failed for the eigen pool");
    PK_HIP(hipMemsetAsync(P->sd.p, 0, sizeof(double) * nd, c.stream));
    PK_HIP(hipMemsetAsync(P->si.p, 0, sizeof(int) * ni, c.stream));
@@ -835,6 +905,7 @@ static int pool_alloc(pk_batch_s* P) {
    B.kprev = take(S);
    B.ks = take(S);
    B.res = take(S);
+   B.dl = take(S);
    int* q = P->si.p;
    auto takei = [&](size_t k) {
       int* p = q;
@@ -849,6 +920,7 @@ static int pool_alloc(pk_batch_s* P) {
    B.it = takei(S);
    B.inner = takei(S);
    B.status = takei(S);
+   B.slow = takei(S);
    // tables: every unit free, every slice empty
    P->h_unit_slot.assign(units, -1);
    PK_HIP(hipMemcpyAsync(P->unit_slot.p, P->h_unit_slot.data(), sizeof(int) * units, hipMemcpyHostToDevice,
@@ -865,7 +937,8 @@ static int pool_alloc(pk_batch_s* P) {
    PK_HIP(hipMemsetAsync(P->err.p, 0, sizeof(int), c.stream));
    auto zero_inner = [&](auto& I, size_t t) {
       return hipMemsetAsync(I.V.p, 0, t * (m + 1) * rows, c.stream) ||
-             hipMemsetAsync(I.z.p, 0, t * rows, c.stream) || hipMemsetAsync(I.w.p, 0, t * rows, c.stream) ||
+             hipMemsetAsync(I.z.p, 0, t * rows, c.stream) || hipMemsetAsync(I.z0.p, 0, t * rows, c.stream) ||
+             hipMemsetAsync(I.w.p, 0, t * rows, c.stream) ||
              hipMemsetAsync(I.e.p, 0, t * rows, c.stream) || hipMemsetAsync(I.rhs.p, 0, t * rows, c.stream);
    };
    if (P->precision == PK_PREC_MIXED)
@@ -996,6 +1069,10 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    PK_HIP(hipMemcpyAsync(P->S.status + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.ks + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.kprev + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.res + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
+   const double dl0 = P->opts.delta;
+   PK_HIP(hipMemcpyAsync(P->S.dl + sidx, &dl0, sizeof(double), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.slow + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.rebuild + sidx, &one, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.phase + sidx, &outer, sizeof(int), hipMemcpyHostToDevice, c.stream));
    // the slot's units become live last
@@ -1071,8 +1148,11 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    bblock_inverse_kernel<T><<<grid_for(rows, 64), 64, 0, c.stream>>>(rows, L, Rs, P->a_val.p, S.rebuild,
                                                                      I.binv.p, P->err.p);
    bclear_rebuild_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
-   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->r.p, I.rhs.p, I.e.p, I.V.p, I.z.p, I.binv.p, o, part, tk, gtk,
-                                                 S);
+   // with the polynomial preconditioner the first term B^-1 t goes to z0 and
+   // the step's bpoly kernel completes z; without it, straight into z
+   T* zfirst = o.poly ? I.z0.p : I.z.p;
+   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->r.p, I.rhs.p, I.e.p, I.V.p, zfirst, I.binv.p, o, part, tk,
+                                                 gtk, S);
    const double s = sizeof(T);
    std::vector<int> hf, ph;
    for (int j = 0; j < m; j++) {
@@ -1096,25 +1176,41 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          // reads w and j+1 basis vectors, writes V[j+1] and z and reads the block
          // inverses.  The two kernels record their execution span on the device
          // clock (clk[0..3]).
-         clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 4);
+         clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
+         if (o.poly)
+            bpoly_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p,
+                                                               I.z.p, S, P->clk.p + 4);
          bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk,
                                                                 S, P->clk.p);
-         borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, I.z.p, I.binv.p, j, part, tk, gtk, S,
-                                                            P->clk.p + 2);
-         unsigned long long h[4];
+         borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, part, tk, gtk,
+                                                            S, P->clk.p + 2);
+         unsigned long long h[6];
          PK_HIP(hipMemcpyAsync(h, P->clk.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
          prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
                   nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s);
          prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz, n * s * (j + 4.0) + bsn * s);
+         if (o.poly)
+            prof_add("gmres_poly", 1, (double)(h[5] - h[4]) / P->clock_hz,
+                     nnz * (s + 4.0) + 3.0 * n * s + bsn * s);
       } else {
+         if (o.poly)
+            bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
+                                                         S, 0);
          bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S);
-         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, I.z.p, I.binv.p, j, part, tk, gtk, S);
+         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, part, tk, gtk, S);
       }
    }
    btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
-   bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p);
-   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, I.z.p, I.binv.p, o, part,
+   if (o.poly) {
+      // e += M^-1 u: u and B^-1 u, then the second term, then the sum
+      bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
+      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1);
+      badd_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, I.z.p, S, I.e.p);
+   } else {
+      bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p, 1, nullptr, nullptr);
+   }
+   brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, zfirst, I.binv.p, o, part,
                                                    tk, gtk, S);
    bfinish_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, P->phi.p, I.e.p, S, P->x.p);
    bphase_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
@@ -1140,7 +1236,7 @@ static int run_tick(pk_batch_s* P, long units) {
       if (!P->clk.p) {
          int khz = 0;
          PK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device));
-         PK_REQUIRE(khz > 0 && !P->clk.alloc(4), "no device clock for profiling");
+         PK_REQUIRE(khz > 0 && !P->clk.alloc(6), "no device clock for profiling");
          P->clock_hz = khz * 1e3;
       }
       return launch_tick<T>(P, units, true);
@@ -1351,6 +1447,13 @@ int pk_batch_set_inner(pk_batch P, int restart, double reduction, int max_inner)
    return 0;
 }
 
+int pk_batch_set_preconditioner(pk_batch P, int poly) {
+   PK_REQUIRE(P && (poly == 0 || poly == 1), "wrong preconditioner");
+   P->opts.poly = poly;
+   P->graph.clear();
+   return 0;
+}
+
 int pk_batch_set_shift(pk_batch P, double delta) {
    PK_REQUIRE(P && delta >= 0.0, "wrong shift");
    P->opts.delta = delta;
@@ -1455,7 +1558,7 @@ int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction,
    PK_REQUIRE(P && tol > 0.0 && max_outer > 0 && reduction > 0.0 && reduction < 1.0 && max_inner > 0 &&
                   delta >= 0.0,
               "wrong pool options");
-   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner};
+   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner, P->opts.poly};
    P->graph.clear();
    return 0;
 }

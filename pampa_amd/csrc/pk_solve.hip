@@ -498,6 +498,11 @@ int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner) {
    return pk_batch_set_inner(E->B, restart, reduction, max_inner);
 }
 
+int pk_eig_set_preconditioner(pk_eig E, int poly) {
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_preconditioner(E->B, poly);
+}
+
 int pk_eig_set_shift(pk_eig E, double delta) {
    PK_REQUIRE(E, "null eigen solver");
    return pk_batch_set_shift(E->B, delta);
