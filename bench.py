@@ -90,7 +90,10 @@ def roofline_pass(pa, d, solves=2):
                 out[name] = {"launches": n, "avg_us": sec / max(n, 1) * 1e6,
                              "bytes_per_launch": byt / max(n, 1), "GBs": byt / sec / 1e9 if sec else 0.0}
             pa.profile(False)
-        return {"roofline_pass": out, "solves": solves}
+        n = 1 << 26  # 256 MB per buffer: the calibration kernel of the traffic counters
+        sec = pa.stream_copy(n, 20)
+        return {"roofline_pass": out, "solves": solves,
+                "stream_copy": {"bytes": 8.0 * n, "us": sec * 1e6, "GBs": 8.0 * n / sec / 1e9}}
     finally:
         os.chdir(cwd)
 

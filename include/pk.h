@@ -49,6 +49,10 @@ int pk_finalize(void);
 int pk_synchronize(void);
 const char* pk_last_error(void);
 int pk_device_memory(double* free_bytes, double* total_bytes); /* hipMemGetInfo of the device */
+/* STREAM-style copy of n floats (16-byte accesses): average seconds per launch
+ * over reps launches; moves 8 n bytes.  Context for the HBM figures and the
+ * calibration kernel of the hardware traffic counters. */
+int pk_stream_copy(long n, int reps, double* avg_seconds);
 
 /* ---- vectors (petsc::create(Vec&), destroy, set, copy; petsc.cxx:113-140,
  *      236-345) ------------------------------------------------------------ */

@@ -125,6 +125,8 @@ def _declare(L):
         f = getattr(L, name)
         f.argtypes = args
         f.restype = _I
+    L.pk_stream_copy.argtypes = [_L, _I, _P]
+    L.pk_stream_copy.restype = _I
     L.pk_pool_bytes.argtypes = [_L, _L, _L, _L, _I, _I, _I]
     L.pk_pool_bytes.restype = _D
     L.pk_last_error.restype = ct.c_char_p
@@ -179,6 +181,13 @@ def profile(on: bool):
     check(lib().pk_profile_enable(int(on)))
     if on:
         check(lib().pk_profile_reset())
+
+
+def stream_copy(n_floats: int, reps: int = 20) -> float:
+    """Average seconds of one STREAM-style copy of n_floats floats (8 n bytes)."""
+    t = ct.c_double()
+    check(lib().pk_stream_copy(n_floats, reps, ct.byref(t)))
+    return t.value
 
 
 def profile_get(name: str):

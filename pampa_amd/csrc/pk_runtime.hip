@@ -184,6 +184,44 @@ int pk_initialize(int device) {
    return 0;
 }
 
+}  // extern "C"
+
+namespace pk {
+// y = x over n floats, 4 per thread in one 16-byte access each
+__global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restrict__ x, float4* __restrict__ y,
+                                                          long n4) {
+   long i = blockIdx.x * 256L + threadIdx.x;
+   if (i < n4) y[i] = x[i];
+}
+}  // namespace pk
+
+extern "C" {
+
+int pk_stream_copy(long n, int reps, double* avg_seconds) {
+   PK_CALL(ensure_init());
+   PK_REQUIRE(n > 0 && n % 4 == 0 && reps > 0 && avg_seconds, "wrong arguments");
+   Ctx& c = ctx();
+   DBuf<float> x, y;
+   PK_REQUIRE(!x.alloc(n) && !y.alloc(n), "device allocation failed");
+   PK_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * n, c.stream));
+   const long n4 = n / 4;
+   hipEvent_t e0, e1;
+   PK_HIP(hipEventCreate(&e0));
+   PK_HIP(hipEventCreate(&e1));
+   stream_copy_kernel<<<grid_for(n4, 256), 256, 0, c.stream>>>((const float4*)x.p, (float4*)y.p, n4);
+   PK_HIP(hipEventRecord(e0, c.stream));
+   for (int r = 0; r < reps; r++)
+      stream_copy_kernel<<<grid_for(n4, 256), 256, 0, c.stream>>>((const float4*)x.p, (float4*)y.p, n4);
+   PK_HIP(hipEventRecord(e1, c.stream));
+   PK_HIP(hipEventSynchronize(e1));
+   float ms = 0.f;
+   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   *avg_seconds = ms * 1e-3 / reps;
+   return 0;
+}
+
 int pk_device_memory(double* free_bytes, double* total_bytes) {
    PK_CALL(ensure_init());
    size_t f = 0, t = 0;
