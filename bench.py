@@ -26,6 +26,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per gmres_spmv_dot launch on the quarter-core workload: two rocprofv3 --pmc
+# passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --roofline-pass`, each scaled by
+# what the same counter reads for stream_copy_kernel moving 2^28 bytes each way
+# (fetch x2.000, write x1.000): 3406.1 KiB x 2.000 + 276.2 KiB = 7.258 MB.
+SPMV_DOT_TRAFFIC_BYTES = 7.258e6
 METRIC = ("eigen solves/s + p50/p99 latency, mixed vs FP64 at |dk|<=1pcm; sweep/SpMV HBM GB/s")
 
 
@@ -367,7 +372,8 @@ def main():
     avg_b = dv["bytes"] / dv["launches"]
     achieved = avg_b / avg_s / 1e9
     roofline = {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": SPMV_DOT_TRAFFIC_BYTES if dname == "gmres_spmv_dot" else None,
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": avg_b,
                 "launches_per_solve": dv["launches"],
                 "note": "latency-bound: the 170x170 operator (4 MB) is cache resident"}
