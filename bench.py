@@ -307,6 +307,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched run")
     ap.add_argument("--spmv-batched-only", action="store_true",
                     help="only the SpMV over every instance of the batched family (FP32 and FP64)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only the S_N sweep timing of the hexagonal S6 lattice (FP32 and FP64)")
     ap.add_argument("--roofline-pass", action="store_true",
                     help="only the profiled solves of the headline workload (for a kernel trace whose "
                          "launches are exactly the ones the roofline figure averages)")
@@ -324,6 +326,10 @@ def main():
     log(f"[bench] workload quarter core 170x170 G=2 at {d1}")
     if args.roofline_pass:
         print(json.dumps(roofline_pass(pa, d1)))
+        return
+    if args.sweep_only:
+        d2 = W.hex_sn_3d(os.path.join(tmp, "hex-s6"))
+        print(json.dumps(sweep_numbers(pa, d2)))
         return
     if args.spmv_batched_only:
         paths, rows = _family(W, os.path.join(tmp, "family"), args.batch)

@@ -80,6 +80,16 @@ void prof_add(const char* name, long launches, double seconds, double bytes);
          kernel<<<(grid), (block), 0, (stream)>>>(__VA_ARGS__);                                         \
    } while (0)
 
+// the same with `shmem` bytes of dynamic local memory
+#define PK_LAUNCH_SHM(name, bytes, kernel, grid, block, shmem, stream, ...)                                   \
+   do {                                                                                                       \
+      hipEvent_t e0_ = nullptr, e1_ = nullptr;                                                                \
+      if (::pk::prof_events((name), (bytes), &e0_, &e1_))                                                     \
+         hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shmem), (stream), e0_, e1_, 0, __VA_ARGS__); \
+      else                                                                                                    \
+         kernel<<<(grid), (block), (shmem), (stream)>>>(__VA_ARGS__);                                         \
+   } while (0)
+
 // Grow the reduction scratch to at least `count` doubles (setup time only).
 int reserve_partials(size_t count);
 

@@ -12,12 +12,15 @@
 // moved to the right-hand side).
 //
 // Parallel structure: each direction has its own upwind level schedule (cells
-// whose upwind neighbours all sit in earlier levels).  Global step s sweeps level
-// s of every direction at once, so one launch covers sum_m |level_s(m)| cells x
-// all groups; a thread owns one (direction, cell) pair and loops over groups,
-// reading the cell's faces once.  The device layout of psi is [m][cell][group]
-// (the G values of an upwind neighbour are contiguous); the reference layout
-// [cell][group][m] is produced only at the boundary of the API.
+// whose upwind neighbours all sit in earlier levels), and directions do not
+// depend on each other within a sweep.  One workgroup sweeps one direction:
+// it walks that direction's levels in order, a workgroup barrier between
+// levels, a thread per (direction, cell) item looping over the groups.  The
+// whole sweep is one launch with no inter-workgroup synchronisation (the
+// level-by-level launches of earlier designs paid a kernel boundary per level).
+// The device layout of psi is [m][cell][group] (the G values of an upwind
+// neighbour are contiguous); the reference layout [cell][group][m] is produced
+// only at the boundary of the API.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -41,9 +44,12 @@ struct SnGeom {
    const double* dirs;
 };
 
+// pos[m C + i]: the item's position in direction-major level order, which is
+// also where its G values sit in psi
 template <class T>
 __global__ void __launch_bounds__(256) sn_sweep_step_kernel(const int* __restrict__ items, long nitems,
                                                             int C, int G, SnGeom geo,
+                                                            const int* __restrict__ pos,
                                                             const T* __restrict__ sigt,
                                                             const T* __restrict__ q, T* psi) {
 #pragma clang fp contract(off)
@@ -66,39 +72,46 @@ __global__ void __launch_bounds__(256) sn_sweep_step_kernel(const int* __restric
          lhs0 += (T)geo.coef[f] * w * A;
       } else {
          const T c = (T)geo.coef[f] * w * A;
-         const T* up = psi + ((long)m * C + i2) * G;
+         const T* up = psi + (long)pos[m * C + i2] * G;
          for (int g = 0; g < G; g++) inflow[g] += c * up[g];
       }
    }
    const int mt = geo.mat[i];
    const T V = (T)geo.vol[i];
-   T* out = psi + ((long)m * C + i) * G;
+   T* out = psi + (long)pos[code] * G;
    const T* qi = q + (long)i * G;
    for (int g = 0; g < G; g++) out[g] = (qi[g] - inflow[g]) / (sigt[mt * G + g] * V + lhs0);
 }
 
-// Per-item records: what a sweep step needs of (direction m, cell i), taken
-// from the geometry once per precision in the step kernel's own arithmetic
-// (same operations in the same order, so the same bits): the flat index m C + i,
-// the outgoing part of the left-hand side lhs0 = sum_out coef w A + sum_vac,out w A,
-// and up to kMaxIn incoming interior faces as (flat upwind index, coef w A) in
-// face order.  Stored structure-of-arrays in step order, so a step reads its
-// records with coalesced loads and has one dependent gather (the upwind values)
-// left, instead of the chain cell -> faces -> normals -> neighbours -> values.
+// Per-item records: what a sweep needs of (direction m, cell i), taken from the
+// geometry and the cross sections once per precision in the step kernel's own
+// arithmetic (same operations in the same order, so the same bits): the
+// denominators den_g = Sigma_t,g V + lhs0 with lhs0 =
+// sum_out coef w A + sum_vac,out w A, and up to kMaxIn incoming interior faces
+// as (source of the upwind value, coef w A) in face order.  An upwind value is
+// read from a slot of the workgroup's local memory when its producer was given
+// one, else from psi in global memory: a source s >= 0 is slot s, s < 0 is the
+// item at position -(s + 1).  nb packs the incoming count (3 bits) and 1 + the item's
+// own output slot (0 = none).  Faces whose weight rounds to zero in T (coef w A
+// = 0) are left out.  Unused incoming entries hold (slot 0, weight 0).  Stored
+// structure-of-arrays in direction-major level order.  The records hold
+// Sigma_t: they are built from the cross sections the sweep was prepared with.
 constexpr int kMaxIn = 4;
 
 template <class T>
 struct SnRec {
    long n;
-   int* code;
-   int* nin;
-   T* lhs;
+   int* nb;
+   T* den;   // [G][n]
    int* up;  // [kMaxIn][n]
    T* cw;    // [kMaxIn][n]
 };
 
+// slot_of[m C + i]: local-memory slot of item (m, i)'s value, -1 if none
 template <class T>
-__global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, SnGeom geo, SnRec<T> R, int* overflow) {
+__global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, int G, SnGeom geo,
+                              const T* __restrict__ sigt, const int* __restrict__ slot_of,
+                              const int* __restrict__ pos, SnRec<T> R, int* overflow) {
 #pragma clang fp contract(off)
    long it = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (it >= n) return;
@@ -116,9 +129,11 @@ __global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, SnGe
          if (w > (T)0) lhs0 += w * A;
       } else if (w > (T)0) {
          lhs0 += (T)geo.coef[f] * w * A;
-      } else {
+      } else if (w < (T)0) {
          if (k < kMaxIn) {
-            R.up[(long)k * n + it] = m * C + i2;
+            const int up = m * C + i2;
+            const int sl = slot_of[up];
+            R.up[(long)k * n + it] = sl >= 0 ? sl : -(pos[up] + 1);
             R.cw[(long)k * n + it] = (T)geo.coef[f] * w * A;
          } else {
             *overflow = 1;
@@ -126,52 +141,256 @@ __global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, SnGe
          k++;
       }
    }
-   R.code[it] = code;
-   R.nin[it] = k < kMaxIn ? k : kMaxIn;
-   R.lhs[it] = lhs0;
-}
-
-// one step from the records: items a .. a+count-1
-template <class T>
-__global__ void __launch_bounds__(256) sn_sweep_rec_kernel(long a, long count, int C, int G, SnRec<T> R,
-                                                           const int* __restrict__ mat,
-                                                           const double* __restrict__ vol,
-                                                           const T* __restrict__ sigt, const T* __restrict__ q,
-                                                           T* psi) {
-#pragma clang fp contract(off)
-   const long t = blockIdx.x * 256L + threadIdx.x;
-   if (t >= count) return;
-   const long it = a + t;
-   const int code = R.code[it];
-   const int nin = R.nin[it];
-   const T lhs0 = R.lhs[it];
-   const int i = code % C;
-   T inflow[kMaxSnGroups];
-   for (int g = 0; g < G; g++) inflow[g] = 0;
-   for (int k = 0; k < nin; k++) {
-      const T c = R.cw[(long)k * R.n + it];
-      const T* up = psi + (long)R.up[(long)k * R.n + it] * G;
-      for (int g = 0; g < G; g++) inflow[g] += c * up[g];
+   for (int j = k; j < kMaxIn; j++) {
+      R.up[(long)j * n + it] = 0;
+      R.cw[(long)j * n + it] = 0;
    }
-   const int mt = mat[i];
-   const T V = (T)vol[i];
-   T* out = psi + (long)code * G;
-   const T* qi = q + (long)i * G;
-   for (int g = 0; g < G; g++) out[g] = (qi[g] - inflow[g]) / (sigt[mt * G + g] * V + lhs0);
+   const int mt = geo.mat[i];
+   const T V = (T)geo.vol[i];
+   for (int g = 0; g < G; g++) R.den[(long)g * n + it] = sigt[mt * G + g] * V + lhs0;
+   R.nb[it] = (k < kMaxIn ? k : kMaxIn) | ((slot_of[code] + 1) << 3);
 }
 
-// phiw[i][g] = sum_m w_m psi[m][i][g] (FP64 accumulation)
+// One whole sweep, one workgroup per direction.  The records of direction m are
+// stored level by level (lptr[m][l] .. lptr[m][l+1]); the workgroup walks its
+// levels in order with a workgroup barrier between them.  Upwind values come
+// from slots of the workgroup's local memory, assigned on the host so that a
+// slot lives from its producer's level to its last consumer's level.
+// Directions never read each other's values: no workgroup waits on another.
+//
+// Latency: everything a level loads from global memory (its records and its
+// source) is loaded D levels ahead, while earlier levels compute; within a
+// level only local memory is read.  Each thread owns P items of a level (P NT
+// >= the widest level; all loads issued together), and every upwind value comes
+// from a slot (the host checks both, else the kernel below runs).  Per item the
+// arithmetic is that of the step kernel (same operations, same order;
+// zero-weight faces add exactly zero there).
+template <class T, int GT>
+struct SnItem {
+   int nb;
+   T den[GT];
+   int up[kMaxIn];
+   T cw[kMaxIn];
+   T qv[GT];  // the item's source
+};
+
+template <class T, int GT>
+__device__ __forceinline__ void sn_load_item(const SnRec<T>& R, const T* __restrict__ qp, int G, int it, int g0,
+                                             SnItem<T, GT>& r) {
+   r.nb = R.nb[it];
+#pragma unroll
+   for (int g = 0; g < GT; g++) r.qv[g] = qp[(long)it * G + g0 + g];
+#pragma unroll
+   for (int g = 0; g < GT; g++) r.den[g] = R.den[(long)(g0 + g) * R.n + it];
+#pragma unroll
+   for (int k = 0; k < kMaxIn; k++) {
+      r.up[k] = R.up[(long)k * R.n + it];
+      r.cw[k] = R.cw[(long)k * R.n + it];
+   }
+}
+
+// one level l: prefetch of level l + D (records and source) into `pre`, the
+// arithmetic of `cur` (level l), the stores, the barrier.  Groups
+// g0 .. g0+GT-1 of G.  Item slot j of a thread covers position a + tid + j NT;
+// lanes past the level's end repeat the level's last item (clamped index: a
+// wave past the end touches one cache line) and store exactly its values
+// again, so that every load and store is unconditional and the waits stay
+// precise.  The source qp and psi are in level order, so both are read and
+// written with contiguous accesses.
+template <class T, int GT, int NT, int P, int D>
+__device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l, int nl, int G, int g0, int tid,
+                                         const SnItem<T, GT> (&cur)[P], SnItem<T, GT> (&pre)[P],
+                                         const T* __restrict__ qp, T* psi, T* slots) {
+#pragma clang fp contract(off)
+   const int a = lp[l], e = lp[l + 1];
+   {
+      const bool more = l + D < nl;
+      const int ap = more ? lp[l + D] : a, ep = more ? lp[l + D + 1] : a + 1;
+#pragma unroll
+      for (int j = 0; j < P; j++) sn_load_item<T, GT>(R, qp, G, min(ap + tid + j * NT, ep - 1), g0, pre[j]);
+   }
+   __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+   for (int j = 0; j < P; j++) {
+      const SnItem<T, GT>& r = cur[j];
+      const int nin = r.nb & 7;
+      T inflow[GT];
+#pragma unroll
+      for (int g = 0; g < GT; g++) inflow[g] = 0;
+#pragma unroll
+      for (int k = 0; k < kMaxIn; k++) {
+         T v[GT];
+#pragma unroll
+         for (int g = 0; g < GT; g++) v[g] = slots[r.up[k] * GT + g];
+#pragma unroll
+         for (int g = 0; g < GT; g++) {
+            const T t = inflow[g] + r.cw[k] * v[g];
+            inflow[g] = k < nin ? t : inflow[g];
+         }
+      }
+      T out[GT];
+#pragma unroll
+      for (int g = 0; g < GT; g++) out[g] = (r.qv[g] - inflow[g]) / r.den[g];
+      const long it = min(a + tid + j * NT, e - 1);
+      T* o = psi + it * G + g0;
+#pragma unroll
+      for (int g = 0; g < GT; g++) o[g] = out[g];
+      const int os = (r.nb >> 3) - 1;
+      if (os >= 0) {
+#pragma unroll
+         for (int g = 0; g < GT; g++) slots[os * GT + g] = out[g];
+      }
+   }
+   __syncthreads();
+}
+
+// Workgroup b sweeps direction m for groups g0 .. g0+GT-1, G = S GT.  When M is
+// a multiple of 8 the S workgroups of a direction get ids of one residue mod 8
+// (the residue picks the chiplet a workgroup is dispatched to), so that they
+// share the direction's records through one L2.  D = 1: the records of level
+// l+1 are loaded during level l (two record sets used in turn); D = 2: those of
+// level l+2 (three sets), which hides a longer load latency at a higher
+// register cost.  The sets rotate with the level loop unrolled D + 1 times, so
+// that no register copy forces an early wait.
+template <class T, int GT, int NT, int P, int D>
+__global__ void __launch_bounds__(NT) sn_sweep_lds_kernel(int G, int M, int lstride, const int* __restrict__ lptr,
+                                                          const int* __restrict__ nlev, SnRec<T> R,
+                                                          const T* __restrict__ qp, T* psi) {
+   static_assert(D == 1 || D == 2, "prefetch distance 1 or 2");
+   extern __shared__ __align__(16) unsigned char sn_smem[];
+   T* slots = reinterpret_cast<T*>(sn_smem);
+   const int S = G / GT;
+   const int b = blockIdx.x;
+   int m, sidx;
+   if (M % 8 == 0) {
+      const int r = b >> 3;
+      sidx = r % S;
+      m = (b & 7) + 8 * (r / S);
+   } else {
+      m = b / S;
+      sidx = b - m * S;
+   }
+   const int g0 = sidx * GT;
+   const int* lp = lptr + (long)m * lstride;
+   const int nl = nlev[m];
+   const int tid = threadIdx.x;
+   SnItem<T, GT> X0[P], X1[P], X2[D == 2 ? P : 1];
+   {
+      const int a0 = lp[0], e0 = lp[1];
+#pragma unroll
+      for (int j = 0; j < P; j++) sn_load_item<T, GT>(R, qp, G, min(a0 + tid + j * NT, e0 - 1), g0, X0[j]);
+      if constexpr (D == 2) {
+         const bool more = nl > 1;
+         const int a1 = more ? lp[1] : a0, e1 = more ? lp[2] : a0 + 1;
+#pragma unroll
+         for (int j = 0; j < P; j++) sn_load_item<T, GT>(R, qp, G, min(a1 + tid + j * NT, e1 - 1), g0, X1[j]);
+      }
+   }
+   int l = 0;
+   if constexpr (D == 1) {
+      for (; l + 1 < nl; l += 2) {
+         sn_level<T, GT, NT, P, 1>(R, lp, l, nl, G, g0, tid, X0, X1, qp, psi, slots);
+         sn_level<T, GT, NT, P, 1>(R, lp, l + 1, nl, G, g0, tid, X1, X0, qp, psi, slots);
+      }
+      if (l < nl) sn_level<T, GT, NT, P, 1>(R, lp, l, nl, G, g0, tid, X0, X1, qp, psi, slots);
+   } else {
+      for (; l + 2 < nl; l += 3) {
+         sn_level<T, GT, NT, P, 2>(R, lp, l, nl, G, g0, tid, X0, X2, qp, psi, slots);
+         sn_level<T, GT, NT, P, 2>(R, lp, l + 1, nl, G, g0, tid, X1, X0, qp, psi, slots);
+         sn_level<T, GT, NT, P, 2>(R, lp, l + 2, nl, G, g0, tid, X2, X1, qp, psi, slots);
+      }
+      if (l < nl) sn_level<T, GT, NT, P, 2>(R, lp, l, nl, G, g0, tid, X0, X2, qp, psi, slots);
+      if (l + 1 < nl) sn_level<T, GT, NT, P, 2>(R, lp, l + 1, nl, G, g0, tid, X1, X0, qp, psi, slots);
+   }
+}
+
+// Any configuration: the same sweep, one item at a time, upwind values from
+// the slots or psi (level-ordered qp and psi as above).
+template <class T, int NT>
+__global__ void __launch_bounds__(NT) sn_sweep_lds_any_kernel(int G, int lstride, const int* __restrict__ lptr,
+                                                              const int* __restrict__ nlev, SnRec<T> R,
+                                                              const T* __restrict__ qp, T* psi) {
+#pragma clang fp contract(off)
+   extern __shared__ __align__(16) unsigned char sn_smem[];
+   T* slots = reinterpret_cast<T*>(sn_smem);
+   const int m = blockIdx.x;
+   const int* lp = lptr + (long)m * lstride;
+   const int nl = nlev[m];
+   for (int l = 0; l < nl; l++) {
+      const int a = lp[l], e = lp[l + 1];
+      for (int it = a + (int)threadIdx.x; it < e; it += NT) {
+         const int nb = R.nb[it];
+         const int nin = nb & 7, os = (nb >> 3) - 1;
+         T inflow[kMaxSnGroups];
+         for (int g = 0; g < G; g++) inflow[g] = 0;
+         for (int k = 0; k < nin; k++) {
+            const int s = R.up[(long)k * R.n + it];
+            const T c = R.cw[(long)k * R.n + it];
+            const T* up = s >= 0 ? slots + (long)s * G : psi + (long)(-(s + 1)) * G;
+            for (int g = 0; g < G; g++) inflow[g] += c * up[g];
+         }
+         T* o = psi + (long)it * G;
+         for (int g = 0; g < G; g++) {
+            const T v = (qp[(long)it * G + g] - inflow[g]) / R.den[(long)g * R.n + it];
+            o[g] = v;
+            if (os >= 0) slots[(long)os * G + g] = v;
+         }
+      }
+      __syncthreads();
+   }
+}
+
+// qp[p][.] = q[i][.] for the item (m, i) at position p: the source in the
+// sweep's level order.  An item's G values are copied as cpi chunks of type V
+// (16, 8 or 4 bytes: the widest that divides G sizeof(T)), one thread a chunk.
+template <class V>
+__global__ void __launch_bounds__(kBlock) sn_permute_kernel(const int* __restrict__ items, long n, int C, int cpi,
+                                                            const V* __restrict__ q, V* __restrict__ qp) {
+   const long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (l >= n * cpi) return;
+   const long it = l / cpi;
+   const int c = (int)(l - it * cpi);
+   const int i = items[it] % C;
+   qp[l] = q[(long)i * cpi + c];
+}
+
+// phiw[i][g] = sum_m w_m psi(m, i, g) (FP64 accumulation, m ascending); the
+// values of item (m, i) sit at position pos[m C + i] of psi.  One thread per
+// cell: the positions are read coalesced, each item's G values as one gather.
+template <class T, int GT>
+__global__ void __launch_bounds__(kBlock) sn_moment_kernel(const T* __restrict__ psi, const int* __restrict__ pos,
+                                                           const double* __restrict__ wts, int M, int C, int G_,
+                                                           double* phiw) {
+   const int G = GT > 0 ? GT : G_;
+   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i >= C) return;
+   constexpr int GM = GT > 0 ? GT : kMaxSnGroups;
+   double s[GM];
+#pragma unroll
+   for (int g = 0; g < GM; g++) s[g] = 0.0;
+   for (int m = 0; m < M; m++) {
+      const T* v = psi + (long)pos[(long)m * C + i] * G;
+      const double w = wts[m];
+#pragma unroll
+      for (int g = 0; g < GM; g++)
+         if (GT > 0 || g < G) s[g] += w * (double)v[g];
+   }
+#pragma unroll
+   for (int g = 0; g < GM; g++)
+      if (GT > 0 || g < G) phiw[i * G + g] = s[g];
+}
+
 template <class T>
-__global__ void sn_moment_kernel(const T* __restrict__ psi, const double* __restrict__ wts, int M,
-                                 long CG, double* phiw) {
-   long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (l >= CG) return;
-   double s = 0.0;
-   for (int m = 0; m < M; m++) s += wts[m] * (double)psi[(long)m * CG + l];
-   phiw[l] = s;
+static void sn_moment(const T* psi, const int* pos, const double* wts, int M, int C, int G, double* phiw,
+                      hipStream_t st) {
+   switch (G) {
+      case 1: sn_moment_kernel<T, 1><<<grid_for(C), kBlock, 0, st>>>(psi, pos, wts, M, C, G, phiw); break;
+      case 2: sn_moment_kernel<T, 2><<<grid_for(C), kBlock, 0, st>>>(psi, pos, wts, M, C, G, phiw); break;
+      case 4: sn_moment_kernel<T, 4><<<grid_for(C), kBlock, 0, st>>>(psi, pos, wts, M, C, G, phiw); break;
+      default: sn_moment_kernel<T, 0><<<grid_for(C), kBlock, 0, st>>>(psi, pos, wts, M, C, G, phiw); break;
+   }
 }
 
-// q = V (sum_g2 Sigma_s(g2->g) phiw(g2) + chi_g/k sum_g2 nuSf(g2) phiw(g2))
 template <class T>
 __global__ void sn_source_kernel(int C, int G, const double* vol, const int* mat, const double* sigs,
                                  const double* nusf, const double* chi, const double* phiw,
@@ -224,13 +443,16 @@ __global__ void __launch_bounds__(kBlock) sn_update_kernel(long CG, double* phiw
 
 // psi_ref[i][g][m] = scale * psi[m][i][g]
 template <class T>
-__global__ void sn_to_reference_kernel(const T* psi, int C, int G, int M, double scale, double* out) {
+__global__ void sn_to_reference_kernel(const T* psi, const int* __restrict__ pos, int C, int G, int M, double scale,
+                                       double* out) {
    long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
    long n = (long)C * G * M;
    if (l >= n) return;
    int m = (int)(l % M);
    long ig = l / M;  // i*G + g
-   out[l] = scale * (double)psi[(long)m * C * G + ig];
+   const long i = ig / G;
+   const int g = (int)(ig - i * G);
+   out[l] = scale * (double)psi[(long)pos[(long)m * C + i] * G + g];
 }
 
 template <class T>
@@ -255,21 +477,31 @@ struct pk_sn_s {
    std::vector<long> step_ptr;    // nsteps + 1
    int nsteps = 0;
    DBuf<int> items;
-   // per-item records (one set per precision, built on first use) and the
-   // captured step sequences; geometry = true sweeps from the geometry instead
+   // per-item records in direction-major level order (one set per precision,
+   // built on first use); geometry = true sweeps from the geometry instead
    bool geometry = false;
-   DBuf<int> rec_code, rec_nin, rec_up, rec_err;
-   DBuf<double> rec_lhs64, rec_cw64;
-   DBuf<float> rec_lhs32, rec_cw32;
-   hipGraphExec_t graph32 = nullptr, graph64 = nullptr;
-   const void* graph_ptrs[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-   ~pk_sn_s() {
-      if (graph32) (void)hipGraphExecDestroy(graph32);
-      if (graph64) (void)hipGraphExecDestroy(graph64);
-   }
+   DBuf<int> ditems, dlptr, dnlev;  // [M C], [M][lstride] (+2 padding), [M]
+   DBuf<int> dpos;                  // [M C]: position of item (m, i) in ditems and psi
+   int lstride = 0;
+   std::vector<int> h_ditems, h_dlptr;
+   std::vector<double> h_dirs;
+   // records of one precision; slots = local-memory value slots they use
+   struct RecBufs {
+      DBuf<int> nb, up;
+      DBuf<double> den64, cw64;
+      DBuf<float> den32, cw32;
+      int slots = 0;
+      int gt = 0;             // groups per workgroup the slots are sized for
+      int P = 0, D = 0;       // items per thread and prefetch distance of that kernel
+      bool complete = false;  // every value that is read has a slot
+      bool fast = false;      // the per-direction kernel applies
+      bool built = false;
+   } rec32, rec64;
+   int maxw = 0;  // widest level over all directions
+   DBuf<int> rec_err;
    DBuf<double> dirs, wts, coef;
-   DBuf<double> psi64, q64, sigt64;
-   DBuf<float> psi32, q32, sigt32;
+   DBuf<double> psi64, q64, sigt64, qp64;  // qp: the source in level order
+   DBuf<float> psi32, q32, sigt32, qp32;
    DBuf<double> phiw, old, st;
    // value accesses of one sweep: per (i,g,m) one psi write plus one upwind psi
    // read per incoming interior face, per (i,g) one source read and one moment write
@@ -293,55 +525,202 @@ template <class T>
 static SnRec<T> recs(pk_sn_s* S);
 template <>
 SnRec<double> recs<double>(pk_sn_s* S) {
-   const long n = S->step_ptr[S->nsteps];
-   return SnRec<double>{n, S->rec_code.p, S->rec_nin.p, S->rec_lhs64.p, S->rec_up.p, S->rec_cw64.p};
+   auto& B = S->rec64;
+   return SnRec<double>{(long)S->M * S->C, B.nb.p, B.den64.p, B.up.p, B.cw64.p};
 }
 template <>
 SnRec<float> recs<float>(pk_sn_s* S) {
-   const long n = S->step_ptr[S->nsteps];
-   return SnRec<float>{n, S->rec_code.p, S->rec_nin.p, S->rec_lhs32.p, S->rec_up.p, S->rec_cw32.p};
+   auto& B = S->rec32;
+   return SnRec<float>{(long)S->M * S->C, B.nb.p, B.den32.p, B.up.p, B.cw32.p};
 }
 
-// records of precision T, built once
+// Local-memory bytes a workgroup may use (the device's per-workgroup limit).
+static int sn_lds_limit() {
+   static int lim = -1;
+   if (lim < 0) {
+      int v = 0;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx().device) != hipSuccess) v = 0;
+      lim = v > 0 ? v : 65536;
+   }
+   return lim;
+}
+
+// Value slots: item (m, i) gets a slot when some later item reads its value
+// (a downwind neighbour: w < 0 seen from the neighbour, the schedule's own
+// predicate) and a slot is free; the slot is held from the item's level to the
+// level of its last reader and is reused from the level after.  Levels run in
+// order inside a workgroup, so a slot is never written while a reader of its
+// previous value can still run.  Returns the peak number of slots in use;
+// complete = every value with a reader got a slot.
+static int sn_assign_slots(const pk_sn_s* S, int cap, std::vector<int>& slot_of, bool& complete) {
+   const int C = S->C, M = S->M;
+   const pk_mesh_s* mesh = S->mesh;
+   const auto& fptr = mesh->h_fptr;
+   const auto& fnb = mesh->h_fnb;
+   const auto& fn = mesh->h_fnorm;
+   slot_of.assign((size_t)M * C, -1);
+   std::vector<int> last(C), fr;
+   std::vector<std::vector<int>> rel;
+   int peak = 0;
+   complete = true;
+   for (int m = 0; m < M; m++) {
+      const double* om = &S->h_dirs[3L * m];
+      const int* lev = &S->level[(size_t)m * C];
+      std::fill(last.begin(), last.end(), -1);
+      for (int i2 = 0; i2 < C; i2++)
+         for (int f = fptr[i2]; f < fptr[i2 + 1]; f++) {
+            const int i = fnb[f];
+            if (i < 0) continue;
+            const double w = om[0] * fn[3L * f] + om[1] * fn[3L * f + 1] + om[2] * fn[3L * f + 2];
+            if (w < 0.0 && lev[i2] > last[i]) last[i] = lev[i2];
+         }
+      fr.clear();
+      for (int sl = cap - 1; sl >= 0; sl--) fr.push_back(sl);
+      const int nl = S->nlev[m];
+      rel.assign(nl + 1, {});
+      const int* lp = &S->h_dlptr[(size_t)m * S->lstride];
+      for (int l = 0; l < nl; l++) {
+         for (int sl : rel[l]) fr.push_back(sl);
+         for (int k = lp[l]; k < lp[l + 1]; k++) {
+            const int i = S->h_ditems[k] - m * C;
+            if (last[i] < 0) continue;
+            if (fr.empty()) {
+               complete = false;
+               continue;
+            }
+            const int sl = fr.back();
+            fr.pop_back();
+            slot_of[(size_t)m * C + i] = sl;
+            rel[last[i] + 1].push_back(sl);
+            peak = std::max(peak, cap - (int)fr.size());
+         }
+      }
+   }
+   return peak;
+}
+
+// Configuration of the per-direction kernel: GT groups per workgroup (G / GT
+// workgroups per direction), P items per thread (P 1024 >= the widest level)
+// and prefetch distance D, within the register budget of 1024-thread
+// workgroups (128 VGPRs; the table lists the largest P per (T, GT, D)).  GT is
+// the smallest divisor of G in {1, 2, 4} whose M G / GT workgroups fit on the
+// compute units at once (one per unit), else the largest that fits in
+// registers; D = 2 where the registers allow it.  False if no configuration
+// applies (then the one-item-at-a-time kernel).
+static int sn_pmax(bool f32, int gt, int D) {
+   if (f32) return D == 1 ? 3 : (gt == 1 ? 3 : gt == 2 ? 2 : 1);
+   return D == 1 ? (gt == 4 ? 1 : gt == 2 ? 2 : 3) : (gt == 1 ? 2 : gt == 2 ? 1 : 0);
+}
+
+template <class T>
+static bool sn_plan(const pk_sn_s* S, int& gt, int& P, int& D) {
+   constexpr bool f32 = sizeof(T) == sizeof(float);
+   P = (S->maxw + 1023) / 1024;
+   if (P > 3) return false;
+   const char* env = getenv("PK_SWEEP_GT");
+   const int force_gt = env ? atoi(env) : 0;
+   env = getenv("PK_SWEEP_D");
+   const int force_d = env ? atoi(env) : 0;
+   const int cus = std::max(1, ctx().num_cus);
+   int best = 0;
+   for (int cand : {1, 2, 4}) {
+      if (S->G % cand || (force_gt && cand != force_gt)) continue;
+      if (P > sn_pmax(f32, cand, 1)) continue;
+      best = cand;
+      if ((long)S->M * (S->G / cand) <= cus) break;
+   }
+   if (!best) return false;
+   gt = best;
+   D = P <= sn_pmax(f32, gt, 2) ? 2 : 1;
+   if (force_d == 1 || force_d == 2) D = (force_d == 2 && P > sn_pmax(f32, gt, 2)) ? 1 : force_d;
+   return true;
+}
+
+// records of precision T (with their value slots), built once
 template <class T>
 static int sn_records(pk_sn_s* S) {
    Ctx& c = ctx();
-   const long n = S->step_ptr[S->nsteps];
+   const long n = (long)S->M * S->C;
    const bool f32 = sizeof(T) == sizeof(float);
-   if (f32 ? S->rec_lhs32.p != nullptr : S->rec_lhs64.p != nullptr) return 0;
-   if (!S->rec_code.p)
-      PK_REQUIRE(!S->rec_code.alloc(n) && !S->rec_nin.alloc(n) && !S->rec_up.alloc((size_t)kMaxIn * n) &&
-                     !S->rec_err.alloc(1),
-                 "device allocation failed");
+   auto& B = f32 ? S->rec32 : S->rec64;
+   if (B.built) return 0;
+   PK_REQUIRE(!B.nb.alloc(n) && !B.up.alloc((size_t)kMaxIn * n), "device allocation failed");
    if (f32)
-      PK_REQUIRE(!S->rec_lhs32.alloc(n) && !S->rec_cw32.alloc((size_t)kMaxIn * n), "device allocation failed");
+      PK_REQUIRE(!B.den32.alloc((size_t)S->G * n) && !B.cw32.alloc((size_t)kMaxIn * n), "device allocation failed");
    else
-      PK_REQUIRE(!S->rec_lhs64.alloc(n) && !S->rec_cw64.alloc((size_t)kMaxIn * n), "device allocation failed");
+      PK_REQUIRE(!B.den64.alloc((size_t)S->G * n) && !B.cw64.alloc((size_t)kMaxIn * n), "device allocation failed");
+   if (!S->rec_err.p) PK_REQUIRE(!S->rec_err.alloc(1), "device allocation failed");
+   // slots sized for the per-direction kernel's groups per workgroup; when some
+   // value gets no slot there, sized for the one-item kernel (all G groups)
+   int gt = S->G, P = 1, D = 1;
+   const bool fast = sn_plan<T>(S, gt, P, D);
+   if (!fast) gt = S->G;
+   auto cap_for = [&](int groups) {
+      return std::max(1, std::min(sn_lds_limit() / (groups * (int)sizeof(T)), (1 << 27)));
+   };
+   std::vector<int> slot_of;
+   B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
+   if (!B.complete && gt != S->G) {
+      gt = S->G;
+      B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
+   }
+   B.gt = gt;
+   B.P = P;
+   B.D = D;
+   B.fast = fast && B.complete && gt <= 4;
+   DBuf<int> dslot;
+   PK_REQUIRE(!dslot.alloc(slot_of.size()), "device allocation failed");
+   PK_HIP(hipMemcpy(dslot.p, slot_of.data(), sizeof(int) * slot_of.size(), hipMemcpyHostToDevice));
    PK_HIP(hipMemsetAsync(S->rec_err.p, 0, sizeof(int), c.stream));
-   sn_rec_kernel<T><<<grid_for(n), kBlock, 0, c.stream>>>(S->items.p, n, S->C, geom(S), recs<T>(S), S->rec_err.p);
+   const T* sigt = f32 ? (const T*)S->sigt32.p : (const T*)S->sigt64.p;
+   sn_rec_kernel<T><<<grid_for(n), kBlock, 0, c.stream>>>(S->ditems.p, n, S->C, S->G, geom(S), sigt, dslot.p,
+                                                        S->dpos.p, recs<T>(S), S->rec_err.p);
    PK_HIP(hipGetLastError());
    int h = 0;
    PK_HIP(hipMemcpyAsync(&h, S->rec_err.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));
    PK_REQUIRE(h == 0, "more incoming faces per cell than the sweep records hold");
+   B.built = true;
+   return 0;
+}
+
+template <class T, int GT, int P, int D>
+static int sn_launch_lds(pk_sn_s* S, const T* qp, T* psi) {
+   Ctx& c = ctx();
+   constexpr int NT = 1024;
+   const auto& B = sizeof(T) == sizeof(float) ? S->rec32 : S->rec64;
+   const size_t shm = (size_t)B.slots * GT * sizeof(T);
+   static bool attr = false;
+   if (!attr) {
+      PK_HIP(hipFuncSetAttribute((const void*)sn_sweep_lds_kernel<T, GT, NT, P, D>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit()));
+      attr = true;
+   }
+   PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_kernel<T, GT, NT, P, D>),
+                 S->M * (S->G / GT), NT, shm, c.stream, S->G, S->M, S->lstride, S->dlptr.p, S->dnlev.p, recs<T>(S),
+                 qp, psi);
    return 0;
 }
 
 template <class T>
-static void sn_launch_steps(pk_sn_s* S, const T* sigt, const T* q, T* psi, bool prof) {
+static int sn_launch_lds_any(pk_sn_s* S, const T* qp, T* psi) {
    Ctx& c = ctx();
-   const SnRec<T> R = recs<T>(S);
-   const double per_item = prof ? sweep_bytes(S, sizeof(T)) / (double)S->step_ptr[S->nsteps] : 0.0;
-   for (int s = 0; s < S->nsteps; s++) {
-      const long a = S->step_ptr[s], n = S->step_ptr[s + 1] - a;
-      PK_LAUNCH("sn_sweep_step", per_item * n, sn_sweep_rec_kernel<T>, grid_for(n, 256), 256, c.stream, a, n,
-                S->C, S->G, R, S->mesh->mat.p, S->mesh->vol.p, sigt, q, psi);
+   constexpr int NT = 1024;
+   const auto& B = sizeof(T) == sizeof(float) ? S->rec32 : S->rec64;
+   const size_t shm = (size_t)B.slots * S->G * sizeof(T);
+   static bool attr = false;
+   if (!attr) {
+      PK_HIP(hipFuncSetAttribute((const void*)sn_sweep_lds_any_kernel<T, NT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit()));
+      attr = true;
    }
+   PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_any_kernel<T, NT>), S->M,
+                 NT, shm, c.stream, S->G, S->lstride, S->dlptr.p, S->dnlev.p, recs<T>(S), qp, psi);
+   return 0;
 }
 
-// One sweep psi = L^-1 q.  The step launches are captured once per precision
-// as a HIP graph (the sweep's buffers are fixed per handle and precision) and
-// relaunched; with the profiler on they are launched one by one.
+// One sweep psi = L^-1 q: one launch of the per-direction kernel (or, with
+// geometry set, one launch per global level step of the geometry kernel).
 template <class T>
 static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
    Ctx& c = ctx();
@@ -351,38 +730,54 @@ static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
       for (int s = 0; s < S->nsteps; s++) {
          long a = S->step_ptr[s], n = S->step_ptr[s + 1] - a;
          PK_LAUNCH("sn_sweep_step", per_item * n, sn_sweep_step_kernel<T>, grid_for(n, 256), 256, c.stream,
-                   S->items.p + a, n, S->C, S->G, g, sigt, q, psi);
+                   S->items.p + a, n, S->C, S->G, g, S->dpos.p, sigt, q, psi);
       }
       PK_HIP(hipGetLastError());
       return 0;
    }
    PK_CALL(sn_records<T>(S));
-   if (profiling()) {
-      sn_launch_steps<T>(S, sigt, q, psi, true);
-      PK_HIP(hipGetLastError());
-      return 0;
+   constexpr bool f32 = sizeof(T) == sizeof(float);
+   const auto& B = f32 ? S->rec32 : S->rec64;
+   // the source in level order (one gather over all items, all compute units)
+   T* qp = f32 ? (T*)S->qp32.p : (T*)S->qp64.p;
+   {
+      const long n = (long)S->M * S->C;
+      const int bytes = S->G * (int)sizeof(T);
+      const double traffic = profiling() ? (double)n * bytes * 2 + n * 4.0 : 0.0;
+      if (bytes % 16 == 0)
+         PK_LAUNCH("sn_permute", traffic, sn_permute_kernel<uint4>, grid_for(n * (bytes / 16)), kBlock, c.stream,
+                   S->ditems.p, n, S->C, bytes / 16, (const uint4*)q, (uint4*)qp);
+      else if (bytes % 8 == 0)
+         PK_LAUNCH("sn_permute", traffic, sn_permute_kernel<uint2>, grid_for(n * (bytes / 8)), kBlock, c.stream,
+                   S->ditems.p, n, S->C, bytes / 8, (const uint2*)q, (uint2*)qp);
+      else
+         PK_LAUNCH("sn_permute", traffic, sn_permute_kernel<unsigned>, grid_for(n * (bytes / 4)), kBlock, c.stream,
+                   S->ditems.p, n, S->C, bytes / 4, (const unsigned*)q, (unsigned*)qp);
    }
-   const int pi = sizeof(T) == sizeof(float) ? 0 : 1;
-   hipGraphExec_t& G = pi == 0 ? S->graph32 : S->graph64;
-   const void** gp = S->graph_ptrs[pi];
-   if (G && (gp[0] != sigt || gp[1] != q || gp[2] != psi)) {
-      (void)hipGraphExecDestroy(G);
-      G = nullptr;
+   int done = -1;
+   if (B.fast) {
+      switch (B.gt * 100 + B.P * 10 + B.D) {
+         case 111: done = sn_launch_lds<T, 1, 1, 1>(S, qp, psi); break;
+         case 121: done = sn_launch_lds<T, 1, 2, 1>(S, qp, psi); break;
+         case 131: done = sn_launch_lds<T, 1, 3, 1>(S, qp, psi); break;
+         case 112: done = sn_launch_lds<T, 1, 1, 2>(S, qp, psi); break;
+         case 122: done = sn_launch_lds<T, 1, 2, 2>(S, qp, psi); break;
+         case 132: if constexpr (f32) done = sn_launch_lds<T, 1, 3, 2>(S, qp, psi); break;
+         case 211: done = sn_launch_lds<T, 2, 1, 1>(S, qp, psi); break;
+         case 221: done = sn_launch_lds<T, 2, 2, 1>(S, qp, psi); break;
+         case 231: if constexpr (f32) done = sn_launch_lds<T, 2, 3, 1>(S, qp, psi); break;
+         case 212: done = sn_launch_lds<T, 2, 1, 2>(S, qp, psi); break;
+         case 222: if constexpr (f32) done = sn_launch_lds<T, 2, 2, 2>(S, qp, psi); break;
+         case 411: done = sn_launch_lds<T, 4, 1, 1>(S, qp, psi); break;
+         case 421: if constexpr (f32) done = sn_launch_lds<T, 4, 2, 1>(S, qp, psi); break;
+         case 431: if constexpr (f32) done = sn_launch_lds<T, 4, 3, 1>(S, qp, psi); break;
+         case 412: if constexpr (f32) done = sn_launch_lds<T, 4, 1, 2>(S, qp, psi); break;
+         default: break;
+      }
+      if (done > 0) return 1;
    }
-   if (!G) {
-      gp[0] = sigt;
-      gp[1] = q;
-      gp[2] = psi;
-      hipGraph_t g = nullptr;
-      PK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      sn_launch_steps<T>(S, sigt, q, psi, false);
-      hipError_t e = hipStreamEndCapture(c.stream, &g);
-      PK_HIP(e);
-      hipError_t ie = hipGraphInstantiate(&G, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      PK_HIP(ie);
-   }
-   PK_HIP(hipGraphLaunch(G, c.stream));
+   if (done != 0) PK_CALL(sn_launch_lds_any<T>(S, qp, psi));
+   PK_HIP(hipGetLastError());
    return 0;
 }
 
@@ -516,12 +911,35 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
          S->step_ptr[s + 1] = (long)items.size();
       }
    }
+   // direction-major copy for the per-direction kernel: the items of direction
+   // m level by level, with absolute level offsets padded to maxlev + 1
+   std::vector<int> ditems;
+   ditems.reserve((size_t)M * C);
+   S->lstride = maxlev + 1;
+   std::vector<int> dlptr((size_t)M * S->lstride, 0);
+   for (int m = 0; m < M; m++) {
+      const auto& L = lists[m];
+      int l = 0;
+      const int base = (int)ditems.size();
+      dlptr[(size_t)m * S->lstride] = base;
+      for (size_t k = 0; k < L.size(); k++) {
+         const int lv = S->level[(size_t)m * C + L[k]];
+         while (l < lv) dlptr[(size_t)m * S->lstride + (++l)] = base + (int)k;
+         ditems.push_back(m * C + L[k]);
+      }
+      while (l < maxlev) dlptr[(size_t)m * S->lstride + (++l)] = (int)ditems.size();
+      for (int k = 0; k < S->nlev[m]; k++)
+         S->maxw = std::max(S->maxw, dlptr[(size_t)m * S->lstride + k + 1] - dlptr[(size_t)m * S->lstride + k]);
+   }
    const char* ge = getenv("PK_SWEEP_GEOMETRY");
    S->geometry = ge && *ge && *ge != '0';
    const int G = S->G;
    S->value_units = (double)((long)C * G * M + nin_total * G) + 2.0 * C * G;
    int e = 0;
-   e |= S->items.alloc(items.size());
+   dlptr.push_back(0);  // padding: a row's last entry may be read two past the last level
+   dlptr.push_back(0);
+   e |= S->items.alloc(items.size()) | S->ditems.alloc(ditems.size()) | S->dlptr.alloc(dlptr.size()) |
+        S->dnlev.alloc(M);
    e |= S->dirs.alloc(3L * M) | S->wts.alloc(M) | S->coef.alloc(mesh->F ? mesh->F : 1);
    e |= S->phiw.alloc((size_t)C * G) | S->old.alloc((size_t)C * G) | S->st.alloc(8);
    if (e) {
@@ -529,6 +947,18 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
       PK_FAIL("device allocation failed");
    }
    PK_HIP(hipMemcpy(S->items.p, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
+   PK_HIP(hipMemcpy(S->ditems.p, ditems.data(), sizeof(int) * ditems.size(), hipMemcpyHostToDevice));
+   PK_HIP(hipMemcpy(S->dlptr.p, dlptr.data(), sizeof(int) * dlptr.size(), hipMemcpyHostToDevice));
+   PK_HIP(hipMemcpy(S->dnlev.p, S->nlev.data(), sizeof(int) * M, hipMemcpyHostToDevice));
+   {
+      std::vector<int> pos((size_t)M * C);
+      for (size_t k = 0; k < ditems.size(); k++) pos[ditems[k]] = (int)k;
+      PK_REQUIRE(!S->dpos.alloc(pos.size()), "device allocation failed");
+      PK_HIP(hipMemcpy(S->dpos.p, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
+   }
+   S->h_ditems.swap(ditems);
+   S->h_dlptr.swap(dlptr);
+   S->h_dirs.assign(dirs, dirs + 3L * M);
 
    PK_HIP(hipMemcpy(S->dirs.p, dirs, sizeof(double) * 3 * M, hipMemcpyHostToDevice));
    PK_HIP(hipMemcpy(S->wts.p, wts, sizeof(double) * M, hipMemcpyHostToDevice));
@@ -563,14 +993,14 @@ static int sn_prepare(pk_sn_s* S) {
    const int nm = S->xs->nm * S->G;
    if (S->precision == PK_PREC_MIXED) {
       if (!S->psi32.p) {
-         PK_REQUIRE(!S->psi32.alloc(CGM) && !S->q32.alloc(CG) && !S->sigt32.alloc(nm),
+         PK_REQUIRE(!S->psi32.alloc(CGM) && !S->q32.alloc(CG) && !S->sigt32.alloc(nm) && !S->qp32.alloc(CGM),
                     "device allocation failed");
          std::vector<float> st(S->xs->h_sigt.begin(), S->xs->h_sigt.end());
          PK_HIP(hipMemcpy(S->sigt32.p, st.data(), sizeof(float) * nm, hipMemcpyHostToDevice));
       }
    } else {
       if (!S->psi64.p) {
-         PK_REQUIRE(!S->psi64.alloc(CGM) && !S->q64.alloc(CG) && !S->sigt64.alloc(nm),
+         PK_REQUIRE(!S->psi64.alloc(CGM) && !S->q64.alloc(CG) && !S->sigt64.alloc(nm) && !S->qp64.alloc(CGM),
                     "device allocation failed");
          PK_HIP(hipMemcpy(S->sigt64.p, S->xs->h_sigt.data(), sizeof(double) * nm, hipMemcpyHostToDevice));
       }
@@ -589,13 +1019,13 @@ int pk_sn_sweep_host(pk_sn S, const double* q, double* psi, double* phiw) {
       std::vector<float> qf(q, q + CG);
       PK_HIP(hipMemcpy(S->q32.p, qf.data(), sizeof(float) * CG, hipMemcpyHostToDevice));
       PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
-      sn_moment_kernel<float><<<grid_for(CG), kBlock, 0, c.stream>>>(S->psi32.p, S->wts.p, S->M, CG, S->phiw.p);
-      sn_to_reference_kernel<float><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi32.p, S->C, S->G, S->M, 1.0, out.p);
+      sn_moment<float>(S->psi32.p, S->dpos.p, S->wts.p, S->M, S->C, S->G, S->phiw.p, c.stream);
+      sn_to_reference_kernel<float><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi32.p, S->dpos.p, S->C, S->G, S->M, 1.0, out.p);
    } else {
       PK_HIP(hipMemcpy(S->q64.p, q, sizeof(double) * CG, hipMemcpyHostToDevice));
       PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
-      sn_moment_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->psi64.p, S->wts.p, S->M, CG, S->phiw.p);
-      sn_to_reference_kernel<double><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi64.p, S->C, S->G, S->M, 1.0, out.p);
+      sn_moment<double>(S->psi64.p, S->dpos.p, S->wts.p, S->M, S->C, S->G, S->phiw.p, c.stream);
+      sn_to_reference_kernel<double><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi64.p, S->dpos.p, S->C, S->G, S->M, 1.0, out.p);
    }
    PK_HIP(hipGetLastError());
    PK_HIP(hipStreamSynchronize(c.stream));
@@ -637,12 +1067,12 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
          sn_source_kernel<float><<<grid_for(C), kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->sigs.p,
                                                                     xs->nusf.p, xs->chi.p, S->phiw.p, st, S->q32.p);
          PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
-         sn_moment_kernel<float><<<grid_for(CG), kBlock, 0, c.stream>>>(S->psi32.p, S->wts.p, M, CG, S->phiw.p);
+         sn_moment<float>(S->psi32.p, S->dpos.p, S->wts.p, M, C, G, S->phiw.p, c.stream);
       } else {
          sn_source_kernel<double><<<grid_for(C), kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->sigs.p,
                                                                      xs->nusf.p, xs->chi.p, S->phiw.p, st, S->q64.p);
          PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
-         sn_moment_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->psi64.p, S->wts.p, M, CG, S->phiw.p);
+         sn_moment<double>(S->psi64.p, S->dpos.p, S->wts.p, M, C, G, S->phiw.p, c.stream);
       }
       S->sweeps++;
       sn_prod_kernel<<<gb, kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->nusf.p, S->phiw.p,
@@ -675,9 +1105,9 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
    // psi of the last sweep, rescaled like phiw (reference layout)
    if (psi) {
       if (S->precision == PK_PREC_MIXED)
-         sn_to_reference_kernel<float><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi32.p, C, G, M, last_scale, psi->d.p);
+         sn_to_reference_kernel<float><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi32.p, S->dpos.p, C, G, M, last_scale, psi->d.p);
       else
-         sn_to_reference_kernel<double><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi64.p, C, G, M, last_scale, psi->d.p);
+         sn_to_reference_kernel<double><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi64.p, S->dpos.p, C, G, M, last_scale, psi->d.p);
    }
    if (phiw_out) PK_HIP(hipMemcpyAsync(phiw_out->d.p, S->phiw.p, sizeof(double) * CG, hipMemcpyDeviceToDevice, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));
