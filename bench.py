@@ -160,10 +160,23 @@ def _oracle_request(oracle, W, path, seed, rel):
     return oracle.eigen_sinvert(R, F)
 
 
-def _accuracy(oracle, W, res, paths, cid, seeds, rel, sample):
+def _oracle_request_sn(oracle, W, path, seed, rel):
+    """FP64 oracle solve of a served S_N request: source iteration with sweeps
+    on the perturbed tables; returns keff and the moment sum_m w_m psi."""
+    case = oracle.pmp.read_input(path)
+    mesh = oracle.build_mesh(case)
+    s = case.solver
+    xs = oracle.xs_tables(case.materials, s.G, diffusion=False)
+    xs.sigt, xs.sigs, _, xs.nusf = W.perturb_tables(xs.sigt, xs.sigs, None, xs.nusf, seed, rel)
+    k, _, phiw, _ = oracle.sn_sweep_eigen(mesh, xs, s, inner=1, tol_k=1e-12, tol_phi=1e-11)
+    return k, phiw
+
+
+def _accuracy(oracle, W, res, paths, cid, seeds, rel, sample, sn_cases=()):
     out = []
     for r in sample:
-        k_o, x_o = _oracle_request(oracle, W, paths[cid[r]], int(seeds[r]), rel)
+        solve = _oracle_request_sn if cid[r] in sn_cases else _oracle_request
+        k_o, x_o = solve(oracle, W, paths[cid[r]], int(seeds[r]), rel)
         out.append({"request": int(r), "dk_pcm": abs(res["keff"][r] - k_o) * 1e5,
                     "flux_l2": float(np.linalg.norm(_shape(res["flux"][r]) - _shape(x_o)) /
                                      np.linalg.norm(_shape(x_o)))})
@@ -171,8 +184,9 @@ def _accuracy(oracle, W, res, paths, cid, seeds, rel, sample):
             "requests": out}
 
 
-def _serve(pa, paths, precision, slots, arrival, cid, seeds, rel, warm, keep=(), budget=64e9):
+def _serve(pa, paths, precision, slots, arrival, cid, seeds, rel, warm, keep=(), budget=64e9, sn_workers=1):
     srv = pa.Server(budget_bytes=budget, max_slots=slots, precision=precision)
+    srv.sn_workers(sn_workers)
     t0 = time.perf_counter()
     for p in paths:
         srv.add_case(p)
@@ -269,28 +283,63 @@ def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
             "spmv_all_instances": spmv}
 
 
-def serving_numbers(pa, W, oracle, tmp, n_base=16, n_req=300, rate=200.0, rel=0.02, seed=11):
-    """Open-loop Poisson stream of seeded +-2% perturbations of n_base
-    previously seen base cases (the batched family), warm-started: the mixed
-    pool with continuous batching vs the plain FP64 drop-in (one at a time) on
-    the same stream."""
+def _sn_bases(W, tmp, n, seed=5):
+    """Base S_N cases of the serving stream: the hexagonal lattice with a
+    seeded number of layers (1-8) and S_N order (4, 6, 8), each with its own
+    materials.  Returns (paths, rows, labels)."""
+    rng = np.random.default_rng(seed)
+    paths, rows, labels = [], [], []
+    for j in range(n):
+        layers, order = int(rng.integers(1, 9)), int(rng.choice([4, 6, 8]))
+        d = os.path.join(tmp, f"sn-{j}")
+        W.hex_sn_3d(d, seed=100 + j, layers=layers, order=order)
+        paths.append(os.path.join(d, "input.pmp"))
+        rows.append(331 * 24 * layers * 4)
+        labels.append(f"S{order}, {layers} layers")
+    return paths, np.array(rows), labels
+
+
+def serving_numbers(pa, W, oracle, tmp, n_base=16, n_sn=4, n_req=300, rate=200.0, rel=0.02, seed=11,
+                    sn_share=0.2):
+    """Open-loop Poisson stream of seeded +-2% perturbations of previously seen
+    base cases, warm-started: 80 % diffusion requests on n_base cases of the
+    batched family, 20 % S_N requests on n_sn hexagonal-lattice cases.  The
+    mixed engine (continuous batching of diffusion requests, S_N requests on 4
+    workers per case beside it) vs the plain FP64 engine with one request at a
+    time per kind (one pool slot, one worker per S_N case), on the same
+    stream."""
     paths, rows = _family(W, tmp, n_base)
+    sn_paths, sn_rows, sn_labels = _sn_bases(W, tmp, n_sn)
+    all_paths = paths + sn_paths
+    sn_cases = set(range(n_base, n_base + n_sn))
     rng = np.random.default_rng(seed)
     arrival = np.cumsum(rng.exponential(1.0 / rate, n_req))
     arrival -= arrival[0]
-    cid = rng.integers(0, n_base, n_req)
+    is_sn = rng.random(n_req) < sn_share
+    cid = np.where(is_sn, n_base + rng.integers(0, n_sn, n_req), rng.integers(0, n_base, n_req))
     seeds = np.arange(1, n_req + 1, dtype=np.uint64)
-    small = [r for r in range(n_req) if rows[cid[r]] <= 20000][:4]
-    out = {"workload": f"{n_req} requests, Poisson {rate:g} req/s, {n_base} base cases of the batched "
-                       f"family ({int(rows.sum())} rows), +-{rel:g} seeded perturbations, warm start"}
-    for name, prec, slots in (("mixed", pa.PK_PREC_MIXED, 256), ("fp64_dropin", pa.PK_PREC_FP64, 1)):
+    small = [r for r in range(n_req) if not is_sn[r] and rows[cid[r]] <= 20000][:4]
+    smallest_sn = n_base + int(np.argmin(sn_rows))
+    small_sn = [r for r in range(n_req) if cid[r] == smallest_sn][:2]
+    out = {"workload": f"{n_req} requests, Poisson {rate:g} req/s, {int((~is_sn).sum())} diffusion on {n_base} "
+                       f"base cases of the batched family ({int(rows.sum())} rows), {int(is_sn.sum())} S_N on "
+                       f"{n_sn} hexagonal-lattice cases ({', '.join(sn_labels)}), +-{rel:g} seeded "
+                       f"perturbations, warm start"}
+    for name, prec, slots, snw in (("mixed", pa.PK_PREC_MIXED, 256, 4), ("fp64_dropin", pa.PK_PREC_FP64, 1, 1)):
         log(f"[bench] serving: {name}")
-        res, st = _serve(pa, paths, prec, slots, arrival, cid, seeds, rel, True, keep=small)
+        res, st = _serve(pa, all_paths, prec, slots, arrival, cid, seeds, rel, True, keep=small + small_sn,
+                         sn_workers=snw)
         span = float((arrival + res["latency"]).max())
-        out[name] = {"solves_per_s": n_req / span, "p50_ms": float(np.percentile(res["latency"], 50) * 1e3),
-                     "p99_ms": float(np.percentile(res["latency"], 99) * 1e3), "makespan_s": span,
+        lat = res["latency"]
+        out[name] = {"solves_per_s": n_req / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3), "makespan_s": span,
+                     "diffusion_p50_ms": float(np.percentile(lat[~is_sn], 50) * 1e3),
+                     "diffusion_p99_ms": float(np.percentile(lat[~is_sn], 99) * 1e3),
+                     "sn_p50_ms": float(np.percentile(lat[is_sn], 50) * 1e3) if is_sn.any() else None,
+                     "sn_p99_ms": float(np.percentile(lat[is_sn], 99) * 1e3) if is_sn.any() else None,
                      "max_in_flight": st["max_live"], "converged": int(res["converged"].sum()),
-                     "accuracy_vs_fp64_oracle": _accuracy(oracle, W, res, paths, cid, seeds, rel, small)}
+                     "accuracy_vs_fp64_oracle": _accuracy(oracle, W, res, all_paths, cid, seeds, rel,
+                                                          small + small_sn, sn_cases)}
     out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64_dropin"]["solves_per_s"]
     return out
 
@@ -307,6 +356,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances in the batched run")
     ap.add_argument("--spmv-batched-only", action="store_true",
                     help="only the SpMV over every instance of the batched family (FP32 and FP64)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving stream (diffusion and S_N requests, mixed vs FP64)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only the S_N sweep timing of the hexagonal S6 lattice (FP32 and FP64)")
     ap.add_argument("--roofline-pass", action="store_true",
@@ -330,6 +381,11 @@ def main():
     if args.sweep_only:
         d2 = W.hex_sn_3d(os.path.join(tmp, "hex-s6"))
         print(json.dumps(sweep_numbers(pa, d2)))
+        return
+    if args.serving_only:
+        import oracle
+        oracle.lib()
+        print(json.dumps(serving_numbers(pa, W, oracle, os.path.join(tmp, "family"))))
         return
     if args.spmv_batched_only:
         paths, rows = _family(W, os.path.join(tmp, "family"), args.batch)

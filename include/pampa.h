@@ -61,16 +61,18 @@ void pampa_ext_time_kernel(int reps, int fp32, double* seconds, double* bytes, i
  * perturbed copies of a few base problems as requests arrive).  A server owns
  * one device eigen pool sized to budget_bytes of device memory (<= 0: 80% of
  * the free device memory) with max_slots problems at once, and a set of
- * resident base cases (diffusion inputs).  Requests are admitted in arrival
- * order as slots and memory free up and are batched into every kernel launch
- * with the others in flight (continuous batching); a result does not depend
- * on what else was in flight.  Independent of the process-global calculation
- * of pampa_initialize. */
+ * resident base cases (diffusion inputs and sweepable S_N inputs).  Diffusion
+ * requests are admitted in arrival order as slots and memory free up and are
+ * batched into every kernel launch with the others in flight (continuous
+ * batching); S_N requests go in arrival order to the case's workers (host
+ * threads, each with its own sweep handle and stream, running beside the
+ * pool).  A result does not depend on what else was in flight.  Independent
+ * of the process-global calculation of pampa_initialize. */
 typedef struct pampa_server_s* pampa_server;
 void pampa_serve_create(double budget_bytes, int max_slots, int precision, pampa_server* S, int* error);
 /* eigen tolerance (relative residual), inner GMRES reduction, Wielandt shift */
 void pampa_serve_set_options(pampa_server S, double tol, double reduction, double shift, int* error);
-/* parse and upload a diffusion input; rows = cells x groups */
+/* parse and upload an input; rows = cells x groups */
 void pampa_serve_add_case(pampa_server S, const char* input, int* case_id, long* rows, int* groups,
                           int* error);
 /* the cross-section tables a request (seed, rel) of the case solves with:
@@ -82,13 +84,17 @@ void pampa_serve_footprint(pampa_server S, int case_id, double* bytes, double* p
 /* n requests (open loop): request r arrives arrival[r] seconds after the call
  * starts (non-decreasing) and asks for case case_id[r] with every cross section
  * scaled by its own U[1-rel, 1+rel] factor from seed[r] (0: none); warm != 0
- * starts it from the case's latest converged flux.  Per request: keff,
- * latency (completion - arrival, s), outer/inner iterations, converged, and,
- * where flux[r] != NULL, the eigenvector (unit 2-norm, cells x groups). */
+ * starts it from the case's latest converged flux (S_N: the worker's previous
+ * solution).  Per request: keff, latency (completion - arrival, s),
+ * outer/inner iterations (S_N: source iterations, sweeps), converged, and,
+ * where flux[r] != NULL, the eigenvector (cells x groups; diffusion: unit
+ * 2-norm; S_N: the moment sum_m w_m psi). */
 void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* case_id,
                      const unsigned long long* seed, double rel, int warm, double* keff, double* latency,
                      int* outer, long* inner, int* converged, double* const* flux, int* error);
 void pampa_serve_forget_warm(pampa_server S, int* error);
+/* workers (sweep handles) per S_N case added after this call (default 1) */
+void pampa_serve_set_sn_workers(pampa_server S, int workers, int* error);
 /* seconds > 0: pampa_serve_run prints a progress line on stderr that often */
 void pampa_serve_set_progress(pampa_server S, double seconds, int* error);
 /* Every case once in the pool, and the SpMV of all their inner operators in

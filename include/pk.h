@@ -223,7 +223,29 @@ int pk_sn_set_tolerances(pk_sn S, double tol_k, double tol_phi, int max_outer);
 /* One sweep psi = L^-1 q for all (g, m); q[cell][group] includes the volume.
  * phiw[cell][group] = sum_m w_m psi.  Host arrays, for parity checks. */
 int pk_sn_sweep_host(pk_sn S, const double* q, double* psi, double* phiw);
+/* Source iteration inside power iteration (NeutronicSolver::solve,
+ * NeutronicSolver.cxx:4-43, in the sweepable regime).  Every handle runs on a
+ * stream of its own with its own scratch: solves of different handles may run
+ * at the same time from different host threads, and beside the library
+ * stream.  The call returns when the handle's work is complete.  A handle
+ * starts from its previous solution (warm) unless reset. */
 int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw);
+/* New cross sections for the handle's tables (a perturbed request on the same
+ * mesh; no counterpart in the reference, which rebuilds the solver): total,
+ * scattering [from][to], nu-fission, kappa-fission and chi per material and
+ * group, uploaded on the handle's stream; the sweep records are rebuilt on the
+ * next sweep, the schedules and value slots are kept. */
+int pk_sn_set_xs(pk_sn S, const double* sigt, const double* sigs, const double* nusf, const double* kapf,
+                 const double* chi);
+/* Allocate the buffers and build the sweep records of the current precision
+ * now (they are otherwise made by the first solve), so that later solves and
+ * cross-section changes allocate nothing. */
+int pk_sn_prepare(pk_sn S);
+/* The scalar-flux moment phiw[cell][group] = sum_m w_m psi of the last solve
+ * (host copy on the handle's stream; production-normalised as iterated). */
+int pk_sn_get_moment(pk_sn S, double* phiw);
+/* Forget the previous solution: the next solve starts cold. */
+int pk_sn_reset(pk_sn S);
 int pk_sn_get_stats(pk_sn S, int* outer, long* sweeps, double* seconds);
 int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_sweep);
 int pk_sn_destroy(pk_sn* S);

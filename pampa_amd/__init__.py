@@ -109,7 +109,7 @@ def _declare(L):
         "pampa_serve_case_tables": [_P, _I, ct.c_ulonglong, _D, _P, _P, _P, _P, _IP],
         "pampa_serve_footprint": [_P, _I, _P, _P, _IP],
         "pampa_serve_run": [_P, _I, _P, _P, _ULLP, _D, _I, _P, _P, _P, _P, _P, _P, _IP],
-        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_set_progress": [_P, _D, _IP], "pampa_serve_time_spmv": [_P, _I, _P, _P, _P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
+        "pampa_serve_forget_warm": [_P, _IP], "pampa_serve_set_progress": [_P, _D, _IP], "pampa_serve_set_sn_workers": [_P, _I, _IP], "pampa_serve_time_spmv": [_P, _I, _P, _P, _P, _IP], "pampa_serve_stats": [_P, _P, _P, _P, _P, _IP],
         "pampa_serve_destroy": [_PP, _IP],
     })
     for name, args in drop.items():
@@ -503,6 +503,10 @@ class Server:
     def progress(self, seconds: float):
         """Print a progress line on stderr every `seconds` during run()."""
         self._call("pampa_serve_set_progress", self.h, seconds)
+
+    def sn_workers(self, n: int):
+        """Workers (sweep handles) per S_N case added after this call."""
+        self._call("pampa_serve_set_sn_workers", self.h, int(n))
 
     def forget_warm(self):
         self._call("pampa_serve_forget_warm", self.h)

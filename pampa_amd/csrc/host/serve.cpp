@@ -1,5 +1,8 @@
-// Serving loop: eigenvalue requests on resident base cases, continuously
-// batched into one device eigen pool (pk_pool_*).
+// Serving loop: eigenvalue requests on resident base cases.  Diffusion
+// requests are continuously batched into one device eigen pool (pk_pool_*);
+// discrete-ordinates requests are solved by the sweep engine (pk_sn_*), one
+// worker thread per resident S_N case, each on the sweep handle's own stream,
+// so that they run on the device beside the pool.
 //
 // A case is a diffusion input (.pmp) parsed and uploaded once: its mesh, its
 // base cross-section tables and its R/F pattern stay on the device.  A request
@@ -10,8 +13,11 @@
 // loop admits waiting requests in arrival order while the pool has room (slots
 // and device memory, first come first served), and collects finished ones.  A
 // case remembers the flux of its latest converged request and can start the
-// next one from it (warm start).
+// next one from it (warm start).  An S_N case takes its requests one at a time
+// in arrival order; its sweep handle keeps the previous solution for the warm
+// start.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <chrono>
 #include <cmath>
@@ -80,7 +86,30 @@ struct Case {
    bool has_warm = false;
    long n = 0;
    double footprint = 0.0;
+   // S_N case: workers, each with its own solver (sweep handle, stream and
+   // previous solution), and the requests waiting for one of them
+   struct Worker {
+      std::unique_ptr<Driver> driver;
+      pk_sn sn = nullptr;
+      XSTables work;
+      std::thread thread;
+      std::atomic<bool> busy{false};
+      int req = -1;  // request on the worker
+      double k = 0.0, t_done = 0.0;
+      int outer = 0, conv = 0, rc = 0;
+      long sweeps = 0;
+      std::string err;
+      std::vector<double> phiw;
+      ~Worker() {
+         if (thread.joinable()) thread.join();
+         if (driver) driver->finalize();
+      }
+   };
+   std::vector<std::unique_ptr<Worker>> workers;
+   bool sn = false;
+   std::deque<int> queue;  // waiting requests, arrival order
    ~Case() {
+      workers.clear();
       pk_mat_destroy(&R);
       pk_mat_destroy(&F);
       pk_vec_destroy(&warm);
@@ -103,6 +132,29 @@ struct Server {
    int max_live = 0;
    double progress = 0.0;  // > 0: a progress line on stderr every `progress` seconds of a run
    ~Server() { pk_batch_destroy(&pool); }
+
+   int sn_workers = 1;  // workers (sweep handles) per S_N case
+
+   // One S_N request on a worker thread: perturbed tables, a solve (warm from
+   // the worker's previous one unless `warm` is 0), the moment to the host.
+   static void solveSN(const Case* c, Case::Worker* w, unsigned long long seed, double rel, int warm,
+                       std::chrono::steady_clock::time_point t0) {
+      w->err.clear();
+      perturb(c->base, seed, rel, w->work);
+      int rc = pk_sn_set_xs(w->sn, w->work.st.data(), w->work.ss.data(), w->work.nf.data(), w->work.kf.data(),
+                            w->work.ch.data());
+      if (!rc && !warm) rc = pk_sn_reset(w->sn);
+      if (!rc) rc = pk_sn_solve(w->sn, &w->k, nullptr, nullptr);
+      w->conv = rc == 0;
+      // a solve that stops at its iteration limit is reported, not an error
+      if (rc && std::string(pk_last_error()).find("did not converge") != std::string::npos) rc = 0;
+      if (!rc) rc = pk_sn_get_stats(w->sn, &w->outer, &w->sweeps, nullptr);
+      if (!rc) rc = pk_sn_get_moment(w->sn, w->phiw.data());
+      if (rc) w->err = pk_last_error();
+      w->rc = rc;
+      w->t_done = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      w->busy.store(false, std::memory_order_release);
+   }
 
    int requestOperators(Case& c, unsigned long long seed, double rel) {
       perturb(c.base, seed, rel, c.work);
@@ -178,7 +230,39 @@ void pampa_serve_add_case(pampa_server S, const char* input, int* case_id, long*
    if (c->driver->initialize((int)argv.size(), argv.data(), dt)) return;
    c->solver = dynamic_cast<DiffusionSolver*>(c->driver->mainSolver());
    if (!c->solver) {
-      report_error(__FILE__, __LINE__, __func__, "serving takes diffusion inputs");
+      // S_N input: sn_workers solvers of the input, each a worker
+      c->sn = true;
+      for (int k = 0; k < S->sn_workers; k++) {
+         auto w = std::make_unique<Case::Worker>();
+         if (k == 0) {
+            w->driver = std::move(c->driver);
+         } else {
+            w->driver = std::make_unique<Driver>();
+            if (w->driver->initialize((int)argv.size(), argv.data(), dt)) return;
+         }
+         auto* sn = dynamic_cast<SNSolver*>(w->driver->mainSolver());
+         if (!sn || !sn->sweepHandle()) {
+            report_error(__FILE__, __LINE__, __func__,
+                         "serving takes diffusion inputs and sweepable S_N inputs (pure upwind, LS off, vacuum)");
+            return;
+         }
+         if (k == 0) {
+            if (sn->materialTables(false, c->base)) return;
+            c->n = sn->numCells() * (long)sn->groups();
+            if (groups) *groups = sn->groups();
+         }
+         w->sn = sn->sweepHandle();
+         w->phiw.assign(c->n, 0.0);
+         if (pk_sn_set_precision(w->sn, S->precision) || pk_sn_prepare(w->sn)) {
+            report_error(__FILE__, __LINE__, __func__, pk_last_error());
+            return;
+         }
+         c->workers.push_back(std::move(w));
+      }
+      *case_id = (int)S->cases.size();
+      if (rows) *rows = c->n;
+      S->cases.push_back(std::move(c));
+      *error = 0;
       return;
    }
    if (c->solver->materialTables(true, c->base)) return;
@@ -249,17 +333,70 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
       return rc != 0;
    };
    double next_report = S->progress;
+   int sn_busy = 0;  // S_N workers running
+   // a worker that has finished: its results into the request's outputs
+   auto finish_sn = [&](Case::Worker& w) {
+      w.thread.join();
+      sn_busy--;
+      const int r = w.req;
+      if (w.rc) {
+         report_error(__FILE__, __LINE__, "pampa_serve_run", w.err.c_str());
+         return true;
+      }
+      if (keff) keff[r] = w.k;
+      if (latency) latency[r] = w.t_done - arrival[r];
+      if (outer) outer[r] = w.outer;
+      if (inner) inner[r] = w.sweeps;
+      if (converged) converged[r] = w.conv;
+      if (flux && flux[r]) std::copy(w.phiw.begin(), w.phiw.end(), flux[r]);
+      w.req = -1;
+      done++;
+      return false;
+   };
+   auto join_all = [&] {
+      for (auto& cp : S->cases)
+         for (auto& w : cp->workers)
+            if (w->thread.joinable()) w->thread.join();
+   };
    while (done < n) {
       const double t = now();
       if (S->progress > 0.0 && t >= next_report) {
          long tk = 0;
          pk_pool_capacity(S->pool, nullptr, nullptr, nullptr, &tk);
-         fprintf(stderr, "[serve] %.1f s: %d of %d done, %d in flight, %zu waiting, %ld ticks\n", t, done, n, live,
-                 pending.size(), tk - ticks0);
+         fprintf(stderr, "[serve] %.1f s: %d of %d done, %d in flight (%d S_N), %zu waiting, %ld ticks\n", t, done,
+                 n, live + sn_busy, sn_busy, pending.size(), tk - ticks0);
          fflush(stderr);
          next_report = t + S->progress;
       }
-      while (next < n && arrival[next] <= t) pending.push_back(next++);
+      while (next < n && arrival[next] <= t) {
+         Case& c = *S->cases[case_id[next]];
+         if (c.sn)
+            c.queue.push_back(next++);
+         else
+            pending.push_back(next++);
+      }
+      // S_N requests: collect finished workers, give waiting requests to idle ones
+      for (auto& cp : S->cases) {
+         Case& c = *cp;
+         if (!c.sn) continue;
+         for (auto& wp : c.workers) {
+            Case::Worker& w = *wp;
+            if (w.thread.joinable() && !w.busy.load(std::memory_order_acquire)) {
+               if (finish_sn(w)) {
+                  join_all();
+                  return;
+               }
+            }
+            if (!w.thread.joinable() && !c.queue.empty()) {
+               const int r = c.queue.front();
+               c.queue.pop_front();
+               w.req = r;
+               w.busy.store(true, std::memory_order_release);
+               w.thread = std::thread(Server::solveSN, &c, &w, seed[r], rel, warm, t0);
+               sn_busy++;
+            }
+         }
+      }
       // admission in arrival order while the pool has room
       while (!pending.empty()) {
          const int r = pending.front();
@@ -268,32 +405,44 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
          if (check(pk_pool_fits(S->pool, c.R, c.F, &fits))) return;
          if (!fits && live == 0) {
             report_error(__FILE__, __LINE__, __func__, "request larger than the eigen pool");
+            join_all();
             return;
          }
          if (!fits) break;
-         if (S->requestOperators(c, seed[r], rel)) return;
+         if (S->requestOperators(c, seed[r], rel)) {
+            join_all();
+            return;
+         }
          int slot = -1;
          int rc = pk_pool_admit(S->pool, c.R, c.F, (warm && c.has_warm) ? c.warm : nullptr, &slot);
          if (rc == PK_POOL_FULL) break;
-         if (check(rc)) return;
+         if (check(rc)) {
+            join_all();
+            return;
+         }
          slot_req[slot] = r;
          pending.pop_front();
          live++;
       }
-      S->max_live = std::max(S->max_live, live);
+      S->max_live = std::max(S->max_live, live + sn_busy);
       if (live == 0) {
-         // idle until the next arrival
-         if (next < n) {
-            const double wait = arrival[next] - now();
-            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
-         }
+         // no diffusion request in the pool: wait for the next arrival or a worker
+         double wait = next < n ? arrival[next] - now() : 1.0;
+         if (sn_busy) wait = std::min(wait, 2e-4);
+         if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
          continue;
       }
       int nlive = 0, ndone = 0;
-      if (check(pk_pool_step(S->pool, 1, &nlive, &ndone))) return;
+      if (check(pk_pool_step(S->pool, 1, &nlive, &ndone))) {
+         join_all();
+         return;
+      }
       if (ndone == 0) continue;
       int cnt = 0;
-      if (check(pk_pool_collect(S->pool, S->max_slots, ready.data(), &cnt))) return;
+      if (check(pk_pool_collect(S->pool, S->max_slots, ready.data(), &cnt))) {
+         join_all();
+         return;
+      }
       const double tc = now();
       for (int i = 0; i < cnt; i++) {
          const int slot = ready[i];
@@ -302,19 +451,21 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
          double k = 0.0, res = 0.0;
          int o = 0, conv = 0;
          long in = 0;
-         if (check(pk_pool_result(S->pool, slot, &k, c.tmp, &o, &in, &res, &conv))) return;
+         if (check(pk_pool_result(S->pool, slot, &k, c.tmp, &o, &in, &res, &conv))) {
+            join_all();
+            return;
+         }
          if (keff) keff[r] = k;
          if (latency) latency[r] = tc - arrival[r];
          if (outer) outer[r] = o;
          if (inner) inner[r] = in;
          if (converged) converged[r] = conv;
-         if (flux && flux[r])
-            if (check(pk_vec_get_values(c.tmp, flux[r]))) return;
-         if (conv) {
-            if (check(pk_vec_copy(c.tmp, c.warm))) return;
-            c.has_warm = true;
+         if ((flux && flux[r] && check(pk_vec_get_values(c.tmp, flux[r]))) ||
+             (conv && check(pk_vec_copy(c.tmp, c.warm))) || check(pk_pool_release(S->pool, slot))) {
+            join_all();
+            return;
          }
-         if (check(pk_pool_release(S->pool, slot))) return;
+         if (conv) c.has_warm = true;
          slot_req[slot] = -1;
          live--;
          done++;
@@ -339,6 +490,7 @@ void pampa_serve_time_spmv(pampa_server S, int reps, double* seconds, double* by
    };
    for (auto& cp : S->cases) {
       Case& c = *cp;
+      if (c.sn) continue;
       int slot = -1;
       if (S->requestOperators(c, 0, 0.0)) return cleanup();
       int rc = pk_pool_admit(S->pool, c.R, c.F, nullptr, &slot);
@@ -360,6 +512,11 @@ void pampa_serve_time_spmv(pampa_server S, int reps, double* seconds, double* by
    *error = 0;
 }
 
+void pampa_serve_set_sn_workers(pampa_server S, int workers, int* error) {
+   *error = (S && workers >= 1) ? 0 : 1;
+   if (!*error) S->sn_workers = workers;
+}
+
 void pampa_serve_set_progress(pampa_server S, double seconds, int* error) {
    *error = S ? 0 : 1;
    if (S) S->progress = seconds;
@@ -368,7 +525,11 @@ void pampa_serve_set_progress(pampa_server S, double seconds, int* error) {
 void pampa_serve_forget_warm(pampa_server S, int* error) {
    *error = S ? 0 : 1;
    if (S)
-      for (auto& c : S->cases) c->has_warm = false;
+      for (auto& c : S->cases) {
+         c->has_warm = false;
+         for (auto& w : c->workers)
+            if (pk_sn_reset(w->sn)) *error = 1;
+      }
 }
 
 void pampa_serve_stats(pampa_server S, double* seconds, long* ticks, int* max_live, double* pool_bytes,

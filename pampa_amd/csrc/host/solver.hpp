@@ -144,6 +144,8 @@ class SNSolver : public NeutronicSolver {
                    double* val) const override;
    int numDirections() const override { return M; }
    int timeKernel(int reps, int fp32, double* seconds, double* bytes) override;
+   // the sweep handle (null when the input is not sweepable)
+   pk_sn sweepHandle() const { return sweepable ? sn : nullptr; }
 
   protected:
    int readKeyword(const std::vector<std::string>& line, std::istream& in) override;

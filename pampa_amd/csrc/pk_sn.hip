@@ -466,6 +466,22 @@ __global__ void sn_fill_kernel(T* x, long n, T v) {
 using namespace pk;
 
 struct pk_sn_s {
+   // execution context of this handle: its own stream, reduction scratch and
+   // pinned scalars, so that solves of different handles can run concurrently
+   // (from different host threads) with each other and with the library stream
+   struct {
+      hipStream_t stream = nullptr;
+      double* partials = nullptr;  // [kMaxRedBlocks * 2]
+      unsigned* ticket = nullptr;
+      double* host_scalars = nullptr;  // pinned [16]
+   } sc;
+   ~pk_sn_s() {
+      if (sc.stream) (void)hipStreamSynchronize(sc.stream);
+      if (sc.partials) (void)hipFree(sc.partials);
+      if (sc.ticket) (void)hipFree(sc.ticket);
+      if (sc.host_scalars) (void)hipHostFree(sc.host_scalars);
+      if (sc.stream) (void)hipStreamDestroy(sc.stream);
+   }
    pk_mesh_s* mesh = nullptr;
    pk_xs_s* xs = nullptr;
    int C = 0, G = 0, M = 0;
@@ -487,7 +503,7 @@ struct pk_sn_s {
    std::vector<double> h_dirs;
    // records of one precision; slots = local-memory value slots they use
    struct RecBufs {
-      DBuf<int> nb, up;
+      DBuf<int> nb, up, slot_of;
       DBuf<double> den64, cw64;
       DBuf<float> den32, cw32;
       int slots = 0;
@@ -520,6 +536,7 @@ static SnGeom geom(pk_sn_s* S) {
 }
 
 static double sweep_bytes(const pk_sn_s* S, double s);
+static int sn_prepare(pk_sn_s* S);
 
 template <class T>
 static SnRec<T> recs(pk_sn_s* S);
@@ -536,12 +553,11 @@ SnRec<float> recs<float>(pk_sn_s* S) {
 
 // Local-memory bytes a workgroup may use (the device's per-workgroup limit).
 static int sn_lds_limit() {
-   static int lim = -1;
-   if (lim < 0) {
+   static const int lim = [] {
       int v = 0;
       if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx().device) != hipSuccess) v = 0;
-      lim = v > 0 ? v : 65536;
-   }
+      return v > 0 ? v : 65536;
+   }();
    return lim;
 }
 
@@ -639,38 +655,48 @@ static bool sn_plan(const pk_sn_s* S, int& gt, int& P, int& D) {
 // records of precision T (with their value slots), built once
 template <class T>
 static int sn_records(pk_sn_s* S) {
-   Ctx& c = ctx();
+   auto& c = S->sc;
    const long n = (long)S->M * S->C;
    const bool f32 = sizeof(T) == sizeof(float);
    auto& B = f32 ? S->rec32 : S->rec64;
    if (B.built) return 0;
-   PK_REQUIRE(!B.nb.alloc(n) && !B.up.alloc((size_t)kMaxIn * n), "device allocation failed");
-   if (f32)
-      PK_REQUIRE(!B.den32.alloc((size_t)S->G * n) && !B.cw32.alloc((size_t)kMaxIn * n), "device allocation failed");
-   else
-      PK_REQUIRE(!B.den64.alloc((size_t)S->G * n) && !B.cw64.alloc((size_t)kMaxIn * n), "device allocation failed");
-   if (!S->rec_err.p) PK_REQUIRE(!S->rec_err.alloc(1), "device allocation failed");
-   // slots sized for the per-direction kernel's groups per workgroup; when some
-   // value gets no slot there, sized for the one-item kernel (all G groups)
-   int gt = S->G, P = 1, D = 1;
-   const bool fast = sn_plan<T>(S, gt, P, D);
-   if (!fast) gt = S->G;
-   auto cap_for = [&](int groups) {
-      return std::max(1, std::min(sn_lds_limit() / (groups * (int)sizeof(T)), (1 << 27)));
-   };
-   std::vector<int> slot_of;
-   B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
-   if (!B.complete && gt != S->G) {
-      gt = S->G;
-      B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
+   // first build: the arrays and the value slots (geometry only, kept for
+   // rebuilds after a cross-section change, which rerun the record kernel only)
+   if (!B.nb.p) {
+      PK_REQUIRE(!B.nb.alloc(n) && !B.up.alloc((size_t)kMaxIn * n), "device allocation failed");
+      if (f32)
+         PK_REQUIRE(!B.den32.alloc((size_t)S->G * n) && !B.cw32.alloc((size_t)kMaxIn * n),
+                    "device allocation failed");
+      else
+         PK_REQUIRE(!B.den64.alloc((size_t)S->G * n) && !B.cw64.alloc((size_t)kMaxIn * n),
+                    "device allocation failed");
    }
-   B.gt = gt;
-   B.P = P;
-   B.D = D;
-   B.fast = fast && B.complete && gt <= 4;
-   DBuf<int> dslot;
-   PK_REQUIRE(!dslot.alloc(slot_of.size()), "device allocation failed");
-   PK_HIP(hipMemcpy(dslot.p, slot_of.data(), sizeof(int) * slot_of.size(), hipMemcpyHostToDevice));
+   if (!S->rec_err.p) PK_REQUIRE(!S->rec_err.alloc(1), "device allocation failed");
+   if (!B.slot_of.p) {
+      // slots sized for the per-direction kernel's groups per workgroup; when
+      // some value gets no slot there, sized for the one-item kernel (all G)
+      int gt = S->G, P = 1, D = 1;
+      const bool fast = sn_plan<T>(S, gt, P, D);
+      if (!fast) gt = S->G;
+      auto cap_for = [&](int groups) {
+         return std::max(1, std::min(sn_lds_limit() / (groups * (int)sizeof(T)), (1 << 27)));
+      };
+      std::vector<int> slot_of;
+      B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
+      if (!B.complete && gt != S->G) {
+         gt = S->G;
+         B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
+      }
+      B.gt = gt;
+      B.P = P;
+      B.D = D;
+      B.fast = fast && B.complete && gt <= 4;
+      PK_REQUIRE(!B.slot_of.alloc(slot_of.size()), "device allocation failed");
+      PK_HIP(hipMemcpyAsync(B.slot_of.p, slot_of.data(), sizeof(int) * slot_of.size(), hipMemcpyHostToDevice,
+                            c.stream));
+      PK_HIP(hipStreamSynchronize(c.stream));
+   }
+   DBuf<int>& dslot = B.slot_of;
    PK_HIP(hipMemsetAsync(S->rec_err.p, 0, sizeof(int), c.stream));
    const T* sigt = f32 ? (const T*)S->sigt32.p : (const T*)S->sigt64.p;
    sn_rec_kernel<T><<<grid_for(n), kBlock, 0, c.stream>>>(S->ditems.p, n, S->C, S->G, geom(S), sigt, dslot.p,
@@ -686,16 +712,13 @@ static int sn_records(pk_sn_s* S) {
 
 template <class T, int GT, int P, int D>
 static int sn_launch_lds(pk_sn_s* S, const T* qp, T* psi) {
-   Ctx& c = ctx();
+   auto& c = S->sc;
    constexpr int NT = 1024;
    const auto& B = sizeof(T) == sizeof(float) ? S->rec32 : S->rec64;
    const size_t shm = (size_t)B.slots * GT * sizeof(T);
-   static bool attr = false;
-   if (!attr) {
-      PK_HIP(hipFuncSetAttribute((const void*)sn_sweep_lds_kernel<T, GT, NT, P, D>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit()));
-      attr = true;
-   }
+   static const hipError_t attr = hipFuncSetAttribute((const void*)sn_sweep_lds_kernel<T, GT, NT, P, D>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit());
+   PK_HIP(attr);
    PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_kernel<T, GT, NT, P, D>),
                  S->M * (S->G / GT), NT, shm, c.stream, S->G, S->M, S->lstride, S->dlptr.p, S->dnlev.p, recs<T>(S),
                  qp, psi);
@@ -704,16 +727,13 @@ static int sn_launch_lds(pk_sn_s* S, const T* qp, T* psi) {
 
 template <class T>
 static int sn_launch_lds_any(pk_sn_s* S, const T* qp, T* psi) {
-   Ctx& c = ctx();
+   auto& c = S->sc;
    constexpr int NT = 1024;
    const auto& B = sizeof(T) == sizeof(float) ? S->rec32 : S->rec64;
    const size_t shm = (size_t)B.slots * S->G * sizeof(T);
-   static bool attr = false;
-   if (!attr) {
-      PK_HIP(hipFuncSetAttribute((const void*)sn_sweep_lds_any_kernel<T, NT>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit()));
-      attr = true;
-   }
+   static const hipError_t attr = hipFuncSetAttribute((const void*)sn_sweep_lds_any_kernel<T, NT>,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit());
+   PK_HIP(attr);
    PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_any_kernel<T, NT>), S->M,
                  NT, shm, c.stream, S->G, S->lstride, S->dlptr.p, S->dnlev.p, recs<T>(S), qp, psi);
    return 0;
@@ -723,7 +743,7 @@ static int sn_launch_lds_any(pk_sn_s* S, const T* qp, T* psi) {
 // geometry set, one launch per global level step of the geometry kernel).
 template <class T>
 static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
-   Ctx& c = ctx();
+   auto& c = S->sc;
    if (S->geometry) {
       SnGeom g = geom(S);
       const double per_item = profiling() ? sweep_bytes(S, sizeof(T)) / (double)S->step_ptr[S->nsteps] : 0.0;
@@ -808,6 +828,18 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
                     "sweeps need vacuum boundaries (reflective boundaries couple directions)");
       }
    auto* S = new pk_sn_s();
+   {
+      auto& sc = S->sc;
+      if (hipStreamCreateWithFlags(&sc.stream, hipStreamNonBlocking) != hipSuccess ||
+          hipMalloc(&sc.partials, sizeof(double) * kMaxRedBlocks * 2) != hipSuccess ||
+          hipMalloc(&sc.ticket, sizeof(unsigned) * 4) != hipSuccess ||
+          hipHostMalloc(&sc.host_scalars, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess ||
+          hipMemsetAsync(sc.ticket, 0, sizeof(unsigned) * 4, sc.stream) != hipSuccess ||
+          hipStreamSynchronize(sc.stream) != hipSuccess) {
+         delete S;
+         PK_FAIL("unable to set up the sweep's stream and scratch");
+      }
+   }
    S->mesh = mesh;
    S->xs = xs;
    S->C = C;
@@ -967,6 +999,68 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
    return 0;
 }
 
+int pk_sn_set_xs(pk_sn S, const double* sigt, const double* sigs, const double* nusf, const double* kapf,
+                 const double* chi) {
+   PK_REQUIRE(S && sigt && sigs && nusf && kapf && chi, "null argument");
+   pk_xs_s* xs = S->xs;
+   auto& c = S->sc;
+   const size_t n1 = (size_t)xs->nm * xs->G, n2 = n1 * xs->G;
+   xs->h_sigt.assign(sigt, sigt + n1);
+   xs->h_sigs.assign(sigs, sigs + n2);
+   xs->h_nusf.assign(nusf, nusf + n1);
+   xs->h_kapf.assign(kapf, kapf + n1);
+   xs->h_chi.assign(chi, chi + n1);
+   // on the handle's stream, after its own kernels that read the tables
+   PK_HIP(hipMemcpyAsync(xs->sigt.p, xs->h_sigt.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->sigs.p, xs->h_sigs.data(), sizeof(double) * n2, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->nusf.p, xs->h_nusf.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->kapf.p, xs->h_kapf.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(xs->chi.p, xs->h_chi.data(), sizeof(double) * n1, hipMemcpyHostToDevice, c.stream));
+   const int nm = (int)n1;
+   if (S->sigt64.p)
+      PK_HIP(hipMemcpyAsync(S->sigt64.p, xs->h_sigt.data(), sizeof(double) * nm, hipMemcpyHostToDevice, c.stream));
+   std::vector<float> st(xs->h_sigt.begin(), xs->h_sigt.end());
+   if (S->sigt32.p) PK_HIP(hipMemcpyAsync(S->sigt32.p, st.data(), sizeof(float) * nm, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   // the records hold Sigma_t V: rebuilt on the next sweep
+   S->rec32.built = false;
+   S->rec64.built = false;
+   return 0;
+}
+
+int pk_sn_prepare(pk_sn S) {
+   PK_REQUIRE(S, "null sweep");
+   PK_CALL(sn_prepare(S));
+   // one sweep of a zero source: records, value slots and kernel attributes in
+   // place, so that later solves allocate nothing
+   auto& c = S->sc;
+   const long CG = (long)S->C * S->G;
+   if (S->precision == PK_PREC_MIXED) {
+      sn_fill_kernel<float><<<grid_for(CG), kBlock, 0, c.stream>>>(S->q32.p, CG, 0.0f);
+      PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
+   } else {
+      sn_fill_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->q64.p, CG, 0.0);
+      PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
+   }
+   PK_HIP(hipGetLastError());
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
+}
+
+int pk_sn_get_moment(pk_sn S, double* phiw) {
+   PK_REQUIRE(S && phiw, "null argument");
+   auto& c = S->sc;
+   PK_HIP(hipMemcpyAsync(phiw, S->phiw.p, sizeof(double) * S->C * S->G, hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
+}
+
+int pk_sn_reset(pk_sn S) {
+   PK_REQUIRE(S, "null sweep");
+   S->has_state = false;
+   return 0;
+}
+
 int pk_sn_get_schedule(pk_sn S, int* num_levels, int* level_of_cell) {
    PK_REQUIRE(S, "null sweep");
    if (num_levels) std::copy(S->nlev.begin(), S->nlev.end(), num_levels);
@@ -988,7 +1082,12 @@ int pk_sn_set_tolerances(pk_sn S, double tol_k, double tol_phi, int max_outer) {
    return 0;
 }
 
+// Buffers of the current precision (allocated once).  Every copy of the
+// sweep handle goes through its own stream: no operation on the legacy NULL
+// stream, which would synchronise with (and break a graph capture on) the
+// library stream of another thread.
 static int sn_prepare(pk_sn_s* S) {
+   auto& c = S->sc;
    const long CGM = (long)S->C * S->G * S->M, CG = (long)S->C * S->G;
    const int nm = S->xs->nm * S->G;
    if (S->precision == PK_PREC_MIXED) {
@@ -996,13 +1095,16 @@ static int sn_prepare(pk_sn_s* S) {
          PK_REQUIRE(!S->psi32.alloc(CGM) && !S->q32.alloc(CG) && !S->sigt32.alloc(nm) && !S->qp32.alloc(CGM),
                     "device allocation failed");
          std::vector<float> st(S->xs->h_sigt.begin(), S->xs->h_sigt.end());
-         PK_HIP(hipMemcpy(S->sigt32.p, st.data(), sizeof(float) * nm, hipMemcpyHostToDevice));
+         PK_HIP(hipMemcpyAsync(S->sigt32.p, st.data(), sizeof(float) * nm, hipMemcpyHostToDevice, c.stream));
+         PK_HIP(hipStreamSynchronize(c.stream));
       }
    } else {
       if (!S->psi64.p) {
          PK_REQUIRE(!S->psi64.alloc(CGM) && !S->q64.alloc(CG) && !S->sigt64.alloc(nm) && !S->qp64.alloc(CGM),
                     "device allocation failed");
-         PK_HIP(hipMemcpy(S->sigt64.p, S->xs->h_sigt.data(), sizeof(double) * nm, hipMemcpyHostToDevice));
+         PK_HIP(hipMemcpyAsync(S->sigt64.p, S->xs->h_sigt.data(), sizeof(double) * nm, hipMemcpyHostToDevice,
+                               c.stream));
+         PK_HIP(hipStreamSynchronize(c.stream));
       }
    }
    return 0;
@@ -1011,33 +1113,36 @@ static int sn_prepare(pk_sn_s* S) {
 int pk_sn_sweep_host(pk_sn S, const double* q, double* psi, double* phiw) {
    PK_REQUIRE(S && q, "null argument");
    PK_CALL(sn_prepare(S));
-   Ctx& c = ctx();
+   auto& c = S->sc;
    const long CG = (long)S->C * S->G, CGM = CG * S->M;
    DBuf<double> out;
    PK_REQUIRE(!out.alloc(CGM), "device allocation failed");
    if (S->precision == PK_PREC_MIXED) {
       std::vector<float> qf(q, q + CG);
-      PK_HIP(hipMemcpy(S->q32.p, qf.data(), sizeof(float) * CG, hipMemcpyHostToDevice));
+      PK_HIP(hipMemcpyAsync(S->q32.p, qf.data(), sizeof(float) * CG, hipMemcpyHostToDevice, c.stream));
+      PK_HIP(hipStreamSynchronize(c.stream));
       PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
       sn_moment<float>(S->psi32.p, S->dpos.p, S->wts.p, S->M, S->C, S->G, S->phiw.p, c.stream);
       sn_to_reference_kernel<float><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi32.p, S->dpos.p, S->C, S->G, S->M, 1.0, out.p);
    } else {
-      PK_HIP(hipMemcpy(S->q64.p, q, sizeof(double) * CG, hipMemcpyHostToDevice));
+      PK_HIP(hipMemcpyAsync(S->q64.p, q, sizeof(double) * CG, hipMemcpyHostToDevice, c.stream));
+      PK_HIP(hipStreamSynchronize(c.stream));
       PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
       sn_moment<double>(S->psi64.p, S->dpos.p, S->wts.p, S->M, S->C, S->G, S->phiw.p, c.stream);
       sn_to_reference_kernel<double><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi64.p, S->dpos.p, S->C, S->G, S->M, 1.0, out.p);
    }
    PK_HIP(hipGetLastError());
    PK_HIP(hipStreamSynchronize(c.stream));
-   if (psi) PK_HIP(hipMemcpy(psi, out.p, sizeof(double) * CGM, hipMemcpyDeviceToHost));
-   if (phiw) PK_HIP(hipMemcpy(phiw, S->phiw.p, sizeof(double) * CG, hipMemcpyDeviceToHost));
+   if (psi) PK_HIP(hipMemcpyAsync(psi, out.p, sizeof(double) * CGM, hipMemcpyDeviceToHost, c.stream));
+   if (phiw) PK_HIP(hipMemcpyAsync(phiw, S->phiw.p, sizeof(double) * CG, hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
    return 0;
 }
 
 int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
    PK_REQUIRE(S && keff, "null argument");
    PK_CALL(sn_prepare(S));
-   Ctx& c = ctx();
+   auto& c = S->sc;
    auto t0 = std::chrono::steady_clock::now();
    const int C = S->C, G = S->G, M = S->M;
    const long CG = (long)C * G, CGM = CG * M;
@@ -1127,7 +1232,7 @@ int pk_sn_get_stats(pk_sn S, int* outer, long* sweeps, double* seconds) {
 int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_sweep) {
    PK_REQUIRE(S && reps > 0, "wrong arguments");
    PK_CALL(sn_prepare(S));
-   Ctx& c = ctx();
+   auto& c = S->sc;
    const long CG = (long)S->C * S->G;
    const bool f32 = S->precision == PK_PREC_MIXED;
    if (f32)
@@ -1157,7 +1262,6 @@ int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_s
 
 int pk_sn_destroy(pk_sn* S) {
    if (!S || !*S) return 0;
-   (void)hipStreamSynchronize(ctx().stream);
    delete *S;
    *S = nullptr;
    return 0;
