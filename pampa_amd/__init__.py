@@ -121,7 +121,10 @@ def _declare(L):
     for name, args in {"pk_sn_get_schedule": [_P, _P, _P], "pk_sn_set_precision": [_P, _I],
                        "pk_sn_set_tolerances": [_P, _D, _D, _I], "pk_sn_sweep_host": [_P, _P, _P, _P],
                        "pk_sn_solve": [_P, _P, _P, _P], "pk_sn_get_stats": [_P, _P, _P, _P],
-                       "pk_sn_time_sweep": [_P, _I, _P, _P], "pk_sn_destroy": [_PP]}.items():
+                       "pk_sn_time_sweep": [_P, _I, _P, _P], "pk_sn_destroy": [_PP],
+                       "pk_sn_set_acceleration": [_P, _I], "pk_sn_reset": [_P], "pk_sn_get_moment": [_P, _P],
+                       "pk_sn_prepare": [_P],
+                       "pk_sn_set_xs": [_P, _P, _P, _P, _P, _P]}.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = _I
@@ -666,3 +669,19 @@ class Sweep(Handle):
         t, b = ct.c_double(), ct.c_double()
         check(lib().pk_sn_time_sweep(self.h, reps, ct.byref(t), ct.byref(b)))
         return t.value, b.value
+
+    def solve(self, accel=5, tol_k=1e-9, tol_phi=1e-8, max_outer=100000, cold=True):
+        """k-eigenvalue solve (source iteration, Anderson depth `accel`):
+        (keff, moment sum_m w_m psi [C*G], outer iterations, sweeps)."""
+        L = lib()
+        check(L.pk_sn_set_acceleration(self.h, int(accel)))
+        check(L.pk_sn_set_tolerances(self.h, tol_k, tol_phi, max_outer))
+        if cold:
+            check(L.pk_sn_reset(self.h))
+        k = ct.c_double()
+        check(L.pk_sn_solve(self.h, ct.byref(k), None, None))
+        phiw = np.zeros(self.C * self.G)
+        check(L.pk_sn_get_moment(self.h, ptr(phiw)))
+        o, sw = ct.c_int(), ct.c_long()
+        check(L.pk_sn_get_stats(self.h, ct.byref(o), ct.byref(sw), None))
+        return k.value, phiw, o.value, sw.value

@@ -442,6 +442,66 @@ __global__ void __launch_bounds__(kBlock) sn_update_kernel(long CG, double* phiw
 }
 
 // psi_ref[i][g][m] = scale * psi[m][i][g]
+// Anderson acceleration of the fixed-point map phi -> g(phi) (one source
+// iteration, production-normalised): with residuals f = g - phi and the last
+// columns df = f_n - f_{n-1}, dg = g_n - g_{n-1} (a ring of depth slots), the
+// next iterate is g_n - sum_j gamma_j dg_j with gamma minimising
+// |f_n - sum_j gamma_j df_j| (Walker and Ni, SIAM J. Numer. Anal. 49 (2011)).
+// The fixed point, so the eigenpair, is that of the plain iteration.
+constexpr int kSnAAMax = 6;
+
+// f = g - phi; when have_prev: df = f - fprev, dg = g - gprev; then fprev = f, gprev = g
+__global__ void __launch_bounds__(kBlock) sn_aa_diff_kernel(long n, const double* __restrict__ g,
+                                                            const double* __restrict__ phi, double* fprev,
+                                                            double* gprev, double* df, double* dg, int have_prev) {
+   const long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (l >= n) return;
+   const double f = g[l] - phi[l], gl = g[l];
+   if (have_prev) {
+      df[l] = f - fprev[l];
+      dg[l] = gl - gprev[l];
+   }
+   fprev[l] = f;
+   gprev[l] = gl;
+}
+
+// out[j] = <df_j, df_slot>, out[depth + j] = <df_j, f> for every slot j (FP64,
+// deterministic grid reduction)
+__global__ void __launch_bounds__(kBlock) sn_aa_dots_kernel(long n, int depth, const double* __restrict__ df,
+                                                            int slot, const double* __restrict__ f,
+                                                            double* partials, unsigned* ticket, double* out) {
+   double v[2 * kSnAAMax];
+#pragma unroll
+   for (int j = 0; j < 2 * kSnAAMax; j++) v[j] = 0.0;
+   for (long l = blockIdx.x * (long)kBlock + threadIdx.x; l < n; l += (long)gridDim.x * kBlock) {
+      const double a = df[(long)slot * n + l], b = f[l];
+#pragma unroll
+      for (int j = 0; j < kSnAAMax; j++) {
+         if (j < depth) {
+            const double d = df[(long)j * n + l];
+            v[j] += d * a;
+            v[kSnAAMax + j] += d * b;
+         }
+      }
+   }
+   (void)grid_reduce<2 * kSnAAMax>(
+       2 * kSnAAMax, [&](int k) { return v[k]; }, partials, ticket, out);
+}
+
+struct SnAACoef {
+   double g[kSnAAMax];
+};
+
+// phi = g - sum_j gamma_j dg_j (in place on g)
+__global__ void __launch_bounds__(kBlock) sn_aa_mix_kernel(long n, int depth, double* phi,
+                                                           const double* __restrict__ dg, SnAACoef c) {
+   const long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (l >= n) return;
+   double v = phi[l];
+   for (int j = 0; j < depth; j++) v -= c.g[j] * dg[(long)j * n + l];
+   phi[l] = v;
+}
+
 template <class T>
 __global__ void sn_to_reference_kernel(const T* psi, const int* __restrict__ pos, int C, int G, int M, double scale,
                                        double* out) {
@@ -471,9 +531,9 @@ struct pk_sn_s {
    // (from different host threads) with each other and with the library stream
    struct {
       hipStream_t stream = nullptr;
-      double* partials = nullptr;  // [kMaxRedBlocks * 2]
+      double* partials = nullptr;  // [kMaxRedBlocks * 2 kSnAAMax]
       unsigned* ticket = nullptr;
-      double* host_scalars = nullptr;  // pinned [16]
+      double* host_scalars = nullptr;  // pinned [32]
    } sc;
    ~pk_sn_s() {
       if (sc.stream) (void)hipStreamSynchronize(sc.stream);
@@ -522,6 +582,9 @@ struct pk_sn_s {
    // value accesses of one sweep: per (i,g,m) one psi write plus one upwind psi
    // read per incoming interior face, per (i,g) one source read and one moment write
    double value_units = 0.0;
+   // Anderson acceleration: depth (0 = plain source iteration) and history
+   int aa_depth = 5;
+   DBuf<double> aa_df, aa_dg, aa_f, aa_g;  // [depth][C G] x 2, [C G] x 2
    // state kept between solves (warm start)
    bool has_state = false;
    double keff = 1.0;
@@ -831,9 +894,9 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
    {
       auto& sc = S->sc;
       if (hipStreamCreateWithFlags(&sc.stream, hipStreamNonBlocking) != hipSuccess ||
-          hipMalloc(&sc.partials, sizeof(double) * kMaxRedBlocks * 2) != hipSuccess ||
+          hipMalloc(&sc.partials, sizeof(double) * kMaxRedBlocks * 2 * kSnAAMax) != hipSuccess ||
           hipMalloc(&sc.ticket, sizeof(unsigned) * 4) != hipSuccess ||
-          hipHostMalloc(&sc.host_scalars, sizeof(double) * 16, hipHostMallocDefault) != hipSuccess ||
+          hipHostMalloc(&sc.host_scalars, sizeof(double) * 32, hipHostMallocDefault) != hipSuccess ||
           hipMemsetAsync(sc.ticket, 0, sizeof(unsigned) * 4, sc.stream) != hipSuccess ||
           hipStreamSynchronize(sc.stream) != hipSuccess) {
          delete S;
@@ -973,7 +1036,7 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
    e |= S->items.alloc(items.size()) | S->ditems.alloc(ditems.size()) | S->dlptr.alloc(dlptr.size()) |
         S->dnlev.alloc(M);
    e |= S->dirs.alloc(3L * M) | S->wts.alloc(M) | S->coef.alloc(mesh->F ? mesh->F : 1);
-   e |= S->phiw.alloc((size_t)C * G) | S->old.alloc((size_t)C * G) | S->st.alloc(8);
+   e |= S->phiw.alloc((size_t)C * G) | S->old.alloc((size_t)C * G) | S->st.alloc(8 + 2 * kSnAAMax);
    if (e) {
       delete S;
       PK_FAIL("device allocation failed");
@@ -1055,6 +1118,12 @@ int pk_sn_get_moment(pk_sn S, double* phiw) {
    return 0;
 }
 
+int pk_sn_set_acceleration(pk_sn S, int depth) {
+   PK_REQUIRE(S && depth >= 0 && depth <= kSnAAMax, "wrong acceleration depth");
+   S->aa_depth = depth;
+   return 0;
+}
+
 int pk_sn_reset(pk_sn S) {
    PK_REQUIRE(S, "null sweep");
    S->has_state = false;
@@ -1089,6 +1158,10 @@ int pk_sn_set_tolerances(pk_sn S, double tol_k, double tol_phi, int max_outer) {
 static int sn_prepare(pk_sn_s* S) {
    auto& c = S->sc;
    const long CGM = (long)S->C * S->G * S->M, CG = (long)S->C * S->G;
+   if (S->aa_depth > 0 && !S->aa_df.p)
+      PK_REQUIRE(!S->aa_df.alloc((size_t)kSnAAMax * CG) && !S->aa_dg.alloc((size_t)kSnAAMax * CG) &&
+                     !S->aa_f.alloc(CG) && !S->aa_g.alloc(CG),
+                 "device allocation failed");
    const int nm = S->xs->nm * S->G;
    if (S->precision == PK_PREC_MIXED) {
       if (!S->psi32.p) {
@@ -1166,6 +1239,14 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
    int it = 0;
    S->sweeps = 0;
    bool conv = false;
+   // Anderson history: ncol difference columns in a ring of `depth` slots (head
+   // = next slot), their Gram matrix H on the host; have_prev once f, g of an
+   // earlier iterate are stored
+   const int depth = std::min(S->aa_depth, kSnAAMax);
+   int ncol = 0, head = 0;
+   bool have_prev = false;
+   double H[kSnAAMax][kSnAAMax] = {};
+   double fbest = 0.0;
    for (it = 0; it < S->max_outer; it++) {
       PK_HIP(hipMemcpyAsync(S->old.p, S->phiw.p, sizeof(double) * CG, hipMemcpyDeviceToDevice, c.stream));
       if (S->precision == PK_PREC_MIXED) {
@@ -1183,14 +1264,26 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
       sn_prod_kernel<<<gb, kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->nusf.p, S->phiw.p,
                                                  c.partials, c.ticket, st + 2);
       sn_update_kernel<<<gb, kBlock, 0, c.stream>>>(CG, S->phiw.p, S->old.p, st, c.partials, c.ticket, st + 3);
+      // phiw = g_n (this iterate's image), old = phi_n
+      const int slot = head;
+      if (depth > 0) {
+         sn_aa_diff_kernel<<<grid_for(CG), kBlock, 0, c.stream>>>(CG, S->phiw.p, S->old.p, S->aa_f.p, S->aa_g.p,
+                                                                S->aa_df.p + (long)slot * CG,
+                                                                S->aa_dg.p + (long)slot * CG, have_prev ? 1 : 0);
+         if (have_prev)
+            sn_aa_dots_kernel<<<gb, kBlock, 0, c.stream>>>(CG, depth, S->aa_df.p, slot, S->aa_f.p, c.partials,
+                                                          c.ticket, st + 8);
+      }
       PK_HIP(hipGetLastError());
-      PK_HIP(hipMemcpyAsync(c.host_scalars, st, sizeof(double) * 5, hipMemcpyDeviceToHost, c.stream));
+      PK_HIP(hipMemcpyAsync(c.host_scalars, st, sizeof(double) * (depth > 0 ? 8 + 2 * kSnAAMax : 5),
+                            hipMemcpyDeviceToHost, c.stream));
       PK_HIP(hipStreamSynchronize(c.stream));
-         double prod_old = c.host_scalars[1], prod_new = c.host_scalars[2];
+      double prod_old = c.host_scalars[1], prod_new = c.host_scalars[2];
       PK_REQUIRE(prod_new > 0.0 && prod_old > 0.0, "zero fission source in the S_N eigen solve");
       double knew = k * prod_new / prod_old;
       last_scale = prod_old / prod_new;
-      double dphi = sqrt(c.host_scalars[3] / c.host_scalars[4]);
+      const double fnorm2 = c.host_scalars[3];
+      double dphi = sqrt(fnorm2 / c.host_scalars[4]);
       double dk = fabs(knew - k);
       k = knew;
       c.host_scalars[0] = k;  // prod stays prod_old after rescaling
@@ -1200,6 +1293,56 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
          conv = true;
          it++;
          break;
+      }
+      if (depth > 0) {
+         // restart the history when the residual grows well past its best
+         if (have_prev && fnorm2 > 100.0 * fbest) ncol = 0;
+         fbest = (have_prev && ncol > 0) ? std::min(fbest, fnorm2) : fnorm2;
+         if (have_prev) {
+            const double* v = c.host_scalars + 8;  // <df_j, df_slot>, then <df_j, f>
+            for (int j = 0; j < depth; j++) H[j][slot] = H[slot][j] = v[j];
+            head = (head + 1) % depth;
+            ncol = std::min(ncol + 1, depth);
+            // least squares over the ncol newest columns: (H + eps I) gamma = <df, f>
+            int idx[kSnAAMax];
+            for (int j = 0; j < ncol; j++) idx[j] = (slot - j + depth) % depth;
+            double A[kSnAAMax][kSnAAMax + 1];
+            double dmax = 0.0;
+            for (int a = 0; a < ncol; a++) dmax = std::max(dmax, H[idx[a]][idx[a]]);
+            for (int a = 0; a < ncol; a++) {
+               for (int b = 0; b < ncol; b++) A[a][b] = H[idx[a]][idx[b]];
+               A[a][a] += 1e-12 * dmax;
+               A[a][ncol] = v[kSnAAMax + idx[a]];
+            }
+            bool ok = dmax > 0.0;
+            for (int col = 0; ok && col < ncol; col++) {
+               int pr = col;
+               for (int r = col + 1; r < ncol; r++)
+                  if (fabs(A[r][col]) > fabs(A[pr][col])) pr = r;
+               if (fabs(A[pr][col]) <= 1e-300) {
+                  ok = false;
+                  break;
+               }
+               if (pr != col)
+                  for (int q = 0; q <= ncol; q++) std::swap(A[pr][q], A[col][q]);
+               for (int r = col + 1; r < ncol; r++) {
+                  const double fct = A[r][col] / A[col][col];
+                  for (int q = col; q <= ncol; q++) A[r][q] -= fct * A[col][q];
+               }
+            }
+            SnAACoef cf{};
+            for (int r = ncol - 1; ok && r >= 0; r--) {
+               double acc = A[r][ncol];
+               for (int q = r + 1; q < ncol; q++) acc -= A[r][q] * cf.g[idx[q]];
+               cf.g[idx[r]] = acc / A[r][r];
+               if (!(fabs(cf.g[idx[r]]) < 1e6)) ok = false;
+            }
+            if (ok)
+               sn_aa_mix_kernel<<<grid_for(CG), kBlock, 0, c.stream>>>(CG, depth, S->phiw.p, S->aa_dg.p, cf);
+            else
+               ncol = 0;
+         }
+         have_prev = true;
       }
    }
    PK_HIP(hipStreamSynchronize(c.stream));

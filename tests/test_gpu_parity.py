@@ -144,6 +144,25 @@ def test_sweep_schedule_and_single_sweep(pk, oracle_mod, tmp_path):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sn_anderson_same_eigenpair_fewer_sweeps(pk, oracle_mod, tmp_path, precision):
+    """Anderson-accelerated source iteration converges to the eigenpair of the
+    plain iteration (and of the FP64 oracle) in fewer sweeps."""
+    d, case, mesh, xs = _hex(oracle_mod, tmp_path, rings=3, layers=4, order=4, G=4)
+    prec = pk.PK_PREC_FP64 if precision == "fp64" else pk.PK_PREC_MIXED
+    sw, _, _ = _device_sweep(pk, oracle_mod, mesh, xs, case, prec)
+    k0, phi0, o0, s0 = sw.solve(accel=0)
+    k5, phi5, o5, s5 = sw.solve(accel=5)
+    k_o, _, phiw_o, _ = oracle_mod.sn_sweep_eigen(mesh, xs, case.solver, inner=1, tol_k=1e-12, tol_phi=1e-11)
+    shape = lambda v: np.asarray(v) / np.sum(v)
+    print(f"[case] S_N Anderson {precision}: plain {s0} sweeps, accelerated {s5} sweeps; "
+          f"dk {abs(k5 - k_o) * 1e5:.2e} pcm", flush=True)
+    for k, phi in ((k0, phi0), (k5, phi5)):
+        assert abs(k - k_o) * 1e5 <= DK_PCM
+        assert np.linalg.norm(shape(phi) - shape(phiw_o)) / np.linalg.norm(shape(phiw_o)) <= FLUX_L2
+    assert s5 < s0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_sn_eigen_parity(oracle_mod, tmp_path, precision):
     d, case, mesh, xs = _hex(oracle_mod, tmp_path, rings=2, layers=3, order=4, G=4)
     k_o, psi_o, phiw_o, its = oracle_mod.sn_sweep_eigen(mesh, xs, case.solver, inner=1,
