@@ -126,6 +126,43 @@ def cpu_baseline(d, target_seconds=15.0):
         os.chdir(cwd)
 
 
+def sn_solve_numbers(pa, d, reps=3):
+    """Cold k-eigenvalue solves of the S6 hexagonal lattice through the drop-in
+    C-ABI (sweeps inside Anderson-accelerated source iteration), FP64 and mixed
+    (FP32 sweeps, FP64 finish): solves/s, sweeps per solve, and the mixed
+    eigenpair against the FP64 one.  (The FP64 path is checked against the
+    oracle on smaller lattices by the tests; the oracle's CPU solve of 61 M
+    angular unknowns takes minutes.)"""
+    cwd = os.getcwd()
+    os.chdir(d)
+    out, sol = {}, {}
+    try:
+        for name, prec in (("fp64", "fp64"), ("mixed", "mixed")):
+            with pa.DropIn("input.pmp", "-silent", "-pk_precision", prec) as run:
+                run.solve()  # records built, buffers allocated
+                times = []
+                for _ in range(reps):
+                    run.cold_start()
+                    pa.lib().pk_synchronize()
+                    t0 = time.perf_counter()
+                    run.solve()
+                    pa.lib().pk_synchronize()
+                    times.append(time.perf_counter() - t0)
+                k, _ = run.result()
+                sol[name] = (k, run.field("scalar-flux"))
+                st = run.stats()
+                out[name] = {"solves_per_s": len(times) / sum(times), "ms_per_solve": 1e3 * sum(times) / len(times),
+                             "sweeps": st["inner"], "keff": k}
+            log(f"[bench]   S6 lattice solve {name}: {out[name]['ms_per_solve']:.1f} ms, {out[name]['sweeps']} sweeps")
+    finally:
+        os.chdir(cwd)
+    (k64, p64), (k32, p32) = sol["fp64"], sol["mixed"]
+    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["mixed_vs_fp64"] = {"dk_pcm": abs(k32 - k64) * 1e5,
+                            "flux_l2": float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))}
+    return out
+
+
 def sweep_numbers(pa, d):
     cwd = os.getcwd()
     os.chdir(d)
@@ -380,7 +417,9 @@ def main():
         return
     if args.sweep_only:
         d2 = W.hex_sn_3d(os.path.join(tmp, "hex-s6"))
-        print(json.dumps(sweep_numbers(pa, d2)))
+        out = sweep_numbers(pa, d2)
+        out["eigen"] = sn_solve_numbers(pa, d2)
+        print(json.dumps(out))
         return
     if args.serving_only:
         import oracle
@@ -449,6 +488,8 @@ def main():
         log("[bench] S6 hex lattice sweep timing")
         sweep = sweep_numbers(pa, d2)
         sweep["workload"] = "hex lattice 331 columns x 24 triangles x 40 layers, G=4, S6"
+        log("[bench] S6 hex lattice eigen solves")
+        sweep["eigen"] = sn_solve_numbers(pa, d2)
 
     batched = serving = None
     if not args.no_batched:

@@ -49,6 +49,10 @@ void pampa_ext_get_operator(const char name[], long* n, long* nnz, int* row_ptr,
 /* Outer iterations, inner iterations (Krylov steps or sweeps), final residual
  * and device seconds of the last solve. */
 void pampa_ext_get_stats(int* outer, long* inner, double* residual, double* seconds, int* error);
+/* The next steady solve starts cold: the S_N solver forgets its previous
+ * solution (a diffusion solve starts from the scalar-flux field, which the
+ * caller sets). */
+void pampa_ext_cold_start(int* error);
 /* Number of cells and energy groups of the main solver's mesh (field sizes). */
 void pampa_ext_get_sizes(int* num_cells, int* num_groups, int* num_directions, int* error);
 /* Device time of the main solver's hot kernel, averaged over `reps` back-to-back

@@ -68,6 +68,10 @@ int pk_vec_axpy(pk_vec y, double a, pk_vec x);          /* VecAXPY */
 int pk_vec_dot(pk_vec x, pk_vec y, double* r);          /* VecDot (fixed-order FP64 reduction) */
 int pk_vec_norm2(pk_vec x, double* r);                  /* VecNorm(NORM_2) */
 int pk_vec_device_ptr(pk_vec v, void** dev);            /* device address (interop) */
+/* Entries in [-rel max|v|, 0) set to 0 (round-off of zeros of an iterative
+ * solution); *negatives = entries below -rel max|v| (the check of
+ * SNSolver.cxx:329 after that clean-up). */
+int pk_vec_clip_noise(pk_vec v, double rel, long* negatives);
 
 /* ---- matrices (petsc::create(Mat&) petsc.cxx:91-111, MatSetValues,
  *      MatAssemblyBegin/End).  CSR with PETSc-AIJ semantics: sorted columns,

@@ -95,7 +95,7 @@ def _declare(L):
         "pampa_get_field": [_P, ct.c_char_p, _IP], "pampa_set_field": [_P, ct.c_char_p, _IP],
         "pampa_ext_get_result": [_P, _P, _IP], "pampa_ext_get_operator": [ct.c_char_p, _P, _P, _P, _P, _P, _IP],
         "pampa_ext_get_stats": [_P, _P, _P, _P, _IP], "pampa_ext_get_sizes": [_P, _P, _P, _IP],
-        "pampa_ext_time_kernel": [_I, _I, _P, _P, _IP],
+        "pampa_ext_time_kernel": [_I, _I, _P, _P, _IP], "pampa_ext_cold_start": [_IP],
     }
     for name, args in {"pk_profile_enable": [_I], "pk_profile_reset": [],
                        "pk_profile_get": [ct.c_char_p, _P, _P, _P]}.items():
@@ -589,6 +589,13 @@ class DropIn:
         o, i, r, s, err = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double(), ct.c_int()
         self.L.pampa_ext_get_stats(ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s), ct.byref(err))
         return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
+
+    def cold_start(self):
+        """The next solve starts cold (pampa_ext_cold_start)."""
+        err = ct.c_int()
+        self.L.pampa_ext_cold_start(ct.byref(err))
+        if err.value:
+            raise PkError("pampa_ext_cold_start failed")
 
     def time_kernel(self, reps=50, fp32=False):
         s, b, err = ct.c_double(), ct.c_double(), ct.c_int()

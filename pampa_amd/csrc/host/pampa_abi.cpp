@@ -94,6 +94,11 @@ void pampa_ext_time_kernel(int reps, int fp32, double* seconds, double* bytes, i
    *error = s ? s->timeKernel(reps, fp32, seconds, bytes) : 1;
 }
 
+void pampa_ext_cold_start(int* error) {
+   pampa::Solver* s = g_driver.mainSolver();
+   *error = s ? s->coldStart() : 1;
+}
+
 void pampa_ext_get_sizes(int* num_cells, int* num_groups, int* num_directions, int* error) {
    pampa::Solver* s = g_driver.mainSolver();
    *error = s ? 0 : 1;

@@ -450,11 +450,11 @@ int SNSolver::solve(int n, double dt, double t) {
    output::indent(true);
    PAMPA_CHECK(n > 0, "SN transients are not supported");
    if (sweepable) {
-      pk_vec phiw = nullptr;
-      PK_CHECK(pk_vec_create((long)mesh->num_cells * G, &phiw));
+      // the moment sum_m w_m psi comes with the solve: phi = 4 pi phiw on the
+      // device (no host pass over the angular flux)
       double k = 1.0;
-      int rc = pk_sn_solve(sn, &k, psi, phiw);
-      pk_vec_destroy(&phiw);
+      int rc = pk_sn_solve(sn, &k, psi, phi);
+      if (!rc) rc = pk_vec_scale(phi, 4.0 * M_PI);
       if (rc) {
          report_error(__FILE__, __LINE__, __func__, pk_last_error());
          PAMPA_CHECK(true, "unable to solve the eigensystem");
@@ -485,7 +485,7 @@ int SNSolver::solve(int n, double dt, double t) {
       keff = 1.0 / lambda;
    }
    // phi = 4 pi sum w psi; phi scaled to the power; psi scaled with sum w psi kapf V
-   PAMPA_CHECK(scalarFluxFromAngular(), "unable to calculate the scalar flux");
+   if (!sweepable) PAMPA_CHECK(scalarFluxFromAngular(), "unable to calculate the scalar flux");
    long C = mesh->num_cells;
    std::vector<double> hphi(C * G);
    PK_CHECK(pk_vec_get_values(phi, hphi.data()));
@@ -505,19 +505,9 @@ int SNSolver::solve(int n, double dt, double t) {
       // sign; values within 10 x tol of the largest one are that noise floor
       // and are set to zero before the reference's negativity check
       // (SNSolver.cxx:329).
-      std::vector<double> hpsi(C * G * (long)M);
-      PK_CHECK(pk_vec_get_values(psi, hpsi.data()));
-      double vmax = 0.0;
-      for (double v : hpsi) vmax = std::max(vmax, std::fabs(v));
-      const double floor_ = 10.0 * options.eps_tol * vmax;
-      bool changed = false;
-      for (double& v : hpsi)
-         if (v < 0.0 && -v <= floor_) {
-            v = 0.0;
-            changed = true;
-         }
-      for (double v : hpsi) PAMPA_CHECK(v < 0.0, "negative values in the angular-flux solution");
-      if (changed) PK_CHECK(pk_vec_set_values(psi, hpsi.data()));
+      long negatives = 0;
+      PK_CHECK(pk_vec_clip_noise(psi, 10.0 * options.eps_tol, &negatives));
+      PAMPA_CHECK(negatives > 0, "negative values in the angular-flux solution");
    }
    output::outdent(true);
    output::print("Done.", true);

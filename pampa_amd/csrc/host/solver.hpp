@@ -61,6 +61,8 @@ class Solver {
    virtual int getStats(int* outer, long* inner, double* residual, double* seconds) const;
    virtual void getSizes(int* cells, int* groups, int* directions) const = 0;
    virtual int timeKernel(int reps, int fp32, double* seconds, double* bytes) = 0;
+   // forget any state a steady solve starts from (other than the fields)
+   virtual int coldStart() { return 0; }
 
   protected:
    std::vector<Field> fields;
@@ -146,6 +148,7 @@ class SNSolver : public NeutronicSolver {
    int timeKernel(int reps, int fp32, double* seconds, double* bytes) override;
    // the sweep handle (null when the input is not sweepable)
    pk_sn sweepHandle() const { return sweepable ? sn : nullptr; }
+   int coldStart() override { return sn ? pk_sn_reset(sn) : 0; }
 
   protected:
    int readKeyword(const std::vector<std::string>& line, std::istream& in) override;

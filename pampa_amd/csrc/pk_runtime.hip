@@ -142,6 +142,42 @@ __global__ void set_kernel(double* x, long n, double a) {
    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (i < n) x[i] = a;
 }
+// max |x| as the bit pattern of a non-negative double (the order of such bit
+// patterns is the order of the values, so the atomic max is exact and does not
+// depend on timing)
+__global__ void __launch_bounds__(kBlock) absmax_kernel(const double* __restrict__ x, long n,
+                                                        unsigned long long* out) {
+   double m = 0.0;
+   for (long l = blockIdx.x * (long)kBlock + threadIdx.x; l < n; l += (long)gridDim.x * kBlock)
+      m = fmax(m, fabs(x[l]));
+   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+   __shared__ double sh[kBlock / 64];
+   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      for (int w = 1; w < kBlock / 64; w++) m = fmax(m, sh[w]);
+      atomicMax(out, (unsigned long long)__double_as_longlong(m));
+   }
+}
+
+// x in [-rel max|x|, 0) -> 0; count the entries below -rel max|x|
+__global__ void __launch_bounds__(kBlock) clip_noise_kernel(double* x, long n, double rel,
+                                                            const unsigned long long* amax,
+                                                            unsigned long long* negatives) {
+   const double fl = rel * __longlong_as_double((long long)*amax);
+   unsigned long long cnt = 0;
+   for (long l = blockIdx.x * (long)kBlock + threadIdx.x; l < n; l += (long)gridDim.x * kBlock) {
+      const double v = x[l];
+      if (v < 0.0) {
+         if (-v <= fl)
+            x[l] = 0.0;
+         else
+            cnt++;
+      }
+   }
+   if (cnt) atomicAdd(negatives, cnt);
+}
+
 __global__ void scale_kernel(double* x, long n, double a) {
    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (i < n) x[i] *= a;
@@ -309,6 +345,25 @@ int pk_vec_scale(pk_vec v, double a) {
    if (v->n == 0) return 0;
    scale_kernel<<<grid_for(v->n), kBlock, 0, ctx().stream>>>(v->d.p, v->n, a);
    PK_HIP(hipGetLastError());
+   return 0;
+}
+
+int pk_vec_clip_noise(pk_vec v, double rel, long* negatives) {
+   PK_REQUIRE(v && rel >= 0.0, "wrong arguments");
+   Ctx& c = ctx();
+   if (negatives) *negatives = 0;
+   if (v->n == 0) return 0;
+   DBuf<unsigned long long> w;
+   PK_REQUIRE(!w.alloc(2), "device allocation failed");
+   PK_HIP(hipMemsetAsync(w.p, 0, sizeof(unsigned long long) * 2, c.stream));
+   const int gb = grid_for(v->n, kBlock, 4 * kMaxRedBlocks);
+   absmax_kernel<<<gb, kBlock, 0, c.stream>>>(v->d.p, v->n, w.p);
+   clip_noise_kernel<<<gb, kBlock, 0, c.stream>>>(v->d.p, v->n, rel, w.p, w.p + 1);
+   PK_HIP(hipGetLastError());
+   unsigned long long h = 0;
+   PK_HIP(hipMemcpyAsync(&h, w.p + 1, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   if (negatives) *negatives = (long)h;
    return 0;
 }
 

@@ -498,7 +498,9 @@ __global__ void __launch_bounds__(kBlock) sn_aa_mix_kernel(long n, int depth, do
    const long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (l >= n) return;
    double v = phi[l];
-   for (int j = 0; j < depth; j++) v -= c.g[j] * dg[(long)j * n + l];
+   // slots outside the window have gamma 0 and may hold no values yet
+   for (int j = 0; j < depth; j++)
+      if (c.g[j] != 0.0) v -= c.g[j] * dg[(long)j * n + l];
    phi[l] = v;
 }
 
@@ -584,6 +586,9 @@ struct pk_sn_s {
    double value_units = 0.0;
    // Anderson acceleration: depth (0 = plain source iteration) and history
    int aa_depth = 5;
+   // mixed precision: FP32 sweeps until the iterate's relative change falls
+   // below switch64 (the FP32 round-off floor is near 1e-7), then FP64 sweeps
+   bool finish64 = true;
    DBuf<double> aa_df, aa_dg, aa_f, aa_g;  // [depth][C G] x 2, [C G] x 2
    // state kept between solves (warm start)
    bool has_state = false;
@@ -1101,7 +1106,8 @@ int pk_sn_prepare(pk_sn S) {
    if (S->precision == PK_PREC_MIXED) {
       sn_fill_kernel<float><<<grid_for(CG), kBlock, 0, c.stream>>>(S->q32.p, CG, 0.0f);
       PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
-   } else {
+   }
+   if (S->precision != PK_PREC_MIXED || S->finish64) {
       sn_fill_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->q64.p, CG, 0.0);
       PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
    }
@@ -1158,11 +1164,18 @@ int pk_sn_set_tolerances(pk_sn S, double tol_k, double tol_phi, int max_outer) {
 static int sn_prepare(pk_sn_s* S) {
    auto& c = S->sc;
    const long CGM = (long)S->C * S->G * S->M, CG = (long)S->C * S->G;
-   if (S->aa_depth > 0 && !S->aa_df.p)
+   if (S->aa_depth > 0 && !S->aa_df.p) {
       PK_REQUIRE(!S->aa_df.alloc((size_t)kSnAAMax * CG) && !S->aa_dg.alloc((size_t)kSnAAMax * CG) &&
                      !S->aa_f.alloc(CG) && !S->aa_g.alloc(CG),
                  "device allocation failed");
+      // the history starts as zeros (slots outside the window are never read
+      // with a nonzero weight, but hold defined values)
+      PK_HIP(hipMemsetAsync(S->aa_df.p, 0, sizeof(double) * kSnAAMax * CG, c.stream));
+      PK_HIP(hipMemsetAsync(S->aa_dg.p, 0, sizeof(double) * kSnAAMax * CG, c.stream));
+      PK_HIP(hipStreamSynchronize(c.stream));
+   }
    const int nm = S->xs->nm * S->G;
+   // mixed precision finishes its solves with FP64 sweeps: both sets
    if (S->precision == PK_PREC_MIXED) {
       if (!S->psi32.p) {
          PK_REQUIRE(!S->psi32.alloc(CGM) && !S->q32.alloc(CG) && !S->sigt32.alloc(nm) && !S->qp32.alloc(CGM),
@@ -1171,7 +1184,8 @@ static int sn_prepare(pk_sn_s* S) {
          PK_HIP(hipMemcpyAsync(S->sigt32.p, st.data(), sizeof(float) * nm, hipMemcpyHostToDevice, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
       }
-   } else {
+   }
+   if (S->precision != PK_PREC_MIXED || S->finish64) {
       if (!S->psi64.p) {
          PK_REQUIRE(!S->psi64.alloc(CGM) && !S->q64.alloc(CG) && !S->sigt64.alloc(nm) && !S->qp64.alloc(CGM),
                     "device allocation failed");
@@ -1247,9 +1261,13 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
    bool have_prev = false;
    double H[kSnAAMax][kSnAAMax] = {};
    double fbest = 0.0;
+   bool use32 = S->precision == PK_PREC_MIXED;
+   const double switch64 = std::max(100.0 * S->tol_phi, 1e-7);
+   bool last32 = use32;
    for (it = 0; it < S->max_outer; it++) {
       PK_HIP(hipMemcpyAsync(S->old.p, S->phiw.p, sizeof(double) * CG, hipMemcpyDeviceToDevice, c.stream));
-      if (S->precision == PK_PREC_MIXED) {
+      last32 = use32;
+      if (use32) {
          sn_source_kernel<float><<<grid_for(C), kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->sigs.p,
                                                                     xs->nusf.p, xs->chi.p, S->phiw.p, st, S->q32.p);
          PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
@@ -1294,6 +1312,7 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
          it++;
          break;
       }
+      if (use32 && S->finish64 && dphi < switch64) use32 = false;
       if (depth > 0) {
          // restart the history when the residual grows well past its best
          if (have_prev && fnorm2 > 100.0 * fbest) ncol = 0;
@@ -1352,7 +1371,7 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
    *keff = k;
    // psi of the last sweep, rescaled like phiw (reference layout)
    if (psi) {
-      if (S->precision == PK_PREC_MIXED)
+      if (last32)
          sn_to_reference_kernel<float><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi32.p, S->dpos.p, C, G, M, last_scale, psi->d.p);
       else
          sn_to_reference_kernel<double><<<grid_for(CGM), kBlock, 0, c.stream>>>(S->psi64.p, S->dpos.p, C, G, M, last_scale, psi->d.p);
