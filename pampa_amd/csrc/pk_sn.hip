@@ -247,16 +247,16 @@ __device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l
 // Workgroup b sweeps direction m for groups g0 .. g0+GT-1, G = S GT.  When M is
 // a multiple of 8 the S workgroups of a direction get ids of one residue mod 8
 // (the residue picks the chiplet a workgroup is dispatched to), so that they
-// share the direction's records through one L2.  D = 1: the records of level
-// l+1 are loaded during level l (two record sets used in turn); D = 2: those of
-// level l+2 (three sets), which hides a longer load latency at a higher
-// register cost.  The sets rotate with the level loop unrolled D + 1 times, so
-// that no register copy forces an early wait.
+// share the direction's records through one L2.  The records and source of
+// step l + D are loaded during step l (D + 1 register sets in turn): a deeper
+// D hides a longer load latency at a higher register cost.  The sets rotate
+// with the step loop unrolled D + 1 times, so that no register copy forces an
+// early wait.
 template <class T, int GT, int NT, int P, int D>
 __global__ void __launch_bounds__(NT) sn_sweep_lds_kernel(int G, int M, int lstride, const int* __restrict__ lptr,
                                                           const int* __restrict__ nlev, SnRec<T> R,
                                                           const T* __restrict__ qp, T* psi) {
-   static_assert(D == 1 || D == 2, "prefetch distance 1 or 2");
+   static_assert(D >= 1 && D <= 4, "prefetch distance 1 to 4");
    extern __shared__ __align__(16) unsigned char sn_smem[];
    T* slots = reinterpret_cast<T*>(sn_smem);
    const int S = G / GT;
@@ -274,34 +274,26 @@ __global__ void __launch_bounds__(NT) sn_sweep_lds_kernel(int G, int M, int lstr
    const int* lp = lptr + (long)m * lstride;
    const int nl = nlev[m];
    const int tid = threadIdx.x;
-   SnItem<T, GT> X0[P], X1[P], X2[D == 2 ? P : 1];
-   {
-      const int a0 = lp[0], e0 = lp[1];
+   // D + 1 record sets in rotation: step l computes set l mod (D+1) and
+   // prefetches step l + D into the set step l - 1 used
+   SnItem<T, GT> X[D + 1][P];
 #pragma unroll
-      for (int j = 0; j < P; j++) sn_load_item<T, GT>(R, qp, G, min(a0 + tid + j * NT, e0 - 1), g0, X0[j]);
-      if constexpr (D == 2) {
-         const bool more = nl > 1;
-         const int a1 = more ? lp[1] : a0, e1 = more ? lp[2] : a0 + 1;
+   for (int u = 0; u < D; u++) {
+      const bool in = u < nl;
+      const int a = in ? lp[u] : lp[0], e = in ? lp[u + 1] : lp[0] + 1;
 #pragma unroll
-         for (int j = 0; j < P; j++) sn_load_item<T, GT>(R, qp, G, min(a1 + tid + j * NT, e1 - 1), g0, X1[j]);
-      }
+      for (int j = 0; j < P; j++) sn_load_item<T, GT>(R, qp, G, min(a + tid + j * NT, e - 1), g0, X[u][j]);
    }
    int l = 0;
-   if constexpr (D == 1) {
-      for (; l + 1 < nl; l += 2) {
-         sn_level<T, GT, NT, P, 1>(R, lp, l, nl, G, g0, tid, X0, X1, qp, psi, slots);
-         sn_level<T, GT, NT, P, 1>(R, lp, l + 1, nl, G, g0, tid, X1, X0, qp, psi, slots);
-      }
-      if (l < nl) sn_level<T, GT, NT, P, 1>(R, lp, l, nl, G, g0, tid, X0, X1, qp, psi, slots);
-   } else {
-      for (; l + 2 < nl; l += 3) {
-         sn_level<T, GT, NT, P, 2>(R, lp, l, nl, G, g0, tid, X0, X2, qp, psi, slots);
-         sn_level<T, GT, NT, P, 2>(R, lp, l + 1, nl, G, g0, tid, X1, X0, qp, psi, slots);
-         sn_level<T, GT, NT, P, 2>(R, lp, l + 2, nl, G, g0, tid, X2, X1, qp, psi, slots);
-      }
-      if (l < nl) sn_level<T, GT, NT, P, 2>(R, lp, l, nl, G, g0, tid, X0, X2, qp, psi, slots);
-      if (l + 1 < nl) sn_level<T, GT, NT, P, 2>(R, lp, l + 1, nl, G, g0, tid, X1, X0, qp, psi, slots);
+   for (; l + D < nl; l += D + 1) {
+#pragma unroll
+      for (int u = 0; u <= D; u++)
+         sn_level<T, GT, NT, P, D>(R, lp, l + u, nl, G, g0, tid, X[u], X[(u + D) % (D + 1)], qp, psi, slots);
    }
+#pragma unroll
+   for (int u = 0; u < D; u++)
+      if (l + u < nl)
+         sn_level<T, GT, NT, P, D>(R, lp, l + u, nl, G, g0, tid, X[u], X[(u + D) % (D + 1)], qp, psi, slots);
 }
 
 // Any configuration: the same sweep, one item at a time, upwind values from
@@ -558,10 +550,10 @@ struct pk_sn_s {
    // per-item records in direction-major level order (one set per precision,
    // built on first use); geometry = true sweeps from the geometry instead
    bool geometry = false;
-   DBuf<int> ditems, dlptr, dnlev;  // [M C], [M][lstride] (+2 padding), [M]
-   DBuf<int> dpos;                  // [M C]: position of item (m, i) in ditems and psi
-   int lstride = 0;
-   std::vector<int> h_ditems, h_dlptr;
+   DBuf<int> ditems;  // [M C]: items in direction-major level order
+   DBuf<int> dpos;    // [M C]: position of item (m, i) in ditems and psi
+   int lstride = 0;   // row stride of h_dlptr
+   std::vector<int> h_ditems, h_dlptr;  // items, and per direction the offsets of its upwind levels
    std::vector<double> h_dirs;
    // records of one precision; slots = local-memory value slots they use
    struct RecBufs {
@@ -571,11 +563,15 @@ struct pk_sn_s {
       int slots = 0;
       int gt = 0;             // groups per workgroup the slots are sized for
       int P = 0, D = 0;       // items per thread and prefetch distance of that kernel
+      // sweep steps of each direction: its upwind levels, a level wider than
+      // P 1024 split into independent sub-steps (items of one level do not
+      // read each other); [M][lstride] (+2 padding) and [M]
+      DBuf<int> lptr, nsteps;
+      int lstride = 0;
       bool complete = false;  // every value that is read has a slot
       bool fast = false;      // the per-direction kernel applies
       bool built = false;
    } rec32, rec64;
-   int maxw = 0;  // widest level over all directions
    DBuf<int> rec_err;
    DBuf<double> dirs, wts, coef;
    DBuf<double> psi64, q64, sigt64, qp64;  // qp: the source in level order
@@ -631,41 +627,43 @@ static int sn_lds_limit() {
 
 // Value slots: item (m, i) gets a slot when some later item reads its value
 // (a downwind neighbour: w < 0 seen from the neighbour, the schedule's own
-// predicate) and a slot is free; the slot is held from the item's level to the
-// level of its last reader and is reused from the level after.  Levels run in
+// predicate) and a slot is free; the slot is held from the item's step to the
+// step of its last reader and is reused from the step after.  Steps run in
 // order inside a workgroup, so a slot is never written while a reader of its
 // previous value can still run.  Returns the peak number of slots in use;
 // complete = every value with a reader got a slot.
-static int sn_assign_slots(const pk_sn_s* S, int cap, std::vector<int>& slot_of, bool& complete) {
+static int sn_assign_slots(const pk_sn_s* S, const std::vector<int>& lptr, const std::vector<int>& nsteps,
+                           int lstride, int cap, std::vector<int>& slot_of, bool& complete) {
    const int C = S->C, M = S->M;
    const pk_mesh_s* mesh = S->mesh;
    const auto& fptr = mesh->h_fptr;
    const auto& fnb = mesh->h_fnb;
    const auto& fn = mesh->h_fnorm;
    slot_of.assign((size_t)M * C, -1);
-   std::vector<int> last(C), fr;
+   std::vector<int> last(C), step(C), fr;
    std::vector<std::vector<int>> rel;
    int peak = 0;
    complete = true;
    for (int m = 0; m < M; m++) {
       const double* om = &S->h_dirs[3L * m];
-      const int* lev = &S->level[(size_t)m * C];
+      const int* lp = &lptr[(size_t)m * lstride];
+      const int ns = nsteps[m];
+      for (int st = 0; st < ns; st++)
+         for (int k = lp[st]; k < lp[st + 1]; k++) step[S->h_ditems[k] - m * C] = st;
       std::fill(last.begin(), last.end(), -1);
       for (int i2 = 0; i2 < C; i2++)
          for (int f = fptr[i2]; f < fptr[i2 + 1]; f++) {
             const int i = fnb[f];
             if (i < 0) continue;
             const double w = om[0] * fn[3L * f] + om[1] * fn[3L * f + 1] + om[2] * fn[3L * f + 2];
-            if (w < 0.0 && lev[i2] > last[i]) last[i] = lev[i2];
+            if (w < 0.0 && step[i2] > last[i]) last[i] = step[i2];
          }
       fr.clear();
       for (int sl = cap - 1; sl >= 0; sl--) fr.push_back(sl);
-      const int nl = S->nlev[m];
-      rel.assign(nl + 1, {});
-      const int* lp = &S->h_dlptr[(size_t)m * S->lstride];
-      for (int l = 0; l < nl; l++) {
-         for (int sl : rel[l]) fr.push_back(sl);
-         for (int k = lp[l]; k < lp[l + 1]; k++) {
+      rel.assign(ns + 1, {});
+      for (int st = 0; st < ns; st++) {
+         for (int sl : rel[st]) fr.push_back(sl);
+         for (int k = lp[st]; k < lp[st + 1]; k++) {
             const int i = S->h_ditems[k] - m * C;
             if (last[i] < 0) continue;
             if (fr.empty()) {
@@ -683,41 +681,75 @@ static int sn_assign_slots(const pk_sn_s* S, int cap, std::vector<int>& slot_of,
    return peak;
 }
 
+// Steps of every direction from its upwind levels, levels wider than `cap`
+// split into near-equal sub-steps (cap <= 0: the levels as they are).
+static void sn_steps(const pk_sn_s* S, int cap, std::vector<int>& lptr, std::vector<int>& nsteps, int& lstride) {
+   const int M = S->M;
+   std::vector<std::vector<int>> b(M);
+   int maxs = 0;
+   for (int m = 0; m < M; m++) {
+      const int* lp = &S->h_dlptr[(size_t)m * S->lstride];
+      b[m].push_back(lp[0]);
+      for (int l = 0; l < S->nlev[m]; l++) {
+         const int a = lp[l], e = lp[l + 1], w = e - a;
+         const int parts = cap > 0 ? (w + cap - 1) / cap : 1;
+         for (int q = 1; q <= parts; q++) b[m].push_back(a + (int)((long)w * q / parts));
+      }
+      maxs = std::max(maxs, (int)b[m].size() - 1);
+   }
+   lstride = maxs + 1;
+   lptr.assign((size_t)M * lstride + 2, 0);
+   nsteps.assign(M, 0);
+   for (int m = 0; m < M; m++) {
+      nsteps[m] = (int)b[m].size() - 1;
+      for (int st = 0; st <= maxs; st++)
+         lptr[(size_t)m * lstride + st] = b[m][std::min(st, nsteps[m])];
+   }
+}
+
 // Configuration of the per-direction kernel: GT groups per workgroup (G / GT
-// workgroups per direction), P items per thread (P 1024 >= the widest level)
-// and prefetch distance D, within the register budget of 1024-thread
-// workgroups (128 VGPRs; the table lists the largest P per (T, GT, D)).  GT is
-// the smallest divisor of G in {1, 2, 4} whose M G / GT workgroups fit on the
-// compute units at once (one per unit), else the largest that fits in
-// registers; D = 2 where the registers allow it.  False if no configuration
-// applies (then the one-item-at-a-time kernel).
-static int sn_pmax(bool f32, int gt, int D) {
-   if (f32) return D == 1 ? 3 : (gt == 1 ? 3 : gt == 2 ? 2 : 1);
-   return D == 1 ? (gt == 4 ? 1 : gt == 2 ? 2 : 3) : (gt == 1 ? 2 : gt == 2 ? 1 : 0);
+// workgroups per direction), P items per thread (steps of at most P 1024
+// items) and prefetch distance D, within the register budget of 1024-thread
+// workgroups (128 VGPRs).  GT is the smallest divisor of G in {1, 2, 4} whose
+// M G / GT workgroups fit on the compute units at once (one per unit);
+// (P, D) the measured best that fits: P = 1, D = 2 (FP32) or the deepest of
+// 3 (GT = 1), 2, 1 (GT = 4) in FP64.  On the S6 lattice (FP32, GT = 1):
+// 0.68 ms per sweep at P = 1 (D = 2, 3 or 4 alike), 0.73 ms at P = 2.
+static bool sn_fits(bool f32, int gt, int P, int D) {
+   if (P < 1 || P > 2 || D < 1 || D > 4) return false;
+   if (f32) return gt == 1 ? (P == 1 || D <= 3) : gt == 2 ? (P == 1 ? D <= 3 : D <= 2) : (P == 1 && D <= 2);
+   return gt == 1 ? (P == 1 ? D <= 3 : D <= 2) : gt == 2 ? (P == 1 && D <= 2) : (P == 1 && D == 1);
 }
 
 template <class T>
 static bool sn_plan(const pk_sn_s* S, int& gt, int& P, int& D) {
    constexpr bool f32 = sizeof(T) == sizeof(float);
-   P = (S->maxw + 1023) / 1024;
-   if (P > 3) return false;
    const char* env = getenv("PK_SWEEP_GT");
    const int force_gt = env ? atoi(env) : 0;
+   env = getenv("PK_SWEEP_P");
+   const int force_p = env ? atoi(env) : 0;
    env = getenv("PK_SWEEP_D");
    const int force_d = env ? atoi(env) : 0;
    const int cus = std::max(1, ctx().num_cus);
-   int best = 0;
+   gt = 0;
    for (int cand : {1, 2, 4}) {
       if (S->G % cand || (force_gt && cand != force_gt)) continue;
-      if (P > sn_pmax(f32, cand, 1)) continue;
-      best = cand;
+      gt = cand;
       if ((long)S->M * (S->G / cand) <= cus) break;
    }
-   if (!best) return false;
-   gt = best;
-   D = P <= sn_pmax(f32, gt, 2) ? 2 : 1;
-   if (force_d == 1 || force_d == 2) D = (force_d == 2 && P > sn_pmax(f32, gt, 2)) ? 1 : force_d;
-   return true;
+   if (!gt) return false;
+   // one item per thread (steps of at most 1024 items) measured fastest; the
+   // prefetch depth matters little beyond 2
+   P = 1;
+   D = f32 ? 2 : (gt == 1 ? 3 : gt == 2 ? 2 : 1);
+   if (force_p || force_d) {
+      const int p2 = force_p ? force_p : P, d2 = force_d ? force_d : D;
+      if (sn_fits(f32, gt, p2, d2)) {
+         P = p2;
+         D = d2;
+      }
+   }
+   return sn_fits(f32, gt, P, D);
 }
 
 // records of precision T (with their value slots), built once
@@ -741,26 +773,37 @@ static int sn_records(pk_sn_s* S) {
    }
    if (!S->rec_err.p) PK_REQUIRE(!S->rec_err.alloc(1), "device allocation failed");
    if (!B.slot_of.p) {
-      // slots sized for the per-direction kernel's groups per workgroup; when
-      // some value gets no slot there, sized for the one-item kernel (all G)
+      // steps and slots for the per-direction kernel; when some value gets no
+      // slot there, the one-item kernel's (Kahn levels, slots of all G groups)
       int gt = S->G, P = 1, D = 1;
       const bool fast = sn_plan<T>(S, gt, P, D);
-      if (!fast) gt = S->G;
       auto cap_for = [&](int groups) {
          return std::max(1, std::min(sn_lds_limit() / (groups * (int)sizeof(T)), (1 << 27)));
       };
-      std::vector<int> slot_of;
-      B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
-      if (!B.complete && gt != S->G) {
+      std::vector<int> slot_of, lptr, nsteps;
+      int lstride = 0;
+      // PK_SWEEP_STEP caps the items of a step below P 1024 (tests of the split)
+      const char* env = getenv("PK_SWEEP_STEP");
+      const int cap = env && atoi(env) > 0 ? std::min(P * 1024, atoi(env)) : P * 1024;
+      sn_steps(S, fast ? cap : 0, lptr, nsteps, lstride);
+      B.slots = std::max(1, sn_assign_slots(S, lptr, nsteps, lstride, cap_for(fast ? gt : S->G), slot_of,
+                                            B.complete));
+      B.fast = fast && B.complete;
+      if (!B.fast) {
          gt = S->G;
-         B.slots = std::max(1, sn_assign_slots(S, cap_for(gt), slot_of, B.complete));
+         sn_steps(S, 0, lptr, nsteps, lstride);
+         B.slots = std::max(1, sn_assign_slots(S, lptr, nsteps, lstride, cap_for(gt), slot_of, B.complete));
       }
       B.gt = gt;
       B.P = P;
       B.D = D;
-      B.fast = fast && B.complete && gt <= 4;
-      PK_REQUIRE(!B.slot_of.alloc(slot_of.size()), "device allocation failed");
+      B.lstride = lstride;
+      PK_REQUIRE(!B.slot_of.alloc(slot_of.size()) && !B.lptr.alloc(lptr.size()) && !B.nsteps.alloc(nsteps.size()),
+                 "device allocation failed");
       PK_HIP(hipMemcpyAsync(B.slot_of.p, slot_of.data(), sizeof(int) * slot_of.size(), hipMemcpyHostToDevice,
+                            c.stream));
+      PK_HIP(hipMemcpyAsync(B.lptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, c.stream));
+      PK_HIP(hipMemcpyAsync(B.nsteps.p, nsteps.data(), sizeof(int) * nsteps.size(), hipMemcpyHostToDevice,
                             c.stream));
       PK_HIP(hipStreamSynchronize(c.stream));
    }
@@ -788,7 +831,7 @@ static int sn_launch_lds(pk_sn_s* S, const T* qp, T* psi) {
                                                       hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit());
    PK_HIP(attr);
    PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_kernel<T, GT, NT, P, D>),
-                 S->M * (S->G / GT), NT, shm, c.stream, S->G, S->M, S->lstride, S->dlptr.p, S->dnlev.p, recs<T>(S),
+                 S->M * (S->G / GT), NT, shm, c.stream, S->G, S->M, B.lstride, B.lptr.p, B.nsteps.p, recs<T>(S),
                  qp, psi);
    return 0;
 }
@@ -803,7 +846,7 @@ static int sn_launch_lds_any(pk_sn_s* S, const T* qp, T* psi) {
                                                       hipFuncAttributeMaxDynamicSharedMemorySize, sn_lds_limit());
    PK_HIP(attr);
    PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_any_kernel<T, NT>), S->M,
-                 NT, shm, c.stream, S->G, S->lstride, S->dlptr.p, S->dnlev.p, recs<T>(S), qp, psi);
+                 NT, shm, c.stream, S->G, B.lstride, B.lptr.p, B.nsteps.p, recs<T>(S), qp, psi);
    return 0;
 }
 
@@ -846,19 +889,18 @@ static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
    if (B.fast) {
       switch (B.gt * 100 + B.P * 10 + B.D) {
          case 111: done = sn_launch_lds<T, 1, 1, 1>(S, qp, psi); break;
-         case 121: done = sn_launch_lds<T, 1, 2, 1>(S, qp, psi); break;
-         case 131: done = sn_launch_lds<T, 1, 3, 1>(S, qp, psi); break;
          case 112: done = sn_launch_lds<T, 1, 1, 2>(S, qp, psi); break;
+         case 113: done = sn_launch_lds<T, 1, 1, 3>(S, qp, psi); break;
+         case 114: if constexpr (f32) done = sn_launch_lds<T, 1, 1, 4>(S, qp, psi); break;
+         case 121: done = sn_launch_lds<T, 1, 2, 1>(S, qp, psi); break;
          case 122: done = sn_launch_lds<T, 1, 2, 2>(S, qp, psi); break;
-         case 132: if constexpr (f32) done = sn_launch_lds<T, 1, 3, 2>(S, qp, psi); break;
+         case 123: if constexpr (f32) done = sn_launch_lds<T, 1, 2, 3>(S, qp, psi); break;
          case 211: done = sn_launch_lds<T, 2, 1, 1>(S, qp, psi); break;
-         case 221: done = sn_launch_lds<T, 2, 2, 1>(S, qp, psi); break;
-         case 231: if constexpr (f32) done = sn_launch_lds<T, 2, 3, 1>(S, qp, psi); break;
          case 212: done = sn_launch_lds<T, 2, 1, 2>(S, qp, psi); break;
+         case 213: if constexpr (f32) done = sn_launch_lds<T, 2, 1, 3>(S, qp, psi); break;
+         case 221: if constexpr (f32) done = sn_launch_lds<T, 2, 2, 1>(S, qp, psi); break;
          case 222: if constexpr (f32) done = sn_launch_lds<T, 2, 2, 2>(S, qp, psi); break;
          case 411: done = sn_launch_lds<T, 4, 1, 1>(S, qp, psi); break;
-         case 421: if constexpr (f32) done = sn_launch_lds<T, 4, 2, 1>(S, qp, psi); break;
-         case 431: if constexpr (f32) done = sn_launch_lds<T, 4, 3, 1>(S, qp, psi); break;
          case 412: if constexpr (f32) done = sn_launch_lds<T, 4, 1, 2>(S, qp, psi); break;
          default: break;
       }
@@ -1028,18 +1070,13 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
          ditems.push_back(m * C + L[k]);
       }
       while (l < maxlev) dlptr[(size_t)m * S->lstride + (++l)] = (int)ditems.size();
-      for (int k = 0; k < S->nlev[m]; k++)
-         S->maxw = std::max(S->maxw, dlptr[(size_t)m * S->lstride + k + 1] - dlptr[(size_t)m * S->lstride + k]);
    }
    const char* ge = getenv("PK_SWEEP_GEOMETRY");
    S->geometry = ge && *ge && *ge != '0';
    const int G = S->G;
    S->value_units = (double)((long)C * G * M + nin_total * G) + 2.0 * C * G;
    int e = 0;
-   dlptr.push_back(0);  // padding: a row's last entry may be read two past the last level
-   dlptr.push_back(0);
-   e |= S->items.alloc(items.size()) | S->ditems.alloc(ditems.size()) | S->dlptr.alloc(dlptr.size()) |
-        S->dnlev.alloc(M);
+   e |= S->items.alloc(items.size()) | S->ditems.alloc(ditems.size());
    e |= S->dirs.alloc(3L * M) | S->wts.alloc(M) | S->coef.alloc(mesh->F ? mesh->F : 1);
    e |= S->phiw.alloc((size_t)C * G) | S->old.alloc((size_t)C * G) | S->st.alloc(8 + 2 * kSnAAMax);
    if (e) {
@@ -1048,8 +1085,6 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
    }
    PK_HIP(hipMemcpy(S->items.p, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice));
    PK_HIP(hipMemcpy(S->ditems.p, ditems.data(), sizeof(int) * ditems.size(), hipMemcpyHostToDevice));
-   PK_HIP(hipMemcpy(S->dlptr.p, dlptr.data(), sizeof(int) * dlptr.size(), hipMemcpyHostToDevice));
-   PK_HIP(hipMemcpy(S->dnlev.p, S->nlev.data(), sizeof(int) * M, hipMemcpyHostToDevice));
    {
       std::vector<int> pos((size_t)M * C);
       for (size_t k = 0; k < ditems.size(); k++) pos[ditems[k]] = (int)k;

@@ -91,20 +91,29 @@ def _device_sweep(pk, oracle_mod, mesh, xs, case, precision):
     return pk.Sweep(dm, dx, dirs, wts, bct, precision), dirs, wts
 
 
-@pytest.mark.parametrize("G,gt", [(2, None), (3, None), (4, None), (4, 2), (4, 1)])
-def test_record_sweep_matches_geometry_sweep(pk, oracle_mod, tmp_path, monkeypatch, G, gt):
+@pytest.mark.parametrize("G,gt,pd,step", [(2, None, None, None), (3, None, None, None), (4, None, None, None),
+                                          (4, 2, None, None), (4, 1, None, None), (4, 1, (1, 4), 64),
+                                          (4, 1, (2, 1), 96), (4, 4, (1, 1), 50)])
+def test_record_sweep_matches_geometry_sweep(pk, oracle_mod, tmp_path, monkeypatch, G, gt, pd, step):
     """Sweeps of the per-direction kernel (workgroups walking one direction's
-    levels for G / gt groups each, upwind values in local-memory slots) and
-    sweeps of the level-step kernel that reads the geometry compute the same
-    items in the same arithmetic: bit-identical angular fluxes in FP64 and FP32,
-    for every groups-per-workgroup split; a repeated sweep gives the same bits."""
+    steps for G / gt groups each, upwind values in local-memory slots, records
+    prefetched D steps ahead, levels split into steps of at most `step` items)
+    and sweeps of the level-step kernel that reads the geometry compute the
+    same items in the same arithmetic: bit-identical angular fluxes in FP64 and
+    FP32, for every configuration; a repeated sweep gives the same bits."""
     d, case, mesh, xs = _hex(oracle_mod, tmp_path, rings=3, layers=4, order=6, G=G)
     rng = np.random.default_rng(3)
     q = rng.uniform(0.1, 2.0, mesh.C * xs.G)
     if gt:
         monkeypatch.setenv("PK_SWEEP_GT", str(gt))
+    if pd:
+        monkeypatch.setenv("PK_SWEEP_P", str(pd[0]))
+        monkeypatch.setenv("PK_SWEEP_D", str(pd[1]))
+    if step:
+        monkeypatch.setenv("PK_SWEEP_STEP", str(step))
     for prec in (pk.PK_PREC_FP64, pk.PK_PREC_MIXED):
-        print(f"[case] records vs geometry, G {G}, groups per workgroup {gt}, precision {prec}", flush=True)
+        print(f"[case] records vs geometry, G {G}, groups per workgroup {gt}, (P, D) {pd}, step {step}, "
+              f"precision {prec}", flush=True)
         rec, _, _ = _device_sweep(pk, oracle_mod, mesh, xs, case, prec)
         psi_r, phi_r = rec.sweep(q)
         psi_r2, _ = rec.sweep(q)
