@@ -213,21 +213,22 @@ __device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l
 #pragma unroll
    for (int j = 0; j < P; j++) {
       const SnItem<T, GT>& r = cur[j];
-      const int nin = r.nb & 7;
+      // every entry, used or not: an unused one is (slot 0, weight +0) and
+      // adds an exact zero after the real faces (slots hold finite values:
+      // zero at the start, then fluxes), so the sum has the step kernel's bits
+      // and the local-memory reads issue together
+      T v[kMaxIn][GT];
+#pragma unroll
+      for (int k = 0; k < kMaxIn; k++)
+#pragma unroll
+         for (int g = 0; g < GT; g++) v[k][g] = slots[r.up[k] * GT + g];
       T inflow[GT];
 #pragma unroll
       for (int g = 0; g < GT; g++) inflow[g] = 0;
 #pragma unroll
-      for (int k = 0; k < kMaxIn; k++) {
-         T v[GT];
+      for (int k = 0; k < kMaxIn; k++)
 #pragma unroll
-         for (int g = 0; g < GT; g++) v[g] = slots[r.up[k] * GT + g];
-#pragma unroll
-         for (int g = 0; g < GT; g++) {
-            const T t = inflow[g] + r.cw[k] * v[g];
-            inflow[g] = k < nin ? t : inflow[g];
-         }
-      }
+         for (int g = 0; g < GT; g++) inflow[g] += r.cw[k] * v[k][g];
       T out[GT];
 #pragma unroll
       for (int g = 0; g < GT; g++) out[g] = (r.qv[g] - inflow[g]) / r.den[g];
@@ -255,7 +256,7 @@ __device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l
 template <class T, int GT, int NT, int P, int D>
 __global__ void __launch_bounds__(NT) sn_sweep_lds_kernel(int G, int M, int lstride, const int* __restrict__ lptr,
                                                           const int* __restrict__ nlev, SnRec<T> R,
-                                                          const T* __restrict__ qp, T* psi) {
+                                                          const T* __restrict__ qp, T* psi, int nslot_values) {
    static_assert(D >= 1 && D <= 4, "prefetch distance 1 to 4");
    extern __shared__ __align__(16) unsigned char sn_smem[];
    T* slots = reinterpret_cast<T*>(sn_smem);
@@ -274,6 +275,9 @@ __global__ void __launch_bounds__(NT) sn_sweep_lds_kernel(int G, int M, int lstr
    const int* lp = lptr + (long)m * lstride;
    const int nl = nlev[m];
    const int tid = threadIdx.x;
+   // the value slots start at zero (read with weight 0 before any write)
+   for (int l = tid; l < nslot_values; l += NT) slots[l] = 0;
+   __syncthreads();
    // D + 1 record sets in rotation: step l computes set l mod (D+1) and
    // prefetches step l + D into the set step l - 1 used
    SnItem<T, GT> X[D + 1][P];
@@ -832,7 +836,7 @@ static int sn_launch_lds(pk_sn_s* S, const T* qp, T* psi) {
    PK_HIP(attr);
    PK_LAUNCH_SHM("sn_sweep", profiling() ? sweep_bytes(S, sizeof(T)) : 0.0, (sn_sweep_lds_kernel<T, GT, NT, P, D>),
                  S->M * (S->G / GT), NT, shm, c.stream, S->G, S->M, B.lstride, B.lptr.p, B.nsteps.p, recs<T>(S),
-                 qp, psi);
+                 qp, psi, B.slots * GT);
    return 0;
 }
 
