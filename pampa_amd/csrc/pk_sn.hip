@@ -94,8 +94,13 @@ __global__ void __launch_bounds__(256) sn_sweep_step_kernel(const int* __restric
 // item at position -(s + 1).  nb packs the incoming count (3 bits) and 1 + the item's
 // own output slot (0 = none).  Faces whose weight rounds to zero in T (coef w A
 // = 0) are left out.  Unused incoming entries hold (slot 0, weight 0).  Stored
-// structure-of-arrays in direction-major level order.  The records hold
-// Sigma_t: they are built from the cross sections the sweep was prepared with.
+// in direction-major level order: nb, up and den structure-of-arrays, the
+// weights of an item together ([n][kMaxIn], one vector load).  When every
+// value has a slot, tp packs an item's topology for the per-direction kernel
+// into 16 bytes (one load): the four source slots as 16-bit fields (slots are
+// local-memory indices, < 65536), then the item's own slot (-1 = none).  The
+// records hold Sigma_t: they are built from the cross sections the sweep was
+// prepared with.
 constexpr int kMaxIn = 4;
 
 template <class T>
@@ -104,7 +109,8 @@ struct SnRec {
    int* nb;
    T* den;   // [G][n]
    int* up;  // [kMaxIn][n]
-   T* cw;    // [kMaxIn][n]
+   T* cw;    // [n][kMaxIn]
+   uint4* tp;  // [n] {up0 | up1 << 16, up2 | up3 << 16, own slot, 0}, or null
 };
 
 // slot_of[m C + i]: local-memory slot of item (m, i)'s value, -1 if none
@@ -120,6 +126,7 @@ __global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, int 
    const double o0 = geo.dirs[3 * m], o1 = geo.dirs[3 * m + 1], o2 = geo.dirs[3 * m + 2];
    T lhs0 = 0;
    int k = 0;
+   unsigned tslot[kMaxIn] = {0, 0, 0, 0};
    for (int f = geo.fptr[i]; f < geo.fptr[i + 1]; f++) {
       const double* nn = geo.fnorm + 3L * f;
       T w = (T)(o0 * nn[0] + o1 * nn[1] + o2 * nn[2]);
@@ -134,7 +141,8 @@ __global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, int 
             const int up = m * C + i2;
             const int sl = slot_of[up];
             R.up[(long)k * n + it] = sl >= 0 ? sl : -(pos[up] + 1);
-            R.cw[(long)k * n + it] = (T)geo.coef[f] * w * A;
+            R.cw[it * kMaxIn + k] = (T)geo.coef[f] * w * A;
+            if (sl >= 0) tslot[k] = (unsigned)sl;
          } else {
             *overflow = 1;
          }
@@ -143,12 +151,15 @@ __global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, int 
    }
    for (int j = k; j < kMaxIn; j++) {
       R.up[(long)j * n + it] = 0;
-      R.cw[(long)j * n + it] = 0;
+      R.cw[it * kMaxIn + j] = 0;
    }
    const int mt = geo.mat[i];
    const T V = (T)geo.vol[i];
    for (int g = 0; g < G; g++) R.den[(long)g * n + it] = sigt[mt * G + g] * V + lhs0;
    R.nb[it] = (k < kMaxIn ? k : kMaxIn) | ((slot_of[code] + 1) << 3);
+   if (R.tp)
+      R.tp[it] = make_uint4((tslot[0] & 0xFFFFu) | (tslot[1] << 16), (tslot[2] & 0xFFFFu) | (tslot[3] << 16),
+                            (unsigned)slot_of[code], 0u);
 }
 
 // One whole sweep, one workgroup per direction.  The records of direction m are
@@ -167,9 +178,8 @@ __global__ void sn_rec_kernel(const int* __restrict__ items, long n, int C, int 
 // zero-weight faces add exactly zero there).
 template <class T, int GT>
 struct SnItem {
-   int nb;
+   uint4 tp;  // source slots and own slot
    T den[GT];
-   int up[kMaxIn];
    T cw[kMaxIn];
    T qv[GT];  // the item's source
 };
@@ -177,15 +187,24 @@ struct SnItem {
 template <class T, int GT>
 __device__ __forceinline__ void sn_load_item(const SnRec<T>& R, const T* __restrict__ qp, int G, int it, int g0,
                                              SnItem<T, GT>& r) {
-   r.nb = R.nb[it];
+   r.tp = R.tp[it];
 #pragma unroll
    for (int g = 0; g < GT; g++) r.qv[g] = qp[(long)it * G + g0 + g];
 #pragma unroll
    for (int g = 0; g < GT; g++) r.den[g] = R.den[(long)(g0 + g) * R.n + it];
-#pragma unroll
-   for (int k = 0; k < kMaxIn; k++) {
-      r.up[k] = R.up[(long)k * R.n + it];
-      r.cw[k] = R.cw[(long)k * R.n + it];
+   if constexpr (sizeof(T) == 4) {
+      const float4 c = reinterpret_cast<const float4*>(R.cw)[it];
+      r.cw[0] = c.x;
+      r.cw[1] = c.y;
+      r.cw[2] = c.z;
+      r.cw[3] = c.w;
+   } else {
+      const double2 c0 = reinterpret_cast<const double2*>(R.cw)[2L * it];
+      const double2 c1 = reinterpret_cast<const double2*>(R.cw)[2L * it + 1];
+      r.cw[0] = c0.x;
+      r.cw[1] = c0.y;
+      r.cw[2] = c1.x;
+      r.cw[3] = c1.y;
    }
 }
 
@@ -217,11 +236,13 @@ __device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l
       // adds an exact zero after the real faces (slots hold finite values:
       // zero at the start, then fluxes), so the sum has the step kernel's bits
       // and the local-memory reads issue together
+      const int up[kMaxIn] = {(int)(r.tp.x & 0xFFFFu), (int)(r.tp.x >> 16), (int)(r.tp.y & 0xFFFFu),
+                              (int)(r.tp.y >> 16)};
       T v[kMaxIn][GT];
 #pragma unroll
       for (int k = 0; k < kMaxIn; k++)
 #pragma unroll
-         for (int g = 0; g < GT; g++) v[k][g] = slots[r.up[k] * GT + g];
+         for (int g = 0; g < GT; g++) v[k][g] = slots[up[k] * GT + g];
       T inflow[GT];
 #pragma unroll
       for (int g = 0; g < GT; g++) inflow[g] = 0;
@@ -236,7 +257,7 @@ __device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l
       T* o = psi + it * G + g0;
 #pragma unroll
       for (int g = 0; g < GT; g++) o[g] = out[g];
-      const int os = (r.nb >> 3) - 1;
+      const int os = (int)r.tp.z;
       if (os >= 0) {
 #pragma unroll
          for (int g = 0; g < GT; g++) slots[os * GT + g] = out[g];
@@ -321,7 +342,7 @@ __global__ void __launch_bounds__(NT) sn_sweep_lds_any_kernel(int G, int lstride
          for (int g = 0; g < G; g++) inflow[g] = 0;
          for (int k = 0; k < nin; k++) {
             const int s = R.up[(long)k * R.n + it];
-            const T c = R.cw[(long)k * R.n + it];
+            const T c = R.cw[(long)it * kMaxIn + k];
             const T* up = s >= 0 ? slots + (long)s * G : psi + (long)(-(s + 1)) * G;
             for (int g = 0; g < G; g++) inflow[g] += c * up[g];
          }
@@ -562,6 +583,7 @@ struct pk_sn_s {
    // records of one precision; slots = local-memory value slots they use
    struct RecBufs {
       DBuf<int> nb, up, slot_of;
+      DBuf<uint4> tp;  // packed topology, per-direction kernel only
       DBuf<double> den64, cw64;
       DBuf<float> den32, cw32;
       int slots = 0;
@@ -611,12 +633,12 @@ static SnRec<T> recs(pk_sn_s* S);
 template <>
 SnRec<double> recs<double>(pk_sn_s* S) {
    auto& B = S->rec64;
-   return SnRec<double>{(long)S->M * S->C, B.nb.p, B.den64.p, B.up.p, B.cw64.p};
+   return SnRec<double>{(long)S->M * S->C, B.nb.p, B.den64.p, B.up.p, B.cw64.p, B.tp.p};
 }
 template <>
 SnRec<float> recs<float>(pk_sn_s* S) {
    auto& B = S->rec32;
-   return SnRec<float>{(long)S->M * S->C, B.nb.p, B.den32.p, B.up.p, B.cw32.p};
+   return SnRec<float>{(long)S->M * S->C, B.nb.p, B.den32.p, B.up.p, B.cw32.p, B.tp.p};
 }
 
 // Local-memory bytes a workgroup may use (the device's per-workgroup limit).
@@ -792,12 +814,14 @@ static int sn_records(pk_sn_s* S) {
       sn_steps(S, fast ? cap : 0, lptr, nsteps, lstride);
       B.slots = std::max(1, sn_assign_slots(S, lptr, nsteps, lstride, cap_for(fast ? gt : S->G), slot_of,
                                             B.complete));
-      B.fast = fast && B.complete;
+      // (slots are 16-bit fields of the packed topology)
+      B.fast = fast && B.complete && B.slots <= 65536;
       if (!B.fast) {
          gt = S->G;
          sn_steps(S, 0, lptr, nsteps, lstride);
          B.slots = std::max(1, sn_assign_slots(S, lptr, nsteps, lstride, cap_for(gt), slot_of, B.complete));
       }
+      if (B.fast) PK_REQUIRE(!B.tp.alloc(n), "device allocation failed");
       B.gt = gt;
       B.P = P;
       B.D = D;
