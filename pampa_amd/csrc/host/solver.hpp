@@ -15,7 +15,7 @@ struct SolverOptions {
    double eps_tol = 1e-8;     // eigen residual |R x - F x/k| / |F x/k|
    int eps_max_it = 20000;
    int precision = PK_PREC_FP64;
-   int gmres_restart = 60;
+   int gmres_restart = 30;  // PETSc's default GMRES restart
    double inner_reduction = 0.1;
    double wielandt_shift = 0.1;
    int preconditioner = 1;  // 0 cell-block Jacobi, 1 with one Richardson correction

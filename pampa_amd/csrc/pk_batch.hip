@@ -439,7 +439,19 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, 
    const double* h = Hb + (long)j * (m + 1);
    const double* vs = S.vs + (long)b * (m + 1);
    T t = w[row];
-   for (int i = 0; i <= j; i++) t -= (T)(h[i] * vs[i]) * V[(long)i * N + row];
+   // basis vectors 8 at a time: their loads issue together, the sum runs in order
+   for (int i0 = 0; i0 <= j; i0 += 8) {
+      T cf[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+         const int i = min(i0 + u, j);
+         cf[u] = (T)(h[i] * vs[i]);
+         vv[u] = V[(long)i * N + row];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+         if (i0 + u <= j) t -= cf[u] * vv[u];
+   }
    V[(long)(j + 1) * N + row] = t;
    z[row] = cell_block_apply<T>(L, b, binv, t, tl);
    const double u = (double)t;
@@ -602,7 +614,18 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const
    __shared__ T u[kRows];
    T acc = 0;
    const double* vs = S.vs + (long)b * (S.m + 1);
-   for (int i = 0; i < k; i++) acc += (T)(S.y[(long)b * S.m + i] * vs[i]) * V[(long)i * N + row];
+   for (int i0 = 0; i0 < k; i0 += 8) {
+      T cf[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+         const int i = min(i0 + u, k - 1);
+         cf[u] = (T)(S.y[(long)b * S.m + i] * vs[i]);
+         vv[u] = V[(long)i * N + row];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+         if (i0 + u < k) acc += cf[u] * vv[u];
+   }
    u[threadIdx.x] = acc;
    __syncthreads();
    T s = 0;

@@ -25,6 +25,31 @@ __device__ __forceinline__ double wave_sum(double v) {
    return v;
 }
 
+// Wave sums of values 0 .. nv-1 into sh[k][wave] (lane 0 writes).  Values are
+// taken kWaveBatch at a time: all their get() calls first (their loads issue
+// together), then their butterflies, so that a batch costs one memory round
+// trip and overlapping shuffles instead of one round trip per value.  Indices
+// past nv - 1 are clamped (the value is computed again and not written).
+constexpr int kWaveBatch = 8;
+
+template <int MAXV, class Get>
+__device__ __forceinline__ void wave_sums(int nv, Get get, double (*sh)[kBlock / 64]) {
+   constexpr int CH = MAXV < kWaveBatch ? MAXV : kWaveBatch;
+   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+   for (int k0 = 0; k0 < nv; k0 += CH) {
+      double v[CH];
+#pragma unroll
+      for (int c = 0; c < CH; c++) v[c] = get(min(k0 + c, nv - 1));
+#pragma unroll
+      for (int c = 0; c < CH; c++) v[c] = wave_sum(v[c]);
+      if (lane == 0) {
+#pragma unroll
+         for (int c = 0; c < CH; c++)
+            if (k0 + c < nv) sh[k0 + c][wid] = v[c];
+      }
+   }
+}
+
 // Sum of one value over the whole workgroup (blockDim.x == kBlock), result in
 // every thread.  `sh` must hold kBlock/64 doubles.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -48,14 +73,7 @@ template <int MAXV, class Get>
 __device__ bool grid_reduce(int nv, Get get, double* partials, unsigned* ticket, double* out) {
    __shared__ double sh[MAXV][kBlock / 64];
    __shared__ int is_last;
-   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-   for (int k = 0; k < MAXV; k++) {
-      if (k < nv) {
-         double v = wave_sum(get(k));
-         if (lane == 0) sh[k][wid] = v;
-      }
-   }
+   wave_sums<MAXV>(nv, get, sh);
    __syncthreads();
    // Partials are written with system-scope stores, which go through to memory
    // instead of sitting dirty in this XCD's L2, and waited for before the
@@ -153,14 +171,7 @@ __device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, 
                            int seg_block, int seg_blocks) {
    __shared__ double sh[MAXV][kBlock / 64];
    __shared__ int is_last;
-   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-   for (int k = 0; k < MAXV; k++) {
-      if (k < nv) {
-         double v = wave_sum(get(k));
-         if (lane == 0) sh[k][wid] = v;
-      }
-   }
+   wave_sums<MAXV>(nv, get, sh);
    __syncthreads();
    if ((int)threadIdx.x < nv) {
       double s = 0.0;

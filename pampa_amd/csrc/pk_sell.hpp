@@ -23,9 +23,23 @@ __device__ __forceinline__ A sell_row_dot(const int* __restrict__ col, const T* 
       acc += (A)(v2 * x2);
       acc += (A)(v3 * x3);
    }
-   for (; k < wd; k++) {
-      const long p = base + (long)k * 64;
-      acc += (A)(val[p] * x[col[p]]);
+   // the last 1 to 3 entries: all loads first (indices clamped to the last
+   // entry), then the products in order
+   const int rem = wd - k;
+   if (rem > 0) {
+      int cc[3];
+      T vv[3], xx[3];
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+         const long p = base + (long)(k + min(u, rem - 1)) * 64;
+         cc[u] = col[p];
+         vv[u] = val[p];
+      }
+#pragma unroll
+      for (int u = 0; u < 3; u++) xx[u] = x[cc[u]];
+#pragma unroll
+      for (int u = 0; u < 3; u++)
+         if (u < rem) acc += (A)(vv[u] * xx[u]);
    }
    return acc;
 }
