@@ -70,6 +70,7 @@ struct BState {  // per-slot state (arrays over slots)
    double* g;      // [S][m+1]
    double* y;      // [S][m]
    double* vs;     // [S][m+1] 1/norm of each stored (unnormalised) basis vector
+   double* hd;     // [S][m+2] the step's reduced values (dots and the new vector's norm)
    double* resid;  // [S][2]: estimate, target
    double* ww;     // [S]
    int* flags;     // [S][4]: stop, steps in cycle, breakdown, steps in this solve
@@ -396,25 +397,113 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* 
    S.start[b] = 0;
 }
 
-// w = A (vs_j z) and h(i,j) = <vs_i V_i, w>, i <= j  (z = B^-1 V_j, unscaled)
+// Step j of the Arnoldi process with delayed normalisation.  V_j is stored
+// unnormalised and its norm is not known yet: w = A z (z = M^-1 V_j) is stored
+// unscaled, and one reduction gives d_i = <vs_i V_i, w> (i < j), d_j = <V_j, w>
+// and n_j = <V_j, V_j>.  The last workgroup sets vs_j = 1/sqrt(n_j) (j >= 1;
+// vs_0 comes from the cycle start), finishes column j-1 (its subdiagonal is
+// sqrt(n_j): Givens rotation, residual estimate, stop test), and unless the
+// cycle stops there stores column j: h(i, j) = vs_j d_i, h(j, j) = vs_j^2 d_j.
+// At j = mlen (the column past the cycle's last) only n_j is reduced.  The
+// epilogue loads everything it reads in one round (one value per thread),
+// keeps the reduced values in local memory, and stores in parallel.
 template <class T>
 __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
                                               T* w, const T* V, int j, double* partials, unsigned* tickets, unsigned* gtickets,
                                               const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
-   const double* vs = S.vs + (long)b * (S.m + 1);
+   const int m = S.m;
+   const double* vs = S.vs + (long)b * (m + 1);
+   __shared__ double hd[kBMaxM + 2];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const long s = row >> 6;
-   const long base = A.off[s] + (threadIdx.x & 63);
-   const T acc = (T)vs[j] * sell_row_dot<T, T>(A.col, aval, z, base, A.width[s], (T)0);
-   w[row] = acc;
-   const double wr = (double)acc;
-   const int nv = j + 1;
-   seg_reduce<kBMaxM>(
-       nv, [&](int k) { return (double)V[(long)k * N + row] * vs[k] * wr; }, partials + (long)L.wg_first[b] * nv,
-       tickets + b, gtickets + L.wg_first[b], S.H + (long)b * (S.m + 1) * S.m + (long)j * (S.m + 1), (int)blockIdx.x - L.wg_first[b],
-       L.wg_count[b]);
+   const int mlen = L.mlen[b];
+   const bool edge = j == mlen;
+   const double vj = (double)V[(long)j * N + row];
+   bool last;
+   if (edge) {
+      last = seg_reduce<1>(
+          1, [&](int) { return vj * vj; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], hd + j + 1,
+          (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
+   } else {
+      const long s = row >> 6;
+      const long base = A.off[s] + (threadIdx.x & 63);
+      const T acc = sell_row_dot<T, T>(A.col, aval, z, base, A.width[s], (T)0);
+      w[row] = acc;
+      const double wr = (double)acc;
+      const int nv = j + 2;
+      last = seg_reduce<kBMaxM + 2>(
+          nv,
+          [&](int k) {
+             return k < j ? (double)V[(long)k * N + row] * vs[k] * wr : (k == j ? vj * wr : vj * vj);
+          },
+          partials + (long)L.wg_first[b] * nv, tickets + b, gtickets + L.wg_first[b], hd,
+          (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
+   }
+   if (!last) return;
+   __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], sc[4];
+   __shared__ int stop_sh;
+   const int ld = m + 1;
+   const int tid = threadIdx.x;
+   double* Hb = S.H + (long)b * (m + 1) * m;
+   int* fl = S.flags + 4 * b;
+   const int jj = j - 1;  // the column finished here (none at j = 0)
+   if (jj >= 0) {
+      if (tid <= jj) hc[tid] = Hb[tid + (long)jj * ld];
+      if (tid < jj) {
+         cc[tid] = S.cs[(long)b * m + tid];
+         ss[tid] = S.sn[(long)b * m + tid];
+      }
+      if (tid == 128) sc[0] = S.g[(long)b * (m + 1) + jj];
+      if (tid == 129) sc[1] = S.resid[2 * b + 1];
+   }
+   if (tid == 130) sc[2] = vs[0];
+   __syncthreads();
+   if (tid == 0) {
+      double vsj = sc[2];
+      int stop = 0;
+      if (jj >= 0) {
+         // Givens rotations of column jj (the rotated entry carried in a
+         // register, the next one read from local memory), then its own
+         const double hn = sqrt(fmax(hd[j + 1], 0.0));
+         vsj = hn > 0.0 ? 1.0 / hn : 0.0;
+         S.vs[(long)b * (m + 1) + j] = vsj;
+         const double gj = sc[0];
+         double cur = hc[0];
+         for (int i = 0; i < jj; i++) {
+            const double nxt = hc[i + 1];
+            hc[i] = cc[i] * cur + ss[i] * nxt;
+            cur = -ss[i] * cur + cc[i] * nxt;
+         }
+         const double a = cur, c2 = hn;
+         const double rr = sqrt(a * a + c2 * c2);
+         const double c = rr > 0.0 ? a / rr : 1.0, sn = rr > 0.0 ? c2 / rr : 0.0;
+         hc[jj] = rr;
+         hc[jj + 1] = 0.0;
+         S.cs[(long)b * m + jj] = c;
+         S.sn[(long)b * m + jj] = sn;
+         S.g[(long)b * (m + 1) + jj + 1] = -sn * gj;
+         S.g[(long)b * (m + 1) + jj] = c * gj;
+         const double res = fabs(sn * gj);
+         S.resid[2 * b] = res;
+         fl[1] = jj + 1;
+         const bool breakdown = !(hn > 1e-300) || !(rr > 0.0);
+         if (breakdown) fl[2] = 1;
+         if (res <= sc[1] || breakdown || jj + 1 == mlen) {
+            fl[0] = 1;
+            stop = 1;
+         }
+      }
+      sc[3] = vsj;
+      stop_sh = stop;
+   }
+   __syncthreads();
+   // the rotated column jj, then (unless the cycle stops) column j
+   if (jj >= 0 && tid <= jj + 1) Hb[tid + (long)jj * ld] = hc[tid];
+   if (stop_sh || edge) return;
+   const double vsj = sc[3];
+   if (tid < j) Hb[tid + (long)j * ld] = vsj * hd[tid];
+   if (tid == j) Hb[j + (long)j * ld] = vsj * vsj * hd[j];
 }
 
 template <class T>
@@ -425,20 +514,19 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
 }
 
 
-// V[j+1] = w - sum_i h(i,j) vs_i V_i (stored unnormalised), z = B^-1 V[j+1],
-// h(j+1,j) = |V[j+1]|, vs_{j+1} = 1/h(j+1,j), Givens update, stop test
+// V[j+1] = vs_j w - sum_i h(i,j) vs_i V_i (stored unnormalised), z = B^-1 V[j+1];
+// its norm is reduced by the next step (spmv_dot_step)
 template <class T>
 __device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, T* V, T* z, const T* binv, int j,
-                                          double* partials, unsigned* tickets, unsigned* gtickets, const BState& S) {
+                                          const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
    __shared__ T tl[kRows];
    const int m = S.m;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   double* Hb = S.H + (long)b * (m + 1) * m;
-   const double* h = Hb + (long)j * (m + 1);
+   const double* h = S.H + (long)b * (m + 1) * m + (long)j * (m + 1);
    const double* vs = S.vs + (long)b * (m + 1);
-   T t = w[row];
+   T t = (T)vs[j] * w[row];
    // basis vectors 8 at a time: their loads issue together, the sum runs in order
    for (int i0 = 0; i0 <= j; i0 += 8) {
       T cf[8], vv[8];
@@ -454,53 +542,12 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, 
    }
    V[(long)(j + 1) * N + row] = t;
    z[row] = cell_block_apply<T>(L, b, binv, t, tl);
-   const double u = (double)t;
-   bool last = seg_reduce<1>(
-       1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
-       (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
-   if (!last) return;
-   __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], gj;
-   const int ld = m + 1;
-   const int tid = threadIdx.x;
-   if (tid <= j) hc[tid] = Hb[tid + (long)j * ld];
-   if (tid < j) {
-      cc[tid] = S.cs[(long)b * m + tid];
-      ss[tid] = S.sn[(long)b * m + tid];
-   }
-   if (tid == 0) gj = S.g[(long)b * (m + 1) + j];
-   __syncthreads();
-   if (tid != 0) return;
-   const double hn = sqrt(fmax(S.ww[b], 0.0));
-   hc[j + 1] = hn;
-   for (int i = 0; i < j; i++) {
-      double a = hc[i], c2 = hc[i + 1];
-      hc[i] = cc[i] * a + ss[i] * c2;
-      hc[i + 1] = -ss[i] * a + cc[i] * c2;
-   }
-   const double a = hc[j], c2 = hc[j + 1];
-   const double rr = sqrt(a * a + c2 * c2);
-   const double c = rr > 0.0 ? a / rr : 1.0, sn = rr > 0.0 ? c2 / rr : 0.0;
-   hc[j] = rr;
-   hc[j + 1] = 0.0;
-   for (int i = 0; i <= j + 1; i++) Hb[i + (long)j * ld] = hc[i];
-   S.cs[(long)b * m + j] = c;
-   S.sn[(long)b * m + j] = sn;
-   S.g[(long)b * (m + 1) + j + 1] = -sn * gj;
-   S.g[(long)b * (m + 1) + j] = c * gj;
-   const double res = fabs(sn * gj);
-   S.resid[2 * b] = res;
-   int* fl = S.flags + 4 * b;
-   fl[1] = j + 1;
-   const bool breakdown = !(hn > 1e-300) || !(rr > 0.0);
-   if (breakdown) fl[2] = 1;
-   if (res <= S.resid[2 * b + 1] || breakdown || j + 1 == L.mlen[b]) fl[0] = 1;
-   S.vs[(long)b * (m + 1) + j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
 }
 
 template <class T>
 __global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, T* z, const T* binv,
-                                                      int j, double* partials, unsigned* tickets, unsigned* gtickets, BState S) {
-   orth_step<T>(N, L, w, V, z, binv, j, partials, tickets, gtickets, S);
+                                                      int j, BState S) {
+   orth_step<T>(N, L, w, V, z, binv, j, S);
 }
 
 // The GMRES steps with their execution span recorded for timing: every
@@ -544,10 +591,9 @@ __global__ void __launch_bounds__(kRows) bpoly_timed_kernel(BLayout L, BSell A, 
 
 template <class T>
 __global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, const T* w, T* V, T* z,
-                                                            const T* binv, int j, double* partials,
-                                                            unsigned* tickets, unsigned* gtickets, BState S, unsigned long long* clk) {
+                                                            const T* binv, int j, BState S, unsigned long long* clk) {
    const unsigned long long t0 = wall_clock64();
-   orth_step<T>(N, L, w, V, z, binv, j, partials, tickets, gtickets, S);
+   orth_step<T>(N, L, w, V, z, binv, j, S);
    record_span(t0, clk);
 }
 
@@ -774,6 +820,7 @@ struct Slot {
    bool used = false, harvested = false;
    long n = 0;
    int bs = 1, pbs = 1;
+   int mlen = 0;               // restart length
    long unit0 = 0, units = 0;  // row units
    long re0 = 0, re = 0;       // R/A entries (x64)
    long fe0 = 0, fe = 0;       // F entries (x64)
@@ -872,7 +919,7 @@ static double pool_bytes(long units, long rent, long fent, long binv, int slots,
    b += rows * t * (m + 1 + 5) + binv * 64.0 * t;                // basis, z z0 w e rhs, block inverses
    if (precision == PK_PREC_MIXED) b += rent * 64.0 * t;         // A copy in FP32
    b += (double)units * (m + 4) * 8.0;                           // reduction partials
-   b += (double)slots * ((m + 1.0) * m + 4.0 * m + 24.0) * 8.0;  // per-slot state
+   b += (double)slots * ((m + 1.0) * m + 6.0 * m + 28.0) * 8.0;  // per-slot state
    return b;
 }
 
@@ -901,7 +948,7 @@ static int pool_alloc(pk_batch_s* P) {
    else
       PK_REQUIRE(!alloc_inner(P->i64, false), "device allocation failed for the eigen pool");
    // per-slot state
-   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 5 * (size_t)m + 26);
+   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 6 * (size_t)m + 28);
    const size_t ni = (size_t)S * 13;
    PK_REQUIRE(!P->sd.alloc(nd) && !P->si.alloc(ni), "device allocation failed for the eigen pool");
    PK_HIP(hipMemsetAsync(P->sd.p, 0, sizeof(double) * nd, c.stream));
@@ -920,6 +967,7 @@ static int pool_alloc(pk_batch_s* P) {
    B.g = take((size_t)S * (m + 1));
    B.y = take((size_t)S * m);
    B.vs = take((size_t)S * (m + 1));
+   B.hd = take((size_t)S * (m + 2));
    B.resid = take(2 * (size_t)S);
    B.ww = take(S);
    B.dots = take(3 * (size_t)S);
@@ -1071,6 +1119,7 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    // more, whose stronger block-Jacobi blocks gain too few iterations from a
    // longer basis to pay for it (measured on the 256 x 256 instances)
    const int i_mlen = bs >= 16 ? std::min(P->m, 30) : P->m;
+   P->slots[sidx].mlen = i_mlen;
    const long l_row0 = row0, l_b0 = b0 * 64;
    PK_HIP(hipMemcpyAsync(P->wg_first.p + sidx, &i_first, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->wg_count.p + sidx, &i_count, sizeof(int), hipMemcpyHostToDevice, c.stream));
@@ -1178,7 +1227,10 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
                                                  gtk, S);
    const double s = sizeof(T);
    std::vector<int> hf, ph;
-   for (int j = 0; j < m; j++) {
+   // steps 0 .. m-1, then the column past the last (j = m: the norm of V_m
+   // that finishes column m-1 of the slots still iterating)
+   for (int j = 0; j <= m; j++) {
+      const bool full = j < m;
       if (prof) {
          // issue only steps in which some slot iterates; bytes of those slots
          hf.resize(4 * (size_t)ns);
@@ -1186,42 +1238,51 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          PK_HIP(hipMemcpyAsync(hf.data(), S.flags, sizeof(int) * 4 * ns, hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipMemcpyAsync(ph.data(), S.phase, sizeof(int) * ns, hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
-         double n = 0.0, nnz = 0.0, bsn = 0.0;
+         double n = 0.0, nnz = 0.0, bsn = 0.0, nedge = 0.0;
          for (int b = 0; b < ns; b++)
             if (P->slots[b].used && ph[b] == kInner && !hf[4 * b]) {
+               if (j == P->slots[b].mlen) {
+                  nedge += P->slots[b].n;
+                  continue;
+               }
                n += P->slots[b].n;
                nnz += (double)P->slots[b].nnz;
                bsn += (double)P->slots[b].n * P->slots[b].pbs;
             }
-         if (n == 0.0) break;
+         if (n == 0.0 && nedge == 0.0) break;
          // algorithmic bytes: spmv_dot is an SpMV (value and column of every
-         // stored entry, z gathered once, w written) plus j+1 basis vectors; orth
-         // reads w and j+1 basis vectors, writes V[j+1] and z and reads the block
-         // inverses.  The two kernels record their execution span on the device
-         // clock (clk[0..3]).
+         // stored entry, z gathered once, w written) plus j+1 basis vectors
+         // (the column past a slot's last reads V_j only); orth reads w and j+1
+         // basis vectors, writes V[j+1] and z and reads the block inverses.
+         // The kernels record their execution span on the device clock.
          clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
-         if (o.poly)
+         if (o.poly && full)
             bpoly_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p,
                                                                I.z.p, S, P->clk.p + 4);
          bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk,
                                                                 S, P->clk.p);
-         borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, part, tk, gtk,
-                                                            S, P->clk.p + 2);
+         if (full)
+            borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, S,
+                                                               P->clk.p + 2);
          unsigned long long h[6];
          PK_HIP(hipMemcpyAsync(h, P->clk.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
-         prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
-                  nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s);
-         prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz, n * s * (j + 4.0) + bsn * s);
-         if (o.poly)
-            prof_add("gmres_poly", 1, (double)(h[5] - h[4]) / P->clock_hz,
-                     nnz * (s + 4.0) + 3.0 * n * s + bsn * s);
+         if (n > 0.0) {
+            prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
+                     nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s + nedge * s);
+            prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz, n * s * (j + 4.0) + bsn * s);
+            if (o.poly)
+               prof_add("gmres_poly", 1, (double)(h[5] - h[4]) / P->clock_hz,
+                        nnz * (s + 4.0) + 3.0 * n * s + bsn * s);
+         } else {
+            prof_add("gmres_norm", 1, (double)(h[1] - h[0]) / P->clock_hz, nedge * s);
+         }
       } else {
-         if (o.poly)
+         if (o.poly && full)
             bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
                                                          S, 0);
          bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S);
-         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, part, tk, gtk, S);
+         if (full) borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, S);
       }
    }
    btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
