@@ -161,10 +161,14 @@ __device__ __forceinline__ double sum_partials(const double* partials, int count
 // `seg_blocks` the segment's workgroup count, `partials` the segment's base
 // (seg_blocks * nv doubles), `ticket` the segment's counter and `gticket` the
 // segment's group counters (one per workgroup).  Segments of more than
-// kSegGroup workgroups reduce in two levels: the last workgroup of each group
-// of kSegGroup sums its group's partials into the group's first row, and the
-// last group to finish sums the group rows; the order of every sum is fixed.
-constexpr int kSegGroup = 32;
+// 16 workgroups reduce in two levels: the last workgroup of each group sums
+// its group's partials into the group's first row, and the last group to
+// finish sums the group rows; the order of every sum is fixed.  The group size
+// (16, 32 or 64 workgroups, about the square root of the segment's count)
+// keeps both levels short.
+__device__ __forceinline__ int seg_group(int seg_blocks) {
+   return seg_blocks <= 256 ? 16 : (seg_blocks <= 1024 ? 32 : 64);
+}
 
 template <int MAXV, class Get>
 __device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, unsigned* gticket, double* out,
@@ -183,9 +187,10 @@ __device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, 
    __builtin_amdgcn_s_waitcnt(0);
    __syncthreads();
    int count = seg_blocks, stride = 1;
-   if (seg_blocks > kSegGroup) {
-      const int g0 = seg_block / kSegGroup * kSegGroup;
-      const int gs = min(kSegGroup, seg_blocks - g0);
+   const int group = seg_group(seg_blocks);
+   if (seg_blocks > group) {
+      const int g0 = seg_block / group * group;
+      const int gs = min(group, seg_blocks - g0);
       if (threadIdx.x == 0)
          is_last = (__hip_atomic_fetch_add(&gticket[g0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
                     (unsigned)gs - 1);
@@ -198,8 +203,8 @@ __device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, 
       if (threadIdx.x == 0) __hip_atomic_store(&gticket[g0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
-      count = (seg_blocks + kSegGroup - 1) / kSegGroup;
-      stride = kSegGroup;
+      count = (seg_blocks + group - 1) / group;
+      stride = group;
    }
    if (threadIdx.x == 0)
       is_last = (__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
