@@ -292,9 +292,13 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
        2, [&](int k) { return k == 0 ? rr : qq; }, partials + (long)L.wg_first[b] * 2, tickets + b, gtickets + L.wg_first[b],
        S.rq + 2 * b, (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    if (!last || threadIdx.x != 0) return;
-   const double k = d[1] / d[0];
-   const double res = sqrt(S.rq[2 * b] / S.rq[2 * b + 1]);
+   // the slot's state, read at once before any store (the stores may alias it)
+   const double d0 = d[0], d1 = d[1], rq0 = S.rq[2 * b], rq1 = S.rq[2 * b + 1];
    const double rprev = S.res[b];  // 0 before the first update
+   const double dl0 = S.dl[b], kprev = S.kprev[b], ks0 = S.ks[b];
+   const int it = S.it[b], shifted = S.shifted[b], slow0 = S.slow[b];
+   const double k = d1 / d0;
+   const double res = sqrt(rq0 / rq1);
    S.k[b] = k;
    S.res[b] = res;
    if (res <= o.tol) {
@@ -302,37 +306,40 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
       S.phase[b] = kDone;
       return;
    }
-   if (!(res == res) || S.it[b] >= o.max_outer) {
+   if (!(res == res) || it >= o.max_outer) {
       S.status[b] = 2;
       S.phase[b] = kDone;
       return;
    }
+   double ks = ks0;
    if (o.delta > 0.0) {
       // Adaptive Wielandt shift: the outer residual falls by about
       // (1/k - 1/k_s) / (1/k_2 - 1/k_s) per update, which tends to 1 when the
       // dominance ratio k_2/k does; three shifted updates in a row that reduce
       // the residual by less than 0.8 halve the shift distance (down to 0.004),
       // at the price of harder inner solves
-      double dl = S.dl[b];
-      if (S.shifted[b] && rprev > 0.0) {
-         S.slow[b] = res > 0.8 * rprev ? S.slow[b] + 1 : 0;
-         if (S.slow[b] >= 3 && dl > 0.004) {
+      double dl = dl0;
+      if (shifted && rprev > 0.0) {
+         int slow = res > 0.8 * rprev ? slow0 + 1 : 0;
+         if (slow >= 3 && dl > 0.004) {
             dl *= 0.5;
             S.dl[b] = dl;
-            S.slow[b] = 0;
-            S.ks[b] = k + dl;
+            slow = 0;
+            ks = k + dl;
+            S.ks[b] = ks;
             S.rebuild[b] = 1;
          }
+         S.slow[b] = slow;
       }
-      const bool settle = S.it[b] >= 2 && fabs(k - S.kprev[b]) < 1e-2 * k;
-      if (settle && (!S.shifted[b] || fabs(k + dl - S.ks[b]) > 0.2 * dl)) {
+      const bool settle = it >= 2 && fabs(k - kprev) < 1e-2 * k;
+      if (settle && (!shifted || fabs(k + dl - ks) > 0.2 * dl)) {
          S.ks[b] = k + dl;
          S.shifted[b] = 1;
          S.rebuild[b] = 1;
       }
    }
    S.kprev[b] = k;
-   S.it[b] += 1;
+   S.it[b] = it + 1;
    S.phase[b] = kInner;
    S.start[b] = 1;
 }
@@ -602,31 +609,46 @@ __global__ void clk_reset_kernel(unsigned long long* clk, int n) {
    if (i < n) clk[i] = (i & 1) ? 0ull : ~0ull;
 }
 
-// y = H^-1 g, one workgroup per slot that took steps this cycle
+// y = H^-1 g, one workgroup per slot that took steps this cycle.  H is loaded
+// into local memory 4 values per thread at a time; the triangular solve runs
+// in one wave, lane r owning row r: from the last column back, the column's
+// unknown is solved by its row's lane and broadcast, and the rows above
+// subtract its contribution (column order: a fixed order of operations).
 __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) {
    const int b = blockIdx.x;
    if (b >= nslots || S.phase[b] != kInner) return;
    const int m = S.m;
    const int k = S.flags[4 * b + 1];
    if (k == 0) return;
-   __shared__ double h[kBMaxM * kBMaxM], gg[kBMaxM], yy[kBMaxM];
+   __shared__ double h[kBMaxM * kBMaxM];
    const double* Hb = S.H + (long)b * (m + 1) * m;
-   for (int p = threadIdx.x; p < k * k; p += blockDim.x) {
-      int i = p % k, q = p / k;
-      h[i + q * kBMaxM] = Hb[i + (long)q * (m + 1)];
-   }
-   if ((int)threadIdx.x < k) gg[threadIdx.x] = S.g[(long)b * (m + 1) + threadIdx.x];
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      for (int i = k - 1; i >= 0; i--) {
-         double s = gg[i];
-         for (int q = i + 1; q < k; q++) s -= h[i + q * kBMaxM] * yy[q];
-         double dd = h[i + i * kBMaxM];
-         yy[i] = dd != 0.0 ? s / dd : 0.0;
+   const int kk = k * k;
+   for (int p0 = threadIdx.x; p0 < kk; p0 += 4 * kRows) {
+      double v[4];
+      int pp[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         pp[u] = min(p0 + u * kRows, kk - 1);
+         const int i = pp[u] % k, q = pp[u] / k;
+         v[u] = Hb[i + (long)q * (m + 1)];
       }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+         if (p0 + u * kRows < kk) h[pp[u] % k + (pp[u] / k) * kBMaxM] = v[u];
    }
+   const int r = threadIdx.x;
+   const double gr0 = r < k ? S.g[(long)b * (m + 1) + r] : 0.0;
    __syncthreads();
-   if ((int)threadIdx.x < k) S.y[(long)b * m + threadIdx.x] = yy[threadIdx.x];
+   if (r >= 64) return;
+   double gr = gr0, yr = 0.0;
+   for (int i = k - 1; i >= 0; i--) {
+      const double dd = h[i + i * kBMaxM];
+      const double gi = __shfl(gr, i, 64);
+      const double yi = dd != 0.0 ? gi / dd : 0.0;
+      if (r == i) yr = yi;
+      if (r < i) gr -= h[r + i * kBMaxM] * yi;
+   }
+   if (r < k) S.y[(long)b * m + r] = yr;
 }
 
 // Second term of the preconditioner M^-1 = B^-1 (2 I - A B^-1):
