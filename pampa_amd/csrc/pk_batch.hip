@@ -411,9 +411,10 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* 
 // vs_0 comes from the cycle start), finishes column j-1 (its subdiagonal is
 // sqrt(n_j): Givens rotation, residual estimate, stop test), and unless the
 // cycle stops there stores column j: h(i, j) = vs_j d_i, h(j, j) = vs_j^2 d_j.
-// At j = mlen (the column past the cycle's last) only n_j is reduced.  The
-// epilogue loads everything it reads in one round (one value per thread),
-// keeps the reduced values in local memory, and stores in parallel.
+// At j = mlen (the column past the cycle's last) only n_j is reduced.  What
+// the epilogue reads is loaded by every workgroup into local memory before the
+// reduction (one value per thread; the last workgroup then has it at hand),
+// the reduced values stay in local memory, and the stores run in parallel.
 template <class T>
 __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
                                               T* w, const T* V, int j, double* partials, unsigned* tickets, unsigned* gtickets,
@@ -427,6 +428,22 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
    const int mlen = L.mlen[b];
    const bool edge = j == mlen;
    const double vj = (double)V[(long)j * N + row];
+   __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], sc[4];
+   __shared__ int stop_sh;
+   const int ld = m + 1;
+   const int tid = threadIdx.x;
+   double* Hb = S.H + (long)b * (m + 1) * m;
+   const int jj = j - 1;  // the column finished here (none at j = 0)
+   if (jj >= 0) {
+      if (tid <= jj) hc[tid] = Hb[tid + (long)jj * ld];
+      if (tid < jj) {
+         cc[tid] = S.cs[(long)b * m + tid];
+         ss[tid] = S.sn[(long)b * m + tid];
+      }
+      if (tid == 128) sc[0] = S.g[(long)b * (m + 1) + jj];
+      if (tid == 129) sc[1] = S.resid[2 * b + 1];
+   }
+   if (tid == 130) sc[2] = vs[0];
    bool last;
    if (edge) {
       last = seg_reduce<1>(
@@ -448,24 +465,7 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    }
    if (!last) return;
-   __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], sc[4];
-   __shared__ int stop_sh;
-   const int ld = m + 1;
-   const int tid = threadIdx.x;
-   double* Hb = S.H + (long)b * (m + 1) * m;
    int* fl = S.flags + 4 * b;
-   const int jj = j - 1;  // the column finished here (none at j = 0)
-   if (jj >= 0) {
-      if (tid <= jj) hc[tid] = Hb[tid + (long)jj * ld];
-      if (tid < jj) {
-         cc[tid] = S.cs[(long)b * m + tid];
-         ss[tid] = S.sn[(long)b * m + tid];
-      }
-      if (tid == 128) sc[0] = S.g[(long)b * (m + 1) + jj];
-      if (tid == 129) sc[1] = S.resid[2 * b + 1];
-   }
-   if (tid == 130) sc[2] = vs[0];
-   __syncthreads();
    if (tid == 0) {
       double vsj = sc[2];
       int stop = 0;
