@@ -783,6 +783,14 @@ __global__ void bphase_kernel(int nslots, BState S) {
    S.phase[b] = kOuter;
 }
 
+// Every slot's phase and the error flag into pinned host memory (one
+// workgroup), for a host that reads them after an event instead of a copy.
+__global__ void __launch_bounds__(kRows) bstatus_kernel(int nslots, BState S, const int* err, int* host) {
+   for (int b = threadIdx.x; b < nslots; b += kRows)
+      __hip_atomic_store(host + b, S.phase[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   if (threadIdx.x == 0) __hip_atomic_store(host + nslots, *err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 }  // namespace pk
 
 using namespace pk;
@@ -899,6 +907,15 @@ struct pk_batch_s {
    std::vector<int> h_phase;
    long admitted = 0;
    Graph graph;
+   // pinned read-back of the phases and the error flag (two sets in turn) and
+   // their events, for the fixed-batch run
+   int* h_status = nullptr;
+   hipEvent_t status_ev[2] = {nullptr, nullptr};
+   ~pk_batch_s() {
+      for (auto e : status_ev)
+         if (e) (void)hipEventDestroy(e);
+      if (h_status) (void)hipHostFree(h_status);
+   }
    DBuf<unsigned long long> clk;  // execution spans of a profiled step's kernels
    double clock_hz = 0.0;           // device wall clock
    double tick_seconds = 0.0;
@@ -1433,6 +1450,55 @@ static int slot_result(pk_batch_s* P, int sidx, SlotResult& R, pk_vec_s* x) {
 
 // ---- fixed batch: every member admitted at once and run to the end ------------------
 
+// Ticks run one ahead of the host's read of the slots' phases: tick t + 1 is
+// queued before the host waits for tick t's phases (a tick over slots that are
+// all DONE changes nothing), so the device does not idle during the read-back.
+// The phases come back through pinned memory, two sets in turn.
+static int fixed_run(pk_batch_s* P) {
+   Ctx& c = ctx();
+   const int ns = P->max_slots;
+   const long units = P->ua.high_water();
+   if (units <= 0) return 0;
+   if (!P->h_status) {
+      PK_HIP(hipHostMalloc((void**)&P->h_status, sizeof(int) * 2 * (ns + 1)));
+      for (auto& e : P->status_ev) PK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+   }
+   auto t0 = std::chrono::steady_clock::now();
+   auto issue = [&](int k) -> int {
+      int* h = P->h_status + k * (ns + 1);
+      for (int q = 0; q <= ns; q++) h[q] = -1;
+      if (P->precision == PK_PREC_MIXED)
+         PK_CALL(run_tick<float>(P, units));
+      else
+         PK_CALL(run_tick<double>(P, units));
+      P->ticks++;
+      bstatus_kernel<<<1, kRows, 0, c.stream>>>(ns, P->S, P->err.p, h);
+      PK_HIP(hipGetLastError());
+      PK_HIP(hipEventRecord(P->status_ev[k], c.stream));
+      return 0;
+   };
+   PK_CALL(issue(0));
+   for (int cur = 0;; cur ^= 1) {
+      PK_CALL(issue(cur ^ 1));
+      PK_HIP(hipEventSynchronize(P->status_ev[cur]));
+      const int* h = P->h_status + cur * (ns + 1);
+      int herr = __atomic_load_n(&h[ns], __ATOMIC_ACQUIRE);
+      int nl = 0;
+      for (int q = 0; q < ns; q++) {
+         const int ph = __atomic_load_n(&h[q], __ATOMIC_ACQUIRE);
+         P->h_phase[q] = ph;
+         if (P->slots[q].used && ph != kDone) nl++;
+      }
+      if (herr != 0 || nl == 0) {
+         PK_HIP(hipStreamSynchronize(c.stream));
+         P->tick_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+         PK_REQUIRE(herr == 0, "singular diagonal block in the block-Jacobi preconditioner");
+         return 0;
+      }
+   }
+}
+
+
 static int fixed_solve(pk_batch_s* P) {
    const int B = (int)P->fixR.size();
    for (int s = 0; s < P->max_slots; s++)
@@ -1446,9 +1512,7 @@ static int fixed_solve(pk_batch_s* P) {
       PK_CALL(rc);
       P->fix_slot[b] = slot;
    }
-   int live = B, done = 0;
-   while (live > 0) PK_CALL(pool_step(P, 1, &live, &done));
-   return 0;
+   return fixed_run(P);
 }
 
 template <class T>
