@@ -52,6 +52,7 @@ struct BLayout {
    const int* fbs;         // [S] cell block of F
    const long* binv_off;   // [S]
    const int* mlen;        // [S] restart length of the slot (<= the pool's)
+   const int* sub;         // [S] 1: the preconditioner adds a sweep over the workgroup's own block
 };
 
 struct BSell {
@@ -371,21 +372,62 @@ __device__ __forceinline__ T cell_block_apply(const BLayout& L, int b, const T* 
    return acc;
 }
 
-// V0 = rhs = r (in T), z = B^-1 V0, e = 0, beta = |V0|, target = reduction beta.
+// S t, the preconditioner's building block: z = B^-1 t (cell blocks), and for
+// slots with sub set one block-Jacobi sweep over the workgroup's own diagonal
+// block of A (the entries whose column is one of the workgroup's rows):
+// z += B^-1 (t - A_loc z).  A fixed linear map of t (the same for a slot alone
+// or in a mix: its workgroups start at its first row).  Every thread calls it
+// (it holds barriers); padding rows get 0.
+constexpr long kSubMaxUnits = 512;
+
+template <class T>
+__device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* binv, const BSell& A, const T* aval,
+                                           T t, T* tl, T* xl) {
+   const T x = cell_block_apply<T>(L, b, binv, t, tl);
+   if (!L.sub[b]) return x;
+   xl[threadIdx.x] = x;
+   __syncthreads();
+   const long row0 = blockIdx.x * (long)kRows;
+   const long row = row0 + threadIdx.x;
+   const long s = row >> 6;
+   const long base = A.off[s] + (threadIdx.x & 63);
+   const int wd = A.width[s];
+   T acc = 0;
+   // entries 4 at a time: their loads first, then the sum in order
+   for (int k0 = 0; k0 < wd; k0 += 4) {
+      int cc[4];
+      T aa[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         const long p = base + (long)min(k0 + u, wd - 1) * 64;
+         cc[u] = A.col[p];
+         aa[u] = aval[p];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+         const long lc = (long)cc[u] - row0;
+         if (k0 + u < wd && lc >= 0 && lc < kRows) acc += aa[u] * xl[lc];
+      }
+   }
+   __syncthreads();  // tl is staged again below
+   return x + cell_block_apply<T>(L, b, binv, t - acc, tl);
+}
+
+// V0 = rhs = r (in T), z = S V0, e = 0, beta = |V0|, target = reduction beta.
 // The basis is stored unnormalised: vs[i] = 1/|V_i| scales it where it is used.
 template <class T>
-__global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, const double* r, T* rhs, T* e, T* V0, T* z,
-                                                        const T* binv, BOpts o, double* partials,
+__global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, BSell A, const T* aval, const double* r, T* rhs,
+                                                        T* e, T* V0, T* z, const T* binv, BOpts o, double* partials,
                                                         unsigned* tickets, unsigned* gtickets, BState S) {
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kInner || !S.start[b]) return;
-   __shared__ T tl[kRows];
+   __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const T v = (T)r[row];
    rhs[row] = v;
    e[row] = 0;
    V0[row] = v;
-   z[row] = cell_block_apply<T>(L, b, binv, v, tl);
+   z[row] = precond_apply<T>(L, b, binv, A, aval, v, tl, xl);
    const double u = (double)v;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
@@ -521,14 +563,14 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
 }
 
 
-// V[j+1] = vs_j w - sum_i h(i,j) vs_i V_i (stored unnormalised), z = B^-1 V[j+1];
+// V[j+1] = vs_j w - sum_i h(i,j) vs_i V_i (stored unnormalised), z = S V[j+1];
 // its norm is reduced by the next step (spmv_dot_step)
 template <class T>
-__device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, T* V, T* z, const T* binv, int j,
-                                          const BState& S) {
+__device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* w, T* V,
+                                          T* z, const T* binv, int j, const BState& S) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
-   __shared__ T tl[kRows];
+   __shared__ T tl[kRows], xl[kRows];
    const int m = S.m;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const double* h = S.H + (long)b * (m + 1) * m + (long)j * (m + 1);
@@ -548,13 +590,13 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const T* w, 
          if (i0 + u <= j) t -= cf[u] * vv[u];
    }
    V[(long)(j + 1) * N + row] = t;
-   z[row] = cell_block_apply<T>(L, b, binv, t, tl);
+   z[row] = precond_apply<T>(L, b, binv, A, aval, t, tl, xl);
 }
 
 template <class T>
-__global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, const T* w, T* V, T* z, const T* binv,
-                                                      int j, BState S) {
-   orth_step<T>(N, L, w, V, z, binv, j, S);
+__global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, BSell A, const T* aval, const T* w, T* V,
+                                                      T* z, const T* binv, int j, BState S) {
+   orth_step<T>(N, L, A, aval, w, V, z, binv, j, S);
 }
 
 // The GMRES steps with their execution span recorded for timing: every
@@ -586,21 +628,22 @@ __global__ void __launch_bounds__(kRows) bpoly_timed_kernel(BLayout L, BSell A, 
    const unsigned long long t0 = wall_clock64();
    const int b = slot_of_wg(L);
    if (gmres_on(S, b)) {
-      __shared__ T tl[kRows];
+      __shared__ T tl[kRows], xl[kRows];
       const long row = blockIdx.x * (long)kRows + threadIdx.x;
       const long s = row >> 6;
       const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
-      const T c = cell_block_apply<T>(L, b, binv, v[row] - az, tl);
+      const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
       z[row] = z0[row] + c;
    }
    record_span(t0, clk);
 }
 
 template <class T>
-__global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, const T* w, T* V, T* z,
-                                                            const T* binv, int j, BState S, unsigned long long* clk) {
+__global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, BSell A, const T* aval, const T* w,
+                                                            T* V, T* z, const T* binv, int j, BState S,
+                                                            unsigned long long* clk) {
    const unsigned long long t0 = wall_clock64();
-   orth_step<T>(N, L, w, V, z, binv, j, S);
+   orth_step<T>(N, L, A, aval, w, V, z, binv, j, S);
    record_span(t0, clk);
 }
 
@@ -651,35 +694,35 @@ __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) 
    if (r < k) S.y[(long)b * m + r] = yr;
 }
 
-// Second term of the preconditioner M^-1 = B^-1 (2 I - A B^-1):
-// z = z0 + B^-1 (v - A z0), z0 = B^-1 v given.  mode 0: slots iterating in
+// Second term of the preconditioner M^-1 = S (2 I - A S):
+// z = z0 + S (v - A z0), z0 = S v given.  mode 0: slots iterating in
 // this GMRES step; mode 1: slots whose cycle took steps (the update).
 template <class T>
 __global__ void __launch_bounds__(kRows) bpoly_kernel(BLayout L, BSell A, const T* aval, const T* v, const T* z0,
                                                       const T* binv, T* z, BState S, int mode) {
    const int b = slot_of_wg(L);
    if (mode == 0 ? !gmres_on(S, b) : (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0)) return;
-   __shared__ T tl[kRows];
+   __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
-   const T c = cell_block_apply<T>(L, b, binv, v[row] - az, tl);
+   const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
    z[row] = z0[row] + c;
 }
 
-// u = sum_i y_i vs_i V[i]; accumulate: e += B^-1 u, else u and B^-1 u are
+// u = sum_i y_i vs_i V[i]; accumulate: e += S u, else u and S u are
 // written out for the second preconditioner term (e += M^-1 u in two kernels)
 template <class T>
-__global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const T* V, const T* binv,
-                                                        BState S, T* e, int accumulate, T* uout, T* zout) {
+__global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, BSell A, const T* aval, const T* V,
+                                                        const T* binv, BState S, T* e, int accumulate, T* uout,
+                                                        T* zout) {
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kInner) return;
    const int k = S.flags[4 * b + 1];
    if (k == 0) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long local = row - L.row_off[b];
-   const int bs = L.pbs[b];
-   __shared__ T u[kRows];
+   __shared__ T u[kRows], xl[kRows];
    T acc = 0;
    const double* vs = S.vs + (long)b * (S.m + 1);
    for (int i0 = 0; i0 < k; i0 += 8) {
@@ -694,16 +737,7 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, const
       for (int u = 0; u < 8; u++)
          if (i0 + u < k) acc += cf[u] * vv[u];
    }
-   u[threadIdx.x] = acc;
-   __syncthreads();
-   T s = 0;
-   if (local < L.nrows[b]) {
-      const long cell = local / bs;
-      const int gr = (int)(local - cell * bs);
-      const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
-      const int first = (int)(threadIdx.x - gr);  // a cell never straddles a workgroup
-      for (int q = 0; q < bs; q++) s += bi[q] * u[first + q];
-   }
+   const T s = precond_apply<T>(L, b, binv, A, aval, acc, u, xl);
    if (accumulate) {
       if (local < L.nrows[b]) e[row] += s;
    } else {
@@ -731,14 +765,14 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    // slots not in the inner phase, or stopped before this cycle took a step,
    // keep their flags
    if (b < 0 || S.phase[b] != kInner || (S.flags[4 * b] && S.flags[4 * b + 1] == 0)) return;
-   __shared__ T tl[kRows];
+   __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
    const long base = A.off[s] + (threadIdx.x & 63);
    const T acc = sell_row_dot<T, T>(A.col, aval, e, base, A.width[s], (T)0);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
-   z[row] = cell_block_apply<T>(L, b, binv, rv, tl);
+   z[row] = precond_apply<T>(L, b, binv, A, aval, rv, tl, xl);
    const double u = (double)rv;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
@@ -885,7 +919,7 @@ struct pk_batch_s {
    int precision = PK_PREC_FP64;
    BOpts opts{1e-8, 0.1, 0.1, 10000, 2000, 1};
    // device arrays
-   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen;
+   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen, sub;
    DBuf<long> row_off, binv_off;
    DBuf<long> r_off, f_off;
    DBuf<int> r_width, f_width, r_col, f_col, rlen;
@@ -928,7 +962,8 @@ struct pk_batch_s {
    double seconds = 0.0;
 
    BLayout layout() const {
-      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p};
+      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p,
+                     sub.p};
    }
    BSell rsell() const { return BSell{r_off.p, r_width.p, r_col.p, rlen.p}; }
    BSell fsell() const { return BSell{f_off.p, f_width.p, f_col.p, rlen.p}; }
@@ -968,7 +1003,7 @@ static int pool_alloc(pk_batch_s* P) {
    const int S = P->max_slots, m = P->m;
    const long rent = P->max_rentries * 64, fent = P->max_fentries * 64, binv = P->max_binv * 64;
    bool bad = P->unit_slot.alloc(units) || P->wg_first.alloc(S) || P->wg_count.alloc(S) || P->nrows.alloc(S) ||
-              P->mlen.alloc(S) ||
+              P->mlen.alloc(S) || P->sub.alloc(S) ||
               P->pbs.alloc(S) || P->fbs.alloc(S) || P->row_off.alloc(S) || P->binv_off.alloc(S) ||
               P->r_off.alloc(sl + 1) || P->f_off.alloc(sl + 1) || P->r_width.alloc(sl) || P->f_width.alloc(sl) ||
               P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
@@ -1159,6 +1194,10 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    // longer basis to pay for it (measured on the 256 x 256 instances)
    const int i_mlen = bs >= 16 ? std::min(P->m, 30) : P->m;
    P->slots[sidx].mlen = i_mlen;
+   // the workgroup-block sweep in the preconditioner: problems of up to
+   // kSubMaxUnits row units, whose steps are latency-bound (the sweep's extra
+   // pass over A costs latency, not bandwidth, and saves 15-30 % of the steps)
+   const int i_sub = units <= kSubMaxUnits ? 1 : 0;
    const long l_row0 = row0, l_b0 = b0 * 64;
    PK_HIP(hipMemcpyAsync(P->wg_first.p + sidx, &i_first, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->wg_count.p + sidx, &i_count, sizeof(int), hipMemcpyHostToDevice, c.stream));
@@ -1166,6 +1205,7 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    PK_HIP(hipMemcpyAsync(P->fbs.p + sidx, &i_bs, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->pbs.p + sidx, &i_pbs, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->mlen.p + sidx, &i_mlen, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->sub.p + sidx, &i_sub, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->row_off.p + sidx, &l_row0, sizeof(long), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->binv_off.p + sidx, &l_b0, sizeof(long), hipMemcpyHostToDevice, c.stream));
    // state: OUTER, rebuild (A = R, block inverses) on the first tick
@@ -1262,8 +1302,8 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    // with the polynomial preconditioner the first term B^-1 t goes to z0 and
    // the step's bpoly kernel completes z; without it, straight into z
    T* zfirst = o.poly ? I.z0.p : I.z.p;
-   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->r.p, I.rhs.p, I.e.p, I.V.p, zfirst, I.binv.p, o, part, tk,
-                                                 gtk, S);
+   bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, P->r.p, I.rhs.p, I.e.p, I.V.p, zfirst, I.binv.p, o,
+                                                 part, tk, gtk, S);
    const double s = sizeof(T);
    std::vector<int> hf, ph;
    // steps 0 .. m-1, then the column past the last (j = m: the norm of V_m
@@ -1301,7 +1341,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk,
                                                                 S, P->clk.p);
          if (full)
-            borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, S,
+            borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S,
                                                                P->clk.p + 2);
          unsigned long long h[6];
          PK_HIP(hipMemcpyAsync(h, P->clk.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
@@ -1321,17 +1361,19 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
             bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
                                                          S, 0);
          bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S);
-         if (full) borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.w.p, I.V.p, zfirst, I.binv.p, j, S);
+         if (full)
+            borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S);
       }
    }
    btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
    if (o.poly) {
       // e += M^-1 u: u and B^-1 u, then the second term, then the sum
-      bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
+      bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
       bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1);
       badd_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, I.z.p, S, I.e.p);
    } else {
-      bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, I.V.p, I.binv.p, S, I.e.p, 1, nullptr, nullptr);
+      bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 1, nullptr,
+                                                     nullptr);
    }
    brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, zfirst, I.binv.p, o, part,
                                                    tk, gtk, S);
