@@ -1317,7 +1317,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          PK_HIP(hipMemcpyAsync(hf.data(), S.flags, sizeof(int) * 4 * ns, hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipMemcpyAsync(ph.data(), S.phase, sizeof(int) * ns, hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
-         double n = 0.0, nnz = 0.0, bsn = 0.0, nedge = 0.0;
+         double n = 0.0, nnz = 0.0, bsn = 0.0, nedge = 0.0, nnz_sub = 0.0;
          for (int b = 0; b < ns; b++)
             if (P->slots[b].used && ph[b] == kInner && !hf[4 * b]) {
                if (j == P->slots[b].mlen) {
@@ -1327,12 +1327,14 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
                n += P->slots[b].n;
                nnz += (double)P->slots[b].nnz;
                bsn += (double)P->slots[b].n * P->slots[b].pbs;
+               if (P->slots[b].units <= kSubMaxUnits) nnz_sub += (double)P->slots[b].nnz;
             }
          if (n == 0.0 && nedge == 0.0) break;
          // algorithmic bytes: spmv_dot is an SpMV (value and column of every
          // stored entry, z gathered once, w written) plus j+1 basis vectors
          // (the column past a slot's last reads V_j only); orth reads w and j+1
-         // basis vectors, writes V[j+1] and z and reads the block inverses.
+         // basis vectors, writes V[j+1] and z, reads the block inverses and,
+         // for slots with the workgroup-block sweep, A once.
          // The kernels record their execution span on the device clock.
          clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
          if (o.poly && full)
@@ -1349,7 +1351,8 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          if (n > 0.0) {
             prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
                      nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s + nedge * s);
-            prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz, n * s * (j + 4.0) + bsn * s);
+            prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz,
+                     n * s * (j + 4.0) + bsn * s + nnz_sub * (s + 4.0));
             if (o.poly)
                prof_add("gmres_poly", 1, (double)(h[5] - h[4]) / P->clock_hz,
                         nnz * (s + 4.0) + 3.0 * n * s + bsn * s);
