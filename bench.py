@@ -29,8 +29,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per gmres_spmv_dot launch on the quarter-core workload: two rocprofv3 --pmc
 # passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --roofline-pass`, each scaled by
 # what the same counter reads for stream_copy_kernel moving 2^28 bytes each way
-# (fetch x2.000, write x1.000): 3362.3 KiB x 2.000 + 277.8 KiB = 7.170 MB.
-SPMV_DOT_TRAFFIC_BYTES = 7.170e6
+# (fetch x2.000, write x1.000): 3095.6 KiB x 2.000 + 274.7 KiB = 6.621 MB.
+SPMV_DOT_TRAFFIC_BYTES = 6.621e6
 METRIC = ("eigen solves/s + p50/p99 latency, mixed vs FP64 at |dk|<=1pcm; sweep/SpMV HBM GB/s")
 
 
