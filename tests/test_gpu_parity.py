@@ -23,6 +23,10 @@ def _case(tmp_path, name):
         return W.quarter_core_2d(str(tmp_path / name))
     if name == "quarter-core-g4":
         return W.quarter_core_2d(str(tmp_path / name), seed=11, side=48, G=4)
+    if name == "quarter-core-300":
+        # 180,000 rows = 704 row units: above the size whose preconditioner adds
+        # the workgroup-block sweep (the plain cell-block Jacobi path)
+        return W.quarter_core_2d(str(tmp_path / name), seed=5, side=300)
     return os.path.join(REF, name)
 
 
@@ -52,7 +56,7 @@ def _product(d, *opts):
         os.chdir(cwd)
 
 
-DIFF = ["slab-1d", "quarter-core-170", "quarter-core-g4", "slabs/reflected-diffusion",
+DIFF = ["slab-1d", "quarter-core-170", "quarter-core-g4", "quarter-core-300", "slabs/reflected-diffusion",
         "pwr-iaea-benchmark/cartesian-diffusion", "pwr-iaea-benchmark/unstructured-diffusion"]
 
 
