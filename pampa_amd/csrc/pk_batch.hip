@@ -52,7 +52,6 @@ struct BLayout {
    const int* fbs;         // [S] cell block of F
    const long* binv_off;   // [S]
    const int* mlen;        // [S] restart length of the slot (<= the pool's)
-   const int* sub;         // [S] 1: the preconditioner adds a sweep over the workgroup's own block
 };
 
 struct BSell {
@@ -372,19 +371,17 @@ __device__ __forceinline__ T cell_block_apply(const BLayout& L, int b, const T* 
    return acc;
 }
 
-// S t, the preconditioner's building block: z = B^-1 t (cell blocks), and for
-// slots with sub set one block-Jacobi sweep over the workgroup's own diagonal
-// block of A (the entries whose column is one of the workgroup's rows):
-// z += B^-1 (t - A_loc z).  A fixed linear map of t (the same for a slot alone
-// or in a mix: its workgroups start at its first row).  Every thread calls it
-// (it holds barriers); padding rows get 0.
-constexpr long kSubMaxUnits = 512;
+// S t, the preconditioner's building block: z = B^-1 t (cell blocks), then one
+// block-Jacobi sweep over the workgroup's own diagonal block of A (the entries
+// whose column is one of the workgroup's rows): z += B^-1 (t - A_loc z).  A
+// fixed linear map of t (the same for a problem alone or in a mix: its
+// workgroups start at its first row).  Every thread calls it (it holds
+// barriers); padding rows get 0.
 
 template <class T>
 __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* binv, const BSell& A, const T* aval,
                                            T t, T* tl, T* xl) {
    const T x = cell_block_apply<T>(L, b, binv, t, tl);
-   if (!L.sub[b]) return x;
    xl[threadIdx.x] = x;
    __syncthreads();
    const long row0 = blockIdx.x * (long)kRows;
@@ -919,7 +916,7 @@ struct pk_batch_s {
    int precision = PK_PREC_FP64;
    BOpts opts{1e-8, 0.1, 0.1, 10000, 2000, 1};
    // device arrays
-   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen, sub;
+   DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen;
    DBuf<long> row_off, binv_off;
    DBuf<long> r_off, f_off;
    DBuf<int> r_width, f_width, r_col, f_col, rlen;
@@ -962,8 +959,7 @@ struct pk_batch_s {
    double seconds = 0.0;
 
    BLayout layout() const {
-      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p,
-                     sub.p};
+      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p};
    }
    BSell rsell() const { return BSell{r_off.p, r_width.p, r_col.p, rlen.p}; }
    BSell fsell() const { return BSell{f_off.p, f_width.p, f_col.p, rlen.p}; }
@@ -1003,7 +999,7 @@ static int pool_alloc(pk_batch_s* P) {
    const int S = P->max_slots, m = P->m;
    const long rent = P->max_rentries * 64, fent = P->max_fentries * 64, binv = P->max_binv * 64;
    bool bad = P->unit_slot.alloc(units) || P->wg_first.alloc(S) || P->wg_count.alloc(S) || P->nrows.alloc(S) ||
-              P->mlen.alloc(S) || P->sub.alloc(S) ||
+              P->mlen.alloc(S) ||
               P->pbs.alloc(S) || P->fbs.alloc(S) || P->row_off.alloc(S) || P->binv_off.alloc(S) ||
               P->r_off.alloc(sl + 1) || P->f_off.alloc(sl + 1) || P->r_width.alloc(sl) || P->f_width.alloc(sl) ||
               P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
@@ -1194,10 +1190,6 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    // longer basis to pay for it (measured on the 256 x 256 instances)
    const int i_mlen = bs >= 16 ? std::min(P->m, 30) : P->m;
    P->slots[sidx].mlen = i_mlen;
-   // the workgroup-block sweep in the preconditioner: problems of up to
-   // kSubMaxUnits row units, whose steps are latency-bound (the sweep's extra
-   // pass over A costs latency, not bandwidth, and saves 15-30 % of the steps)
-   const int i_sub = units <= kSubMaxUnits ? 1 : 0;
    const long l_row0 = row0, l_b0 = b0 * 64;
    PK_HIP(hipMemcpyAsync(P->wg_first.p + sidx, &i_first, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->wg_count.p + sidx, &i_count, sizeof(int), hipMemcpyHostToDevice, c.stream));
@@ -1205,7 +1197,6 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    PK_HIP(hipMemcpyAsync(P->fbs.p + sidx, &i_bs, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->pbs.p + sidx, &i_pbs, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->mlen.p + sidx, &i_mlen, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->sub.p + sidx, &i_sub, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->row_off.p + sidx, &l_row0, sizeof(long), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->binv_off.p + sidx, &l_b0, sizeof(long), hipMemcpyHostToDevice, c.stream));
    // state: OUTER, rebuild (A = R, block inverses) on the first tick
@@ -1317,7 +1308,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
          PK_HIP(hipMemcpyAsync(hf.data(), S.flags, sizeof(int) * 4 * ns, hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipMemcpyAsync(ph.data(), S.phase, sizeof(int) * ns, hipMemcpyDeviceToHost, c.stream));
          PK_HIP(hipStreamSynchronize(c.stream));
-         double n = 0.0, nnz = 0.0, bsn = 0.0, nedge = 0.0, nnz_sub = 0.0;
+         double n = 0.0, nnz = 0.0, bsn = 0.0, nedge = 0.0;
          for (int b = 0; b < ns; b++)
             if (P->slots[b].used && ph[b] == kInner && !hf[4 * b]) {
                if (j == P->slots[b].mlen) {
@@ -1327,14 +1318,13 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
                n += P->slots[b].n;
                nnz += (double)P->slots[b].nnz;
                bsn += (double)P->slots[b].n * P->slots[b].pbs;
-               if (P->slots[b].units <= kSubMaxUnits) nnz_sub += (double)P->slots[b].nnz;
             }
          if (n == 0.0 && nedge == 0.0) break;
          // algorithmic bytes: spmv_dot is an SpMV (value and column of every
          // stored entry, z gathered once, w written) plus j+1 basis vectors
          // (the column past a slot's last reads V_j only); orth reads w and j+1
-         // basis vectors, writes V[j+1] and z, reads the block inverses and,
-         // for slots with the workgroup-block sweep, A once.
+         // basis vectors, writes V[j+1] and z, reads the block inverses and A
+         // (the preconditioner's workgroup-block sweep).
          // The kernels record their execution span on the device clock.
          clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
          if (o.poly && full)
@@ -1352,7 +1342,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
             prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
                      nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s + nedge * s);
             prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz,
-                     n * s * (j + 4.0) + bsn * s + nnz_sub * (s + 4.0));
+                     n * s * (j + 4.0) + bsn * s + nnz * (s + 4.0));
             if (o.poly)
                prof_add("gmres_poly", 1, (double)(h[5] - h[4]) / P->clock_hz,
                         nnz * (s + 4.0) + 3.0 * n * s + bsn * s);

@@ -24,8 +24,7 @@ def _case(tmp_path, name):
     if name == "quarter-core-g4":
         return W.quarter_core_2d(str(tmp_path / name), seed=11, side=48, G=4)
     if name == "quarter-core-300":
-        # 180,000 rows = 704 row units: above the size whose preconditioner adds
-        # the workgroup-block sweep (the plain cell-block Jacobi path)
+        # 180,000 rows = 704 row units: reductions in groups of 32 workgroups
         return W.quarter_core_2d(str(tmp_path / name), seed=5, side=300)
     return os.path.join(REF, name)
 
