@@ -17,13 +17,15 @@
 //   OUTER  k = <Fx,Fx>/<Fx,Rx>, phi = x/|x|, r = F phi/k - R phi,
 //          res = |r|/|F phi/k|; res <= tol -> DONE; after the estimate settles
 //          shift ks = k + delta (rebuild A and its block inverses) -> INNER
-//   INNER  right-preconditioned GMRES(m) with cell-block Jacobi on A e = r,
-//          one restart cycle per tick, to |r - A e| <= reduction |r|;
-//          then x = phi + e -> OUTER
+//   INNER  right-preconditioned GMRES(m) on A e = r (preconditioner: cell-block
+//          Jacobi with a sweep over each workgroup's block, in a two-term
+//          polynomial), one restart cycle per tick, to |r - A e| <= reduction
+//          |r|; then x = phi + e -> OUTER
 // A tick is a fixed kernel sequence, captured once as a HIP graph (per grid
-// size) and relaunched; the host syncs once per tick to learn which slots are
-// DONE.  FP64 outer quantities; the GMRES basis, operator copy and block
-// inverses in FP32 (mixed) or FP64, with FP64 reductions and Givens rotations.
+// size) and relaunched; the host reads the slots' phases once per tick (a
+// fixed batch queues the next tick first).  FP64 outer quantities; the GMRES
+// basis, operator copy and block inverses in FP32 (mixed) or FP64, with FP64
+// reductions and Givens rotations.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -95,7 +97,7 @@ struct BState {  // per-slot state (arrays over slots)
 struct BOpts {
    double tol, delta, reduction;
    int max_outer, max_inner;
-   int poly;  // 1: preconditioner B^-1 (2 I - A B^-1) (one Richardson correction), 0: B^-1
+   int poly;  // 1: preconditioner S (2 I - A S) (one Richardson correction), 0: S (precond_apply)
 };
 
 __device__ __forceinline__ int slot_of_wg(const BLayout& L) { return L.unit_slot[blockIdx.x]; }
@@ -753,7 +755,7 @@ __global__ void badd_kernel(long N, BLayout L, const T* z, BState S, T* e) {
    e[row] += z[row];
 }
 
-// true residual V[0] = rhs - A e after the cycle, z = B^-1 V[0], and the stop decision
+// true residual V[0] = rhs - A e after the cycle, z = S V[0], and the stop decision
 template <class T>
 __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, const T* aval, const T* e,
                                                          const T* rhs, T* V0, T* z, const T* binv, BOpts o,
@@ -1290,7 +1292,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    bblock_inverse_kernel<T><<<grid_for(rows, 64), 64, 0, c.stream>>>(rows, L, Rs, P->a_val.p, S.rebuild,
                                                                      I.binv.p, P->err.p);
    bclear_rebuild_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
-   // with the polynomial preconditioner the first term B^-1 t goes to z0 and
+   // with the polynomial preconditioner the first term S t goes to z0 and
    // the step's bpoly kernel completes z; without it, straight into z
    T* zfirst = o.poly ? I.z0.p : I.z.p;
    bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, P->r.p, I.rhs.p, I.e.p, I.V.p, zfirst, I.binv.p, o,
@@ -1360,7 +1362,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    }
    btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
    if (o.poly) {
-      // e += M^-1 u: u and B^-1 u, then the second term, then the sum
+      // e += M^-1 u: u and S u, then the second term, then the sum
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
       bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1);
       badd_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, I.z.p, S, I.e.p);
