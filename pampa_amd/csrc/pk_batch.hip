@@ -141,7 +141,7 @@ __global__ void place_x_kernel(long n, long padded, const double* x0, double* x)
 // ---- shifted operator and block inverses (slots flagged for rebuild) -------------
 
 __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BSell F,
-                              const double* fval, BState S, double* aval) {
+                              const double* fval, BState S, double* aval, float* aval32) {
    long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= N) return;
    const int b = slot_of_row(L, row);
@@ -163,19 +163,8 @@ __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BS
          if (c / fb == cell) v -= fval[fbase + (c % fb) * 64] * inv_ks;
       }
       aval[p] = v;
+      if (aval32) aval32[p] = (float)v;
    }
-}
-
-// values of rebuilt slots, converted to the inner precision
-template <class T>
-__global__ void to_t_kernel(long N, BLayout L, const int* rebuild, const double* a, T* out, BSell A) {
-   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (row >= N) return;
-   const int b = slot_of_row(L, row);
-   if (b < 0 || !rebuild[b]) return;
-   const long s = row >> 6;
-   const long base = A.off[s] + (row & 63);
-   for (int k = 0; k < A.width[s]; k++) out[base + (long)k * 64] = (T)a[base + (long)k * 64];
 }
 
 template <class T>
@@ -244,11 +233,6 @@ __global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* 
    }
    T* dst = binv + L.binv_off[b] + cell * bs * bs;
    for (int p = 0; p < bs * bs; p++) dst[p] = (T)m[p];
-}
-
-__global__ void bclear_rebuild_kernel(int nslots, BState S) {
-   int b = blockIdx.x * blockDim.x + threadIdx.x;
-   if (b < nslots) S.rebuild[b] = 0;
 }
 
 // ---- outer step ------------------------------------------------------------------------
@@ -695,10 +679,11 @@ __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) 
 
 // Second term of the preconditioner M^-1 = S (2 I - A S):
 // z = z0 + S (v - A z0), z0 = S v given.  mode 0: slots iterating in
-// this GMRES step; mode 1: slots whose cycle took steps (the update).
+// this GMRES step; mode 1: slots whose cycle took steps (the update: also
+// e += z).
 template <class T>
 __global__ void __launch_bounds__(kRows) bpoly_kernel(BLayout L, BSell A, const T* aval, const T* v, const T* z0,
-                                                      const T* binv, T* z, BState S, int mode) {
+                                                      const T* binv, T* z, BState S, int mode, T* e) {
    const int b = slot_of_wg(L);
    if (mode == 0 ? !gmres_on(S, b) : (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0)) return;
    __shared__ T tl[kRows], xl[kRows];
@@ -706,7 +691,9 @@ __global__ void __launch_bounds__(kRows) bpoly_kernel(BLayout L, BSell A, const 
    const long s = row >> 6;
    const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
    const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
-   z[row] = z0[row] + c;
+   const T zr = z0[row] + c;
+   z[row] = zr;
+   if (mode == 1) e[row] += zr;
 }
 
 // u = sum_i y_i vs_i V[i]; accumulate: e += S u, else u and S u are
@@ -743,16 +730,6 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, BSell
       uout[row] = acc;
       zout[row] = s;
    }
-}
-
-// e += z for slots whose cycle took steps
-template <class T>
-__global__ void badd_kernel(long N, BLayout L, const T* z, BState S, T* e) {
-   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (row >= N) return;
-   const int b = slot_of_row(L, row);
-   if (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0) return;
-   e[row] += z[row];
 }
 
 // true residual V[0] = rhs - A e after the cycle, z = S V[0], and the stop decision
@@ -811,7 +788,9 @@ __global__ void __launch_bounds__(kRows) bspmv_kernel(BLayout L, BSell A, const 
 
 __global__ void bphase_kernel(int nslots, BState S) {
    int b = blockIdx.x * blockDim.x + threadIdx.x;
-   if (b >= nslots || S.phase[b] != kInner || !S.flags[4 * b]) return;
+   if (b >= nslots) return;
+   S.rebuild[b] = 0;  // consumed by this tick's shift and block inverses
+   if (S.phase[b] != kInner || !S.flags[4 * b]) return;
    S.inner[b] += S.flags[4 * b + 3];
    S.phase[b] = kOuter;
 }
@@ -1285,13 +1264,14 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    brf_kernel<<<nwg, kRows, 0, c.stream>>>(L, Rs, P->r_val.p, Fs, P->f_val.p, P->x.p, P->y1.p, P->y2.p, part, tk, gtk,
                                            S);
    bresid_kernel<<<nwg, kRows, 0, c.stream>>>(L, P->x.p, P->y1.p, P->y2.p, P->phi.p, P->r.p, o, part, tk, gtk, S);
+   // the shifted operator (and its FP32 copy in mixed pools) of slots being
+   // rebuilt; their rebuild flags are cleared by the tick's last kernel
+   float* a32 = nullptr;
+   if constexpr (sizeof(T) == sizeof(float)) a32 = I.aval.p;
    bshift_kernel<<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, Rs, P->r_val.p, Fs, P->f_val.p, S,
-                                                          P->a_val.p);
-   if (fp32)
-      to_t_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, S.rebuild, P->a_val.p, I.aval.p, Rs);
+                                                          P->a_val.p, a32);
    bblock_inverse_kernel<T><<<grid_for(rows, 64), 64, 0, c.stream>>>(rows, L, Rs, P->a_val.p, S.rebuild,
                                                                      I.binv.p, P->err.p);
-   bclear_rebuild_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
    // with the polynomial preconditioner the first term S t goes to z0 and
    // the step's bpoly kernel completes z; without it, straight into z
    T* zfirst = o.poly ? I.z0.p : I.z.p;
@@ -1354,7 +1334,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
       } else {
          if (o.poly && full)
             bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
-                                                         S, 0);
+                                                         S, 0, nullptr);
          bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S);
          if (full)
             borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S);
@@ -1364,8 +1344,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    if (o.poly) {
       // e += M^-1 u: u and S u, then the second term, then the sum
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
-      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1);
-      badd_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, I.z.p, S, I.e.p);
+      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1, I.e.p);
    } else {
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 1, nullptr,
                                                      nullptr);
