@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -123,7 +124,7 @@ struct Case {
 
 struct Server {
    pk_batch pool = nullptr;
-   int precision = PK_PREC_FP64, restart = 60, max_slots = 0;
+   int precision = PK_PREC_FP64, restart = 30, max_slots = 0;
    double pool_bytes = 0.0;
    double tol = 1e-8, reduction = 0.1, shift = 0.1;
    std::vector<std::unique_ptr<Case>> cases;
@@ -189,7 +190,12 @@ void pampa_serve_create(double budget_bytes, int max_slots, int precision, pampa
    // capacity from the budget: per row of the row space, room for 24 stored
    // entries of R (and A), 16 of F and 16 block-inverse values (16-group 3-D
    // operators); rows scale down until the pool fits the budget
-   const int restart = 60;
+   // GMRES restart length of the pool (one tick = one cycle): PETSc's 30, as
+   // in the drop-in.  On warm-started request streams 60 gave the same solves/s
+   // with a higher median latency, and slowed the one-slot FP64 engine by 9 %.
+   // PK_SERVE_RESTART overrides it (1..64) for tuning.
+   int restart = 30;
+   if (const char* e = getenv("PK_SERVE_RESTART")) restart = std::max(1, std::min(64, atoi(e)));
    long rows = 1L << 30;
    while (rows > 256 && pk_pool_bytes(rows, rows * 24, rows * 16, rows * 16, max_slots, precision, restart) >
                             budget_bytes)
