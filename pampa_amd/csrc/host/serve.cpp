@@ -193,9 +193,14 @@ void pampa_serve_create(double budget_bytes, int max_slots, int precision, pampa
    // GMRES restart length of the pool (one tick = one cycle): PETSc's 30, as
    // in the drop-in.  On warm-started request streams 60 gave the same solves/s
    // with a higher median latency, and slowed the one-slot FP64 engine by 9 %.
-   // PK_SERVE_RESTART overrides it (1..64) for tuning.
+   // PK_SERVE_RESTART overrides it (an integer in 1..64; anything else is
+   // ignored) for tuning.
    int restart = 30;
-   if (const char* e = getenv("PK_SERVE_RESTART")) restart = std::max(1, std::min(64, atoi(e)));
+   if (const char* e = getenv("PK_SERVE_RESTART")) {
+      char* end = nullptr;
+      const long v = strtol(e, &end, 10);
+      if (end != e && *end == '\0' && v >= 1 && v <= 64) restart = (int)v;
+   }
    long rows = 1L << 30;
    while (rows > 256 && pk_pool_bytes(rows, rows * 24, rows * 16, rows * 16, max_slots, precision, restart) >
                             budget_bytes)
