@@ -1,0 +1,96 @@
+"""VTK output of the drop-in (`-vtk`): the legacy ASCII file the reference's
+vtk.cxx:28-170 writes (header, points, cells, cell types, then the materials and
+the solution as cell scalars), read back and checked against the mesh sizes and
+the fields the C-ABI returns for the same solve.
+
+Not yet run on a GPU: kept under a name that pytest does not collect until it
+has passed there once."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import REF
+
+pytestmark = pytest.mark.gpu
+
+HEADER = ["# vtk DataFile Version 3.0", "FVM mesh", "ASCII", "DATASET UNSTRUCTURED_GRID"]
+VTK_TYPE = {2: 3, 3: 5, 4: 9, 6: 13, 8: 12, 12: 16}  # points per cell -> VTK cell type
+
+
+def read_vtk(path):
+    lines = open(path).read().split("\n")
+    assert lines[:4] == HEADER
+    out = {"scalars": {}}
+    i = 4
+    while i < len(lines):
+        t = lines[i].split()
+        if not t:
+            i += 1
+            continue
+        if t[0] == "POINTS":
+            n = int(t[1])
+            assert t[2] == "double"
+            out["points"] = np.array([[float(v) for v in lines[i + 1 + p].split()] for p in range(n)])
+            i += 1 + n
+        elif t[0] == "CELLS":
+            nc, total = int(t[1]), int(t[2])
+            cells = [[int(v) for v in lines[i + 1 + c].split()] for c in range(nc)]
+            assert all(c[0] == len(c) - 1 for c in cells)
+            assert total == sum(len(c) for c in cells)
+            out["cells"] = [c[1:] for c in cells]
+            i += 1 + nc
+        elif t[0] == "CELL_TYPES":
+            nc = int(t[1])
+            out["types"] = [int(lines[i + 1 + c]) for c in range(nc)]
+            i += 1 + nc
+        elif t[0] == "CELL_DATA":
+            out["ncell_data"] = int(t[1])
+            i += 1
+        elif t[0] == "SCALARS":
+            assert t[2:] == ["double", "1"] and lines[i + 1] == "LOOKUP_TABLE default"
+            nc = out["ncell_data"]
+            out["scalars"][t[1]] = np.array([float(lines[i + 2 + c]) for c in range(nc)])
+            i += 2 + nc
+        else:
+            raise AssertionError(f"unexpected line {i}: {lines[i]!r}")
+    return out
+
+
+@pytest.mark.parametrize("case", ["pwr-iaea-benchmark/cartesian-diffusion", "slabs/reflected-s2"])
+def test_vtk_output_matches_fields(case, tmp_path):
+    import pampa_amd
+    # the case's whole benchmark folder: inputs name files beside it (../materials)
+    top = case.split("/")[0]
+    shutil.copytree(os.path.join(REF, top), tmp_path / top)
+    d = tmp_path / case
+    print(f"[case] vtk {case}", flush=True)
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        with pampa_amd.DropIn("input.pmp", "-silent", "-vtk") as run:
+            run.solve()
+            C, G, M = run.sizes()
+            phi = run.field("scalar-flux").reshape(C, G)
+            q = run.field("power")
+            psi = run.field("angular-flux").reshape(C, G, M) if M > 1 else None
+    finally:
+        os.chdir(cwd)
+    v = read_vtk(str(d / "output_0.vtk"))
+    np_ = len(v["points"])
+    assert len(v["cells"]) == C and len(v["types"]) == C and v["ncell_data"] == C
+    assert all(0 <= p < np_ for c in v["cells"] for p in c)
+    assert v["types"] == [VTK_TYPE[len(c)] for c in v["cells"]]
+    mat = v["scalars"]["materials"]
+    assert np.all(mat >= 1) and np.all(mat == np.round(mat))
+    # fields: written with 7 significant digits
+    for g in range(G):
+        name = f"flux_{g + 1}" if G > 1 else "flux"
+        np.testing.assert_allclose(v["scalars"][name], phi[:, g], rtol=6e-7, atol=0)
+    np.testing.assert_allclose(v["scalars"]["power"], q, rtol=6e-7, atol=0)
+    if psi is not None:
+        for g in range(G):
+            for m in range(M):
+                name = "flux" + (f"_{g + 1}" if G > 1 else "") + f"_{m + 1}"
+                np.testing.assert_allclose(v["scalars"][name], psi[:, g, m], rtol=6e-7, atol=0)
