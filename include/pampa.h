@@ -21,21 +21,21 @@
 extern "C" {
 #endif
 
-/* pampa.hxx:9 / pampa.cxx:11-35 */
+/* pampa.hxx:6 / pampa.cxx:11-35 */
 void pampa_initialize(int argc, char* argv[], double** dt, int* ndt, int* error);
-/* pampa.hxx:12 / pampa.cxx:38-56 */
+/* pampa.hxx:9 / pampa.cxx:38-56 */
 void pampa_initialize_steady_state(int argc, char* argv[], int* error);
-/* pampa.hxx:15 / pampa.cxx:59-65 */
+/* pampa.hxx:12 / pampa.cxx:59-65 */
 void pampa_solve(int n, double dt, double t, int* error);
-/* pampa.hxx:18 / pampa.cxx:68-74 */
+/* pampa.hxx:15 / pampa.cxx:68-74 */
 void pampa_solve_steady_state(int* error);
-/* pampa.hxx:21 / pampa.cxx:77-86 */
+/* pampa.hxx:18 / pampa.cxx:77-86 */
 void pampa_finalize(double** dt, int* error);
-/* pampa.hxx:24 / pampa.cxx:89-95 */
+/* pampa.hxx:21 / pampa.cxx:89-95 */
 void pampa_finalize_steady_state(int* error);
-/* pampa.hxx:27 / pampa.cxx:98-104 */
+/* pampa.hxx:24 / pampa.cxx:98-104 */
 void pampa_get_field(double* v, const char name[], int* error);
-/* pampa.hxx:30 / pampa.cxx:107-113 */
+/* pampa.hxx:27 / pampa.cxx:107-113 */
 void pampa_set_field(const double* v, const char name[], int* error);
 
 /* Extensions (no reference counterpart). */

@@ -345,6 +345,12 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
    };
    double next_report = S->progress;
    int sn_busy = 0;  // S_N workers running
+   // no worker carries a request of an earlier run into this one
+   for (auto& cp : S->cases)
+      for (auto& w : cp->workers) {
+         if (w->thread.joinable()) w->thread.join();
+         w->req = -1;
+      }
    // a worker that has finished: its results into the request's outputs
    auto finish_sn = [&](Case::Worker& w) {
       w.thread.join();
@@ -413,7 +419,10 @@ void pampa_serve_run(pampa_server S, int n, const double* arrival, const int* ca
          const int r = pending.front();
          Case& c = *S->cases[case_id[r]];
          int fits = 0;
-         if (check(pk_pool_fits(S->pool, c.R, c.F, &fits))) return;
+         if (check(pk_pool_fits(S->pool, c.R, c.F, &fits))) {
+            join_all();
+            return;
+         }
          if (!fits && live == 0) {
             report_error(__FILE__, __LINE__, __func__, "request larger than the eigen pool");
             join_all();

@@ -469,10 +469,15 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
       if (tid == 129) sc[1] = S.resid[2 * b + 1];
    }
    if (tid == 130) sc[2] = vs[0];
+   // Every slot reduces in a window of m + 2 partials per workgroup at its own
+   // units, whatever its restart length: slots at the edge step (one value)
+   // and slots mid-cycle (j + 2 values) share this launch, and windows scaled
+   // by the value count would overlap between them.
+   double* const part = partials + (long)L.wg_first[b] * (m + 2);
    bool last;
    if (edge) {
       last = seg_reduce<1>(
-          1, [&](int) { return vj * vj; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], hd + j + 1,
+          1, [&](int) { return vj * vj; }, part, tickets + b, gtickets + L.wg_first[b], hd + j + 1,
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    } else {
       const long s = row >> 6;
@@ -486,7 +491,7 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
           [&](int k) {
              return k < j ? (double)V[(long)k * N + row] * vs[k] * wr : (k == j ? vj * wr : vj * vj);
           },
-          partials + (long)L.wg_first[b] * nv, tickets + b, gtickets + L.wg_first[b], hd,
+          part, tickets + b, gtickets + L.wg_first[b], hd,
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    }
    if (!last) return;
