@@ -43,7 +43,10 @@ void pampa_set_field(const double* v, const char name[], int* error);
  * NeutronicSolver.cxx:119-127). */
 void pampa_ext_get_result(double* keff, double* power, int* error);
 /* Assembled operator "R" or "F" of the main solver as CSR: call with null
- * arrays to get n and nnz, then with arrays of n+1 / nnz / nnz entries. */
+ * arrays to get n and nnz, then with arrays of n+1 / nnz / nnz entries.
+ * "b" is the right-hand side of the last time step (n > 0): nnz = 0 and the
+ * n values in val (row_ptr and col_idx are not used).  After a time step "R"
+ * is that step's operator (DiffusionSolver.cxx:134-175). */
 void pampa_ext_get_operator(const char name[], long* n, long* nnz, int* row_ptr, int* col_idx,
                             double* val, int* error);
 /* Outer iterations, inner iterations (Krylov steps or sweeps), final residual

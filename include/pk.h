@@ -124,6 +124,19 @@ int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const in
                                     const double* chip, const double* chid, const double* beta,
                                     double keff, pk_vec S, pk_vec phi0, pk_mat* R, pk_vec b);
 
+/* R and F of SNSolver::buildMatrices (SNSolver.cxx:344-631, n = 0) in the
+ * reference form, assembled on the device: row l = (i G + g) M + m holds the
+ * cell's dense G M block (total reaction, isotropic scattering; F: chi nusf w V)
+ * and one entry per physical neighbour (face-interpolation weights of
+ * SNSolver.cxx:159-208 with this delta; least-squares boundary correction when
+ * ls_coef != NULL; reflected directions on reflective boundaries), values
+ * accumulated in the reference's MatSetValues order.  dirs [M][3], wts [M],
+ * refl [M][3] (reflected direction per axis, AngularQuadratureSet.cxx:185-205);
+ * ls_coef [num_faces][3] per face slot (SNSolver.cxx:212-268, boundary cells),
+ * bc_type by bc id.  Replaces the MatSetValues loop of the S_N solver. */
+int pk_sn_assemble(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double* wts, const int* refl,
+                   double delta, const double* ls_coef, int num_bc, const int* bc_type, pk_mat* R, pk_mat* F);
+
 /* Power normalisation and power/production rates on the device
  * (NeutronicSolver.cxx:46-116): phi scaled so that sum phi kapf V = power
  * (power < 0: phi left as is, rates only); q[i] = sum_g phi kapf V,
@@ -145,8 +158,12 @@ int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer);
 int pk_eig_set_precision(pk_eig E, int precision);
 int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner);
 int pk_eig_set_shift(pk_eig E, double delta); /* Wielandt shift k_s = k + delta (0: none) */
-/* inner preconditioner: 0 = cell-block Jacobi B^-1; 1 (default) = B^-1 (2 I - A B^-1),
- * one Richardson correction (one more SpMV per GMRES step, about half the steps) */
+/* inner preconditioner, built on S = cell-block Jacobi B^-1 followed by one
+ * block-Jacobi sweep over each 256-row workgroup's diagonal block of A:
+ * 0 = S; 1 (default) = S (2 I - A S), one Richardson correction (one more SpMV
+ * per GMRES step, about half the steps); 2 = B^-1 alone (no sweep: for
+ * operators on which the sweep amplifies, such as S_N with downwind
+ * face interpolation, mixed-face-interpolation < 1) */
 int pk_eig_set_preconditioner(pk_eig E, int poly);
 int pk_eig_set_initial(pk_eig E, pk_vec x);
 int pk_eig_solve(pk_eig E);

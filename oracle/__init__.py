@@ -58,6 +58,10 @@ def _declare(L):
     L.orc_unst_build.argtypes = ([_I, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _I, _P, _I]
                                  + [_P] * 8)
     L.orc_diff_assemble.argtypes = [_I, _I] + [_P] * 8 + [_I] + [_P] * 13
+    L.orc_diff_assemble_transient.argtypes = ([_I, _I] + [_P] * 8 + [_I] + [_P] * 7 + [_D, _P, _P, _D]
+                                              + [_P] * 3)
+    L.orc_diff_rhs.argtypes = [_I, _I, _P, _P, _P, _P, _D, _P, _P, _P]
+    L.orc_diff_rhs.restype = None
     L.orc_sn_gauss.argtypes = [_I, _I, _P, _P, _P, _P, _D, _P]
     L.orc_sn_ls.argtypes = [_I, _I, _P, _P, _P, _P, _I, _P, _P]
     L.orc_sn_assemble.argtypes = ([_I, _I] + [_P] * 8 + [_I, _I] + [_P] * 4 + [_I]
@@ -251,15 +255,82 @@ def assemble_diffusion(mesh: Mesh, xs: XS, bct, bcv):
     return CSR(N, r_ptr, r_col, r_val), CSR(N, f_ptr, f_col, f_val)
 
 
-def eigen_sinvert(R: CSR, F: CSR, tol=1e-13, v0=None):
+@dataclass
+class TransientXS:
+    """Per-material time-step data: neutron velocity, prompt and delayed
+    spectra [nm][G], total delayed fraction [nm]."""
+    vel: np.ndarray
+    chip: np.ndarray
+    chid: np.ndarray
+    beta: np.ndarray
+
+
+def transient_tables(materials, G: int) -> TransientXS:
+    nm = len(materials)
+    t = TransientXS(np.zeros((nm, G)), np.zeros((nm, G)), np.zeros((nm, G)), np.zeros(nm))
+    for i, mat in enumerate(materials):
+        nd = mat.nd
+        if nd.velocity is None:
+            raise pmp.PmpError("missing neutron velocities")
+        t.vel[i] = nd.velocity
+        t.chip[i] = nd.chi_prompt
+        t.chid[i] = nd.chi_delayed
+        t.beta[i] = mat.beta_total
+    return t
+
+
+def assemble_diffusion_transient(mesh: Mesh, xs: XS, bct, bcv, tr: TransientXS, dt: float, keff: float):
+    """R of a time step n > 0 of DiffusionSolver::buildMatrices
+    (DiffusionSolver.cxx:134-148,156-175): the steady rows plus V/(v dt) on the
+    diagonal and the prompt fission -(1 - beta) chip(g) nusf(g2) V / keff on the
+    cell block."""
+    L = lib()
+    G = xs.G
+    N = mesh.C * G
+    r_ptr = np.zeros(N + 1, np.int32)
+    args = [mesh.C, mesh.fmax, _p(mesh.nfaces), _p(mesh.vol), _p(mesh.cen), _p(mesh.cmat),
+            _p(mesh.farea), _p(mesh.fcen), _p(mesh.fnorm), _p(mesh.fnb), G, _p(xs.sigt),
+            _p(xs.sigs), _p(xs.nusf), _p(xs.D), _p(bct), _p(bcv), _p(tr.vel), float(dt), _p(tr.chip),
+            _p(tr.beta), float(keff)]
+    if L.orc_diff_assemble_transient(*args, _p(r_ptr), None, None):
+        raise pmp.PmpError("boundary condition not implemented")
+    r_col = np.zeros(r_ptr[-1], np.int32)
+    r_val = np.zeros(r_ptr[-1])
+    L.orc_diff_assemble_transient(*args, _p(r_ptr), _p(r_col), _p(r_val))
+    return CSR(N, r_ptr, r_col, r_val)
+
+
+def diffusion_rhs(mesh: Mesh, tr: TransientXS, dt: float, S: np.ndarray, phi0: np.ndarray):
+    """b of a time step (DiffusionSolver.cxx:134-143): chid S + V/(v dt) phi0."""
+    G = tr.vel.shape[1]
+    b = np.zeros(mesh.C * G)
+    S = np.ascontiguousarray(S, np.float64)
+    phi0 = np.ascontiguousarray(phi0, np.float64)
+    lib().orc_diff_rhs(mesh.C, G, _p(mesh.cmat), _p(mesh.vol), _p(tr.chid), _p(tr.vel), float(dt), _p(S),
+                       _p(phi0), _p(b))
+    return b
+
+
+def linear_solve(A: CSR, b: np.ndarray) -> np.ndarray:
+    """The time step's linear system R phi = b (petsc::solve(KSP),
+    petsc.cxx:382-425, with the KSP of petsc.cxx:147-172) solved exactly by a
+    sparse LU: the solution the reference's iterative solve converges to."""
+    import scipy.sparse.linalg as sla
+    lu = sla.splu(A.scipy().tocsc(), permc_spec="MMD_AT_PLUS_A")
+    return lu.solve(np.asarray(b, np.float64))
+
+
+def eigen_sinvert(R: CSR, F: CSR, tol=1e-13, v0=None, permc="MMD_AT_PLUS_A"):
     """Largest 1/lambda of R x = lambda F x by shift-and-invert at sigma = 0:
     an exact sparse LU of R and an implicitly restarted Arnoldi iteration on
     R^-1 F, the algorithm the reference configures in SLEPc
-    (petsc.cxx:193-197).  Returns (keff, x) with x real and of positive sum."""
+    (petsc.cxx:193-197).  Returns (keff, x) with x real and of positive sum.
+    `permc` is SuperLU's column ordering (COLAMD keeps the fill of S_N
+    operators with downwind couplings, delta < 1, far smaller)."""
     import scipy.sparse.linalg as sla
     Rs = R.scipy().tocsc()
     Fs = F.scipy()
-    lu = sla.splu(Rs, permc_spec="MMD_AT_PLUS_A")
+    lu = sla.splu(Rs, permc_spec=permc)
     op = sla.LinearOperator(Rs.shape, matvec=lambda v: lu.solve(Fs @ v), dtype=np.float64)
     if v0 is None:
         v0 = np.ones(R.n)

@@ -370,22 +370,25 @@ enum { BC_NONE = 0, BC_VACUUM = 1, BC_REFLECTIVE = 2, BC_ROBIN = 3 };
  * chi[m*G+g] (effective spectrum), D[m*G+g].  Boundary table indexed by
  * bc id = boundary_index + 1: bct[id], bcv[id] (Robin f at t).
  * Two-pass: with r_col == NULL only r_ptr/f_ptr are filled.
- * Time-dependent rows (n > 0, DiffusionSolver.cxx:134-148,168-175) are added
- * when dtv != NULL: dtv[m*G+g] = 1/(v*dt), and then F is replaced by the
- * prompt-fission term folded into R with chip/beta/keff. */
-int orc_diff_assemble(int C, int fmax, const int* nfaces, const double* vol, const double* cen,
-                      const int* cmat, const double* farea, const double* fcen, const double* fnorm,
-                      const int* fnb, int G, const double* sigt, const double* sigs,
-                      const double* nusf, const double* chi, const double* D, const int* bct,
-                      const double* bcv, int* r_ptr, int* r_col, double* r_val, int* f_ptr,
-                      int* f_col, double* f_val) {
+ * Time-dependent rows (n > 0, DiffusionSolver.cxx:134-148,156-175) when
+ * vel != NULL: the diagonal gains d = V/(v dt) after the total-reaction term
+ * (:138-146), and every (g2 -> g) entry of the cell block gains the prompt
+ * fission -(1 - beta) chip(g) nusf(g2) V / keff after its scattering term
+ * (:168-175); F is not assembled then (f_ptr/f_col/f_val may be NULL). */
+static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, const double* cen,
+                     const int* cmat, const double* farea, const double* fcen, const double* fnorm,
+                     const int* fnb, int G, const double* sigt, const double* sigs,
+                     const double* nusf, const double* chi, const double* D, const int* bct,
+                     const double* bcv, const double* vel, double dt, const double* chip,
+                     const double* beta, double keff, int* r_ptr, int* r_col, double* r_val,
+                     int* f_ptr, int* f_col, double* f_val) {
    int cap = G + fmax;
    int* cols = malloc(sizeof(int) * cap);
    double* vals = malloc(sizeof(double) * cap);
    int err = ORC_OK;
    if (r_col == NULL) {
       r_ptr[0] = 0;
-      f_ptr[0] = 0;
+      if (f_ptr) f_ptr[0] = 0;
    }
    for (int i = 0; i < C && !err; i++) {
       int m = cmat[i];
@@ -395,15 +398,25 @@ int orc_diff_assemble(int C, int fmax, const int* nfaces, const double* vol, con
          int n = 0;
          cols[n] = l;
          vals[n] = sigt[m * G + g] * V;
+         if (vel) {
+            double d = V / (vel[m * G + g] * dt);
+            vals[0] += d;
+         }
          n = 1;
          for (int g2 = 0; g2 < G; g2++) {
             double s = -sigs[(m * G + g2) * G + g] * V;
+            int at = 0;
             if (g2 == g)
                vals[0] += s;
             else {
+               at = n;
                cols[n] = i * G + g2;
                vals[n] = s;
                n++;
+            }
+            if (vel) {
+               double pf = -(1.0 - beta[m]) * chip[m * G + g] * nusf[m * G + g2] * V / keff;
+               vals[at] += pf;
             }
          }
          for (int f = 0; f < nfaces[i]; f++) {
@@ -443,20 +456,61 @@ int orc_diff_assemble(int C, int fmax, const int* nfaces, const double* vol, con
          int nr = aij_merge(n, cols, vals, 0);
          if (r_col == NULL) {
             r_ptr[l + 1] = r_ptr[l] + nr;
-            f_ptr[l + 1] = f_ptr[l] + G;
+            if (f_ptr) f_ptr[l + 1] = f_ptr[l] + G;
          } else {
             memcpy(r_col + r_ptr[l], cols, sizeof(int) * nr);
             memcpy(r_val + r_ptr[l], vals, sizeof(double) * nr);
-            for (int g2 = 0; g2 < G; g2++) {
-               f_col[f_ptr[l] + g2] = i * G + g2;
-               f_val[f_ptr[l] + g2] = chi[m * G + g] * nusf[m * G + g2] * V;
-            }
+            if (f_col)
+               for (int g2 = 0; g2 < G; g2++) {
+                  f_col[f_ptr[l] + g2] = i * G + g2;
+                  f_val[f_ptr[l] + g2] = chi[m * G + g] * nusf[m * G + g2] * V;
+               }
          }
       }
    }
    free(cols);
    free(vals);
    return err;
+}
+
+/* Steady rows (n = 0): R and F. */
+int orc_diff_assemble(int C, int fmax, const int* nfaces, const double* vol, const double* cen,
+                      const int* cmat, const double* farea, const double* fcen, const double* fnorm,
+                      const int* fnb, int G, const double* sigt, const double* sigs,
+                      const double* nusf, const double* chi, const double* D, const int* bct,
+                      const double* bcv, int* r_ptr, int* r_col, double* r_val, int* f_ptr,
+                      int* f_col, double* f_val) {
+   return diff_rows(C, fmax, nfaces, vol, cen, cmat, farea, fcen, fnorm, fnb, G, sigt, sigs, nusf, chi,
+                    D, bct, bcv, NULL, 0.0, NULL, NULL, 0.0, r_ptr, r_col, r_val, f_ptr, f_col, f_val);
+}
+
+/* Time-step rows (n > 0): R only, with vel[m*G+g], chip[m*G+g], beta[m]
+ * (the material's total delayed fraction) and the keff of the steady solve. */
+int orc_diff_assemble_transient(int C, int fmax, const int* nfaces, const double* vol,
+                                const double* cen, const int* cmat, const double* farea,
+                                const double* fcen, const double* fnorm, const int* fnb, int G,
+                                const double* sigt, const double* sigs, const double* nusf,
+                                const double* D, const int* bct, const double* bcv,
+                                const double* vel, double dt, const double* chip,
+                                const double* beta, double keff, int* r_ptr, int* r_col,
+                                double* r_val) {
+   return diff_rows(C, fmax, nfaces, vol, cen, cmat, farea, fcen, fnorm, fnb, G, sigt, sigs, nusf, NULL,
+                    D, bct, bcv, vel, dt, chip, beta, keff, r_ptr, r_col, r_val, NULL, NULL, NULL);
+}
+
+/* Right-hand side of a time step (DiffusionSolver.cxx:134-143):
+ * b(i,g) = chid(g) S_i, then += V/(v dt) phi0(i,g). */
+void orc_diff_rhs(int C, int G, const int* cmat, const double* vol, const double* chid,
+                  const double* vel, double dt, const double* S, const double* phi0, double* b) {
+   for (int i = 0; i < C; i++) {
+      int m = cmat[i];
+      for (int g = 0; g < G; g++) {
+         double v = chid[m * G + g] * S[i];
+         double d = vol[i] / (vel[m * G + g] * dt);
+         v += d * phi0[i * G + g];
+         b[i * G + g] = v;
+      }
+   }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -570,7 +624,11 @@ int orc_sn_assemble(int C, int fmax, const int* nfaces, const double* vol, const
             /* The per-face MatSetValues(ADD) calls come first, then the block row
              * (SNSolver.cxx:512,551,595 before :602).  Collect in that order. */
             int n = 0;
+            /* the diagonal starts with the total reaction and the (g, m)
+             * self-scattering, and only then takes the face terms
+             * (SNSolver.cxx:397,426-428 before :461-597) */
             double diag = sigt[mt * G + g] * V;
+            diag += -sigs[(mt * G + g) * G + g] * wts[m] * V;
             for (int f = 0; f < nfaces[i]; f++) {
                int s = i * fmax + f, i2 = fnb[s];
                const double* om = dirs + 3 * m;
@@ -632,22 +690,17 @@ int orc_sn_assemble(int C, int fmax, const int* nfaces, const double* vol, const
                   n++;
                }
             }
-            /* block row: diagonal carries the total, scattering on the block */
-            int nd = n;
+            /* block row: the diagonal, then scattering on the rest of the block */
             cols[n] = l;
             vals[n] = diag;
             n++;
             for (int g2 = 0; g2 < G; g2++)
                for (int m2 = 0; m2 < M; m2++) {
                   int l2 = (i * G + g2) * M + m2;
-                  double sv = -sigs[(mt * G + g2) * G + g] * wts[m2] * V;
-                  if (l2 == l)
-                     vals[nd] += sv;
-                  else {
-                     cols[n] = l2;
-                     vals[n] = sv;
-                     n++;
-                  }
+                  if (l2 == l) continue;
+                  cols[n] = l2;
+                  vals[n] = -sigs[(mt * G + g2) * G + g] * wts[m2] * V;
+                  n++;
                }
             int nr = aij_merge(n, cols, vals, 1);
             if (r_col == NULL) {

@@ -610,6 +610,12 @@ class DropIn:
                                       ct.byref(err))
         if err.value:
             raise PkError(f"operator {name} not available")
+        if name == "b":
+            v = np.zeros(n.value)
+            self.L.pampa_ext_get_operator(b"b", ct.byref(n), ct.byref(nnz), None, None, ptr(v), ct.byref(err))
+            if err.value:
+                raise PkError("right-hand side not available")
+            return v
         p = np.zeros(n.value + 1, np.int32)
         c = np.zeros(max(nnz.value, 1), np.int32)
         v = np.zeros(max(nnz.value, 1))

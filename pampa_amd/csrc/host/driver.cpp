@@ -53,8 +53,9 @@ int Driver::setOption(const std::string& name, const std::string& value) {
    } else if (n == "ksp_max_it") {
       PAMPA_CHECK(input::read(options.ksp_max_it, 1, INT_MAX, value), "wrong ksp_max_it");
    } else if (n == "pk_preconditioner") {
-      PAMPA_CHECK(value != "bj" && value != "bj2", "wrong pk_preconditioner (bj or bj2)");
-      options.preconditioner = value == "bj2" ? 1 : 0;
+      PAMPA_CHECK(value != "bj" && value != "bj2" && value != "cell",
+                  "wrong pk_preconditioner (bj, bj2 or cell)");
+      options.preconditioner = value == "bj2" ? 1 : (value == "cell" ? 2 : 0);
    } else if (n == "ksp_gmres_restart") {
       PAMPA_CHECK(input::read(options.gmres_restart, 1, 64, value), "wrong ksp_gmres_restart");
    } else if (n == "pk_precision") {

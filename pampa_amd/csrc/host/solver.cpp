@@ -219,7 +219,7 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       PK_CHECK(pk_eig_set_precision(eps, options.precision));
       PK_CHECK(pk_eig_set_inner(eps, options.gmres_restart, options.inner_reduction, 2000));
       PK_CHECK(pk_eig_set_shift(eps, options.wielandt_shift));
-      PK_CHECK(pk_eig_set_preconditioner(eps, options.preconditioner));
+      PK_CHECK(pk_eig_set_preconditioner(eps, options.preconditioner < 0 ? 1 : options.preconditioner));
       PK_CHECK(pk_eig_set_initial(eps, phi));
       int rc = pk_eig_solve(eps);
       if (rc) {
@@ -302,6 +302,14 @@ int DiffusionSolver::timeKernel(int reps, int fp32, double* seconds, double* byt
 
 int DiffusionSolver::getOperator(const std::string& which, long* n, long* nnz, int* ptr, int* col,
                                  double* val) const {
+   if (which == "b") {
+      // right-hand side of the last time step: n values in val, no pattern
+      PAMPA_CHECK(!b, "the time-step right-hand side is not assembled");
+      *n = pk_vec_size(b);
+      *nnz = 0;
+      if (val) PK_CHECK(pk_vec_get_values(b, val));
+      return 0;
+   }
    pk_mat M = which == "R" ? R : which == "F" ? F : nullptr;
    PAMPA_CHECK(!M, "operator '" + which + "' is not assembled");
    PK_CHECK(pk_mat_get_size(M, n, nnz));
@@ -471,6 +479,11 @@ int SNSolver::solve(int n, double dt, double t) {
       PK_CHECK(pk_eig_set_precision(eps, options.precision));
       PK_CHECK(pk_eig_set_inner(eps, options.gmres_restart, options.inner_reduction, 20000));
       PK_CHECK(pk_eig_set_shift(eps, options.wielandt_shift));
+      // with downwind face interpolation (delta < 1) the workgroup-block sweep
+      // of the preconditioner amplifies instead of smoothing (|I - A S| ~ 40 on
+      // such cores, where B^-1 alone converges): cell-block Jacobi there
+      int prec = options.preconditioner < 0 ? (delta < 1.0 ? 2 : 1) : options.preconditioner;
+      PK_CHECK(pk_eig_set_preconditioner(eps, prec));
       PK_CHECK(pk_eig_set_initial(eps, psi));
       int rc = pk_eig_solve(eps);
       if (rc) {

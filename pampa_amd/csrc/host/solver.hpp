@@ -18,7 +18,9 @@ struct SolverOptions {
    int gmres_restart = 30;  // PETSc's default GMRES restart
    double inner_reduction = 0.1;
    double wielandt_shift = 0.1;
-   int preconditioner = 1;  // 0 cell-block Jacobi, 1 with one Richardson correction
+   // 0 cell-block Jacobi + workgroup-block sweep (bj), 1 with one Richardson
+   // correction (bj2), 2 cell-block Jacobi alone (cell); -1: the solver's choice
+   int preconditioner = -1;
    double ksp_rtol = 1e-5, ksp_atol = 1e-50;  // PETSc KSP defaults
    int ksp_max_it = 10000;
    double sn_tol_k = 1e-9, sn_tol_phi = 1e-8;

@@ -277,6 +277,210 @@ __global__ void __launch_bounds__(kBlock) scale_power_kernel(MeshView M, XSView 
    block_reduce_finish<2>(v, partials, ticket, out);
 }
 
+// ---- reference-form S_N operator (SNSolver.cxx:159-208,344-631) ----------------------
+
+// Face-interpolation weights of interior faces (SNSolver.cxx:193-198):
+// coef[4 f + 0..1] outgoing (w_i, w_i2), coef[4 f + 2..3] incoming.
+__global__ void sn_gauss_kernel(MeshView M, double delta, double* coef) {
+#pragma clang fp contract(off)
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= M.C) return;
+   const double* ci = M.cen + 3 * (long)i;
+   for (int f = M.fptr[i]; f < M.fptr[i + 1]; f++) {
+      const int i2 = M.fnb[f];
+      if (i2 < 0) continue;
+      const double* xf = M.fcen + 3 * (long)f;
+      const double* c2 = M.cen + 3 * (long)i2;
+      // math::distance: squares summed in coordinate order (math.cxx:82-92)
+      double rif = 0.0, ri2f = 0.0, rii2 = 0.0;
+      for (int k = 0; k < 3; k++) {
+         double a = ci[k] - xf[k], b = c2[k] - xf[k], d = c2[k] - ci[k];
+         rif += a * a;
+         ri2f += b * b;
+         rii2 += d * d;
+      }
+      rif = sqrt(rif);
+      ri2f = sqrt(ri2f);
+      rii2 = sqrt(rii2);
+      coef[4 * (long)f + 0] = (ri2f + delta * rif) / rii2;
+      coef[4 * (long)f + 1] = (1.0 - delta) * rif / rii2;
+      coef[4 * (long)f + 2] = (1.0 - delta) * ri2f / rii2;
+      coef[4 * (long)f + 3] = (rif + delta * ri2f) / rii2;
+   }
+}
+
+struct SnRefView {
+   int M;               // directions
+   const double* dirs;  // [M][3]
+   const double* wts;   // [M]
+   const int* refl;     // [M][3] reflected direction per axis
+   const double* coef;  // [faces][4]
+   const double* lsc;   // [faces][3] least-squares weights of boundary cells, or null (LS off)
+   const int* bct;      // boundary type by bc id
+};
+
+constexpr int kMaxRefl = 8;
+
+// One thread per row l = (i G + g) M + m of R (and of F when fcol != null).
+// R accumulates in the reference's order: per face, the MatSetValues(ADD)
+// calls of the neighbour, least-squares and reflected entries
+// (SNSolver.cxx:512,551,595), then the block row (:602), whose diagonal holds
+// the total reaction, the self-scattering and then every face term in face
+// order (:397,426-428,479-597).  A neighbour reached through several faces
+// keeps one entry (sums in face order); the row is written sorted by column.
+__global__ void sn_fill_kernel(MeshView M, XSView X, SnRefView Q, const int* rptr, int* rcol, double* rval,
+                               int* fcol, double* fval, int* err) {
+#pragma clang fp contract(off)
+   const int G = X.G, Md = Q.M, GM = G * Md;
+   const long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (l >= (long)M.C * GM) return;
+   const int i = (int)(l / GM), gm = (int)(l % GM), g = gm / Md, m = gm % Md;
+   const int mt = M.mat[i];
+   const double V = M.vol[i];
+   const double* om = Q.dirs + 3 * m;
+   const double* ci = M.cen + 3 * (long)i;
+   double diag = X.sigt[mt * G + g] * V;
+   diag += -X.sigs[(mt * G + g) * G + g] * Q.wts[m] * V;
+   int ncol[kMaxFaces];
+   double nval[kMaxFaces];
+   int nn = 0;
+   int rdir[kMaxRefl];
+   double rv[kMaxRefl];
+   int nr = 0;
+   auto add_nb = [&](int c, double v) {
+      int k = 0;
+      while (k < nn && ncol[k] != c) k++;
+      if (k == nn) {
+         if (nn == kMaxFaces) {
+            *err = 2;
+            return;
+         }
+         ncol[nn] = c;
+         nval[nn++] = v;
+      } else {
+         nval[k] += v;
+      }
+   };
+   const int f0 = M.fptr[i], f1 = M.fptr[i + 1];
+   for (int f = f0; f < f1; f++) {
+      const int i2 = M.fnb[f];
+      const double* nf = M.fnorm + 3 * (long)f;
+      double w = 0.0;
+      w += om[0] * nf[0];
+      w += om[1] * nf[1];
+      w += om[2] * nf[2];
+      const double A = M.farea[f];
+      if (i2 < 0) {
+         const int t = Q.bct[-i2];
+         if (t == PK_BC_VACUUM) {
+            if (w > 0.0) {
+               diag += w * A;
+               if (Q.lsc) {
+                  const double* xf = M.fcen + 3 * (long)f;
+                  const double dp[3] = {xf[0] - ci[0], xf[1] - ci[1], xf[2] - ci[2]};
+                  for (int f2 = f0; f2 < f1; f2++) {
+                     const int i3 = M.fnb[f2];
+                     if (i3 < 0) continue;
+                     const double* gc = Q.lsc + 3 * (long)f2;
+                     double wi3 = 0.0;
+                     wi3 += dp[0] * gc[0];
+                     wi3 += dp[1] * gc[1];
+                     wi3 += dp[2] * gc[2];
+                     diag += -wi3 * w * A;
+                     add_nb(i3, wi3 * w * A);
+                  }
+               }
+            }
+         } else if (t == PK_BC_REFLECTIVE) {
+            if (w > 0.0) {
+               diag += w * A;
+            } else {
+               // reflection axis from the normal of face (f - f0) of cells 0, 1, 2
+               // (the cell index is shadowed there, SNSolver.cxx:541-545)
+               const int lf = f - f0;
+               int m2 = -1;
+               for (int a = 0; a < 3; a++) {
+                  if (a >= M.C || lf >= M.fptr[a + 1] - M.fptr[a]) {
+                     *err = 3;
+                     return;
+                  }
+                  const double d = M.fnorm[3 * (long)(M.fptr[a] + lf) + a];
+                  if (fabs(d) > 1.0 - 1.0e-6) m2 = Q.refl[3 * m + a];
+               }
+               if (m2 < 0) {
+                  *err = 3;
+                  return;
+               }
+               int k = 0;
+               while (k < nr && rdir[k] != m2) k++;
+               if (k == nr) {
+                  if (nr == kMaxRefl) {
+                     *err = 2;
+                     return;
+                  }
+                  rdir[nr] = m2;
+                  rv[nr++] = w * A;
+               } else {
+                  rv[k] += w * A;
+               }
+            }
+         } else {
+            *err = 1;
+         }
+      } else {
+         const double* cf = Q.coef + 4 * (long)f;
+         const double wi = w > 0.0 ? cf[0] : cf[2];
+         const double wi2 = w > 0.0 ? cf[1] : cf[3];
+         diag += wi * w * A;
+         add_nb(i2, wi2 * w * A);
+      }
+   }
+   // neighbours by cell index
+   for (int a = 1; a < nn; a++) {
+      const int c = ncol[a];
+      const double v = nval[a];
+      int b = a - 1;
+      while (b >= 0 && ncol[b] > c) {
+         ncol[b + 1] = ncol[b];
+         nval[b + 1] = nval[b];
+         b--;
+      }
+      ncol[b + 1] = c;
+      nval[b + 1] = v;
+   }
+   long p = rptr[l];
+   int a = 0;
+   for (; a < nn && ncol[a] < i; a++, p++) {
+      rcol[p] = ncol[a] * GM + gm;
+      rval[p] = nval[a];
+   }
+   const long c0 = (long)i * GM;
+   for (int g2 = 0; g2 < G; g2++) {
+      const double sv = -X.sigs[(mt * G + g2) * G + g];
+      for (int m2 = 0; m2 < Md; m2++, p++) {
+         rcol[p] = (int)(c0 + g2 * Md + m2);
+         double v = (g2 == g && m2 == m) ? diag : sv * Q.wts[m2] * V;
+         if (g2 == g)
+            for (int k = 0; k < nr; k++)
+               if (rdir[k] == m2) v = rv[k] + v;
+         rval[p] = v;
+      }
+   }
+   for (; a < nn; a++, p++) {
+      rcol[p] = ncol[a] * GM + gm;
+      rval[p] = nval[a];
+   }
+   if (fcol) {
+      const long q = l * GM;
+      const double cn = X.chi[mt * G + g];
+      for (int g2 = 0; g2 < G; g2++)
+         for (int m2 = 0; m2 < Md; m2++) {
+            fcol[q + g2 * Md + m2] = (int)(c0 + g2 * Md + m2);
+            fval[q + g2 * Md + m2] = cn * X.nusf[mt * G + g2] * Q.wts[m2] * V;
+         }
+   }
+}
+
 }  // namespace pk
 
 using namespace pk;
@@ -453,6 +657,61 @@ int pk_diffusion_assemble(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type
                           const double* bc_value, pk_mat* R, pk_mat* F) {
    return diffusion_assemble_impl(mesh, xs, num_bc, bc_type, bc_value, nullptr, 0.0, nullptr,
                                   nullptr, 1.0, R, F);
+}
+
+int pk_sn_assemble(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double* wts, const int* refl,
+                   double delta, const double* ls_coef, int num_bc, const int* bc_type, pk_mat* R, pk_mat* F) {
+   PK_REQUIRE(mesh && xs && R && F && dirs && wts && refl && bc_type && M > 0, "null argument");
+   Ctx& c = ctx();
+   const int G = xs->G, GM = G * M;
+   const long N = (long)mesh->C * GM;
+   PK_REQUIRE(N < (1L << 31) && N * (long)GM < (1L << 40), "operator too large for 32-bit indices");
+   DBuf<double> ddir, dwts, dcoef, dlsc;
+   DBuf<int> drefl, dbct, derr;
+   PK_REQUIRE(!upload(ddir, dirs, (size_t)3 * M) && !upload(dwts, wts, (size_t)M) &&
+                  !upload(drefl, refl, (size_t)3 * M) && !upload(dbct, bc_type, (size_t)num_bc) &&
+                  !dcoef.alloc((size_t)4 * (mesh->F ? mesh->F : 1)) && !derr.alloc(1),
+              "device allocation failed");
+   if (ls_coef) PK_REQUIRE(!upload(dlsc, ls_coef, (size_t)3 * mesh->F), "device allocation failed");
+   PK_HIP(hipMemsetAsync(derr.p, 0, sizeof(int), c.stream));
+   PK_HIP(hipMemsetAsync(dcoef.p, 0, sizeof(double) * 4 * (mesh->F ? mesh->F : 1), c.stream));
+   sn_gauss_kernel<<<grid_for(mesh->C), kBlock, 0, c.stream>>>(view(mesh), delta, dcoef.p);
+   PK_HIP(hipGetLastError());
+   pk_mat_s* r = new pk_mat_s();
+   r->n = N;
+   r->bs = GM;
+   pk_mat_s* f = new pk_mat_s();
+   f->n = N;
+   f->bs = GM;
+   f->nnz = N * GM;
+   // R: the cell's G M block and one entry per distinct physical neighbour
+   // (the diffusion pattern with blocks of G M rows)
+   if (diffusion_pattern(mesh, GM, r->ptr, r->nnz) || r->col.alloc(r->nnz) || r->val.alloc(r->nnz) ||
+       f->ptr.alloc(N + 1) || f->col.alloc(f->nnz) || f->val.alloc(f->nnz)) {
+      pk_mat_destroy(&r);
+      pk_mat_destroy(&f);
+      PK_FAIL("device allocation failed for the S_N operator");
+   }
+   seq_ptr_kernel<<<grid_for(N + 1), kBlock, 0, c.stream>>>(f->ptr.p, N, GM);
+   SnRefView Q{M, ddir.p, dwts.p, drefl.p, dcoef.p, ls_coef ? dlsc.p : nullptr, dbct.p};
+   sn_fill_kernel<<<grid_for(N), kBlock, 0, c.stream>>>(view(mesh), view(xs), Q, r->ptr.p, r->col.p, r->val.p,
+                                                        f->col.p, f->val.p, derr.p);
+   PK_HIP(hipGetLastError());
+   int herr = 0;
+   PK_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   if (herr) {
+      pk_mat_destroy(&r);
+      pk_mat_destroy(&f);
+      PK_FAIL(herr == 1   ? "boundary condition not implemented"
+              : herr == 2 ? "too many neighbours per cell"
+                          : "reflected direction not found");
+   }
+   PK_CALL(finalize_device_csr(r));
+   PK_CALL(finalize_device_csr(f));
+   *R = r;
+   *F = f;
+   return 0;
 }
 
 int pk_diffusion_refill(pk_mesh mesh, pk_xs xs, pk_mat R, pk_mat F) {

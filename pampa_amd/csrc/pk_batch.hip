@@ -54,6 +54,7 @@ struct BLayout {
    const int* fbs;         // [S] cell block of F
    const long* binv_off;   // [S]
    const int* mlen;        // [S] restart length of the slot (<= the pool's)
+   int sweep;              // 1: S includes the workgroup-block sweep, 0: S = B^-1
 };
 
 struct BSell {
@@ -98,6 +99,7 @@ struct BOpts {
    double tol, delta, reduction;
    int max_outer, max_inner;
    int poly;  // 1: preconditioner S (2 I - A S) (one Richardson correction), 0: S (precond_apply)
+   int sweep = 1;  // 1: S = B^-1 followed by the workgroup-block sweep; 0: S = B^-1
 };
 
 __device__ __forceinline__ int slot_of_wg(const BLayout& L) { return L.unit_slot[blockIdx.x]; }
@@ -368,6 +370,10 @@ template <class T>
 __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* binv, const BSell& A, const T* aval,
                                            T t, T* tl, T* xl) {
    const T x = cell_block_apply<T>(L, b, binv, t, tl);
+   if (!L.sweep) {
+      __syncthreads();  // tl is staged again by the caller's next apply
+      return x;
+   }
    xl[threadIdx.x] = x;
    __syncthreads();
    const long row0 = blockIdx.x * (long)kRows;
@@ -945,7 +951,8 @@ struct pk_batch_s {
    double seconds = 0.0;
 
    BLayout layout() const {
-      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p};
+      return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p,
+                     opts.sweep};
    }
    BSell rsell() const { return BSell{r_off.p, r_width.p, r_col.p, rlen.p}; }
    BSell fsell() const { return BSell{f_off.p, f_width.p, f_col.p, rlen.p}; }
@@ -1639,8 +1646,9 @@ int pk_batch_set_inner(pk_batch P, int restart, double reduction, int max_inner)
 }
 
 int pk_batch_set_preconditioner(pk_batch P, int poly) {
-   PK_REQUIRE(P && (poly == 0 || poly == 1), "wrong preconditioner");
-   P->opts.poly = poly;
+   PK_REQUIRE(P && (poly == 0 || poly == 1 || poly == 2), "wrong preconditioner");
+   P->opts.poly = poly == 1;
+   P->opts.sweep = poly != 2;
    P->graph.clear();
    return 0;
 }
@@ -1749,7 +1757,7 @@ int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction,
    PK_REQUIRE(P && tol > 0.0 && max_outer > 0 && reduction > 0.0 && reduction < 1.0 && max_inner > 0 &&
                   delta >= 0.0,
               "wrong pool options");
-   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner, P->opts.poly};
+   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner, P->opts.poly, P->opts.sweep};
    P->graph.clear();
    return 0;
 }

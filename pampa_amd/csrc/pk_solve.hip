@@ -599,7 +599,13 @@ int pk_lin_solve(pk_lin L, pk_vec b, pk_vec x) {
    long n = L->A->n;
    if (!L->ready) {
       PK_REQUIRE(!L->r.alloc(n), "device allocation failed");
-      PK_CALL(L->g.setup(&L->A->sell, L->A->sell.val.p, n, L->A->bs, 30));
+      // block-Jacobi blocks: the cell block when whole cells fit a workgroup,
+      // else its largest power-of-two divisor up to 16 (a block-diagonal part
+      // of the cell block; point Jacobi for odd group counts)
+      int pb = 1;
+      while (pb * 2 <= kMaxBs && L->A->bs % (pb * 2) == 0) pb *= 2;
+      if (L->A->bs <= kMaxBs && kBlock % L->A->bs == 0) pb = L->A->bs;
+      PK_CALL(L->g.setup(&L->A->sell, L->A->sell.val.p, n, pb, 30));
       PK_CALL(L->g.factor(L->A->sell));
       L->ready = true;
    }

@@ -13,6 +13,7 @@ Workloads:
   quarter_core_2d   170 x 170 cm, 1 cm cells, 17 x 17 blocks, 4 materials, 2 groups
   hex_sn_3d         hexagonal lattice of triangular prisms, 4 groups, S_N
   batched_2d        one instance of the batched diffusion family (side, groups drawn)
+  transient_core_2d small quarter core with time-step data (velocities, spectra, precursors)
 """
 from __future__ import annotations
 
@@ -47,7 +48,10 @@ def _fmt(v: float) -> str:
     return repr(float(v))
 
 
-def write_material(path: str, G: int, sigt, sigs, D=None, nusf=None, chi=None):
+def write_material(path: str, G: int, sigt, sigs, D=None, nusf=None, chi=None, chid=None, vel=None,
+                   lam=None, beta=None):
+    """One material file; chid (delayed spectrum), vel (neutron velocity) and a
+    precursor block (decay constants lam, fractions beta) are written when given."""
     lines = ["nuclear-data {", f"   energy-groups {G}", "   sigma-total",
              "   " + " ".join(_fmt(v) for v in sigt)]
     if nusf is not None:
@@ -57,7 +61,15 @@ def write_material(path: str, G: int, sigt, sigs, D=None, nusf=None, chi=None):
         lines += ["   diffusion-coefficient", "   " + " ".join(_fmt(v) for v in D)]
     if chi is not None:
         lines += ["   fission-spectrum", "   " + " ".join(_fmt(v) for v in chi)]
+    if chid is not None:
+        lines += ["   fission-spectrum-delayed", "   " + " ".join(_fmt(v) for v in chid)]
+    if vel is not None:
+        lines += ["   neutron-velocity", "   " + " ".join(_fmt(v) for v in vel)]
     lines += ["}"]
+    if lam is not None:
+        lines += ["precursor-data {", f"   precursor-groups {len(lam)}",
+                  "   lambda", "   " + " ".join(_fmt(v) for v in lam),
+                  "   beta", "   " + " ".join(_fmt(v) for v in beta), "}"]
     if nusf is not None:
         lines += ["fuel 1"]
     with open(path, "w") as fh:
@@ -243,6 +255,37 @@ def quarter_core_2d(d: str, seed: int = 1, side: int = 170, pitch: float = 1.0, 
         fh.write("bc -x reflective\nbc +x vacuum\nbc -y reflective\nbc +y vacuum\n\n")
         fh.write(f"materials {side * side}\n{_materials_line(layout)}\n")
     _write_input(d, [n for n, _ in mats], ["cartesian", "diffusion", f"energy-groups {G}"])
+    return d
+
+
+def transient_core_2d(d: str, seed: int = 7, side: int = 24, G: int = 3, dt: float = 0.01, steps: int = 2):
+    """Quarter core of side x side 1 cm cells (as quarter_core_2d) whose
+    materials carry time-step data: every material a neutron velocity per group
+    (U[1e5, 2e7] fast ... U[2e3, 1e4] thermal, geometric in the group), fuel
+    materials prompt spectrum (chi of the group set), a delayed spectrum
+    U-drawn and normalised (different from the prompt one), and two precursor
+    groups (lambda U[0.01, 3], beta U[5e-4, 3e-3]).  The input asks for
+    `steps` time steps of `dt` s.  Draw order: the quarter-core draws, then per
+    material velocities, delayed spectrum, lambda, beta."""
+    quarter_core_2d(d, seed=seed, side=side, G=G)
+    rng = SplitMix64(seed ^ 0x5DEECE66D)
+    names = ["fuel-a", "fuel-b", "fuel-absorber", "reflector"]
+    # re-read the drawn base materials by redrawing them with the same seed
+    base = SplitMix64(seed)
+    mats = [multigroup_material(base, G, True), multigroup_material(base, G, True)]
+    fac = base.uniform(1.5, 2.5)
+    mats += [multigroup_material(base, G, True, absorber_factor=fac), multigroup_material(base, G, False)]
+    for n, m in zip(names, mats):
+        vel = [rng.uniform(1e5, 2e7) * (2e3 / 1e5) ** (g / (G - 1)) for g in range(G)]
+        extra = dict(vel=vel)
+        if "nusf" in m:
+            raw = [rng.uniform(0.1, 1.0) for _ in range(G)]
+            extra["chid"] = [v / sum(raw) for v in raw]
+            extra["lam"] = [rng.uniform(0.01, 3.0) for _ in range(2)]
+            extra["beta"] = [rng.uniform(5e-4, 3e-3) for _ in range(2)]
+        write_material(os.path.join(d, n + ".pmp"), **m, **extra)
+    with open(os.path.join(d, "input.pmp"), "a") as fh:
+        fh.write(f"\ndt -{steps}\n{_fmt(dt)}\n")
     return d
 
 

@@ -250,3 +250,147 @@ def test_sweep_eigen_equals_assembled(oracle_mod, tmp_path):
     a = phiw / np.linalg.norm(phiw)
     b = pw / np.linalg.norm(pw)
     assert np.linalg.norm(a - b) < 1e-8
+
+
+# ---- transient rows (time steps n > 0) -------------------------------------------------
+
+def _transient_setup(oracle_mod, d):
+    case = oracle_mod.pmp.read_input(os.path.join(d, "input.pmp"))
+    mesh = oracle_mod.build_mesh(case)
+    xs = oracle_mod.xs_tables(case.materials, case.solver.G, True)
+    bct, bcv = oracle_mod.bc_table(mesh, case.solver.bcs)
+    tr = oracle_mod.transient_tables(case.materials, case.solver.G)
+    return case, mesh, xs, bct, bcv, tr
+
+
+def _production(mesh, xs, phi, keff):
+    """P_i = sum_g nuSf phi V / keff (NeutronicSolver.cxx:95-104)."""
+    G = xs.G
+    return np.sum(phi.reshape(mesh.C, G) * xs.nusf[mesh.cmat], axis=1) * mesh.vol / keff
+
+
+def test_transient_step_from_steady_state_is_stationary(oracle_mod, tmp_path):
+    """From the steady eigenpair, with the equilibrium delayed source
+    S_i = beta P_i, a time step returns the same flux: R_t phi0 = b holds
+    because chi_eff = (1 - beta) chi_p + beta chi_d.  Prompt and delayed spectra
+    differ and every material has its own beta and velocities, so a dropped or
+    mis-signed term, a spectrum taken at the wrong group index or the two
+    spectra exchanged breaks the identity."""
+    from pampa_amd import workloads as W
+    d = W.transient_core_2d(str(tmp_path / "t"), side=16, G=3)
+    case, mesh, xs, bct, bcv, tr = _transient_setup(oracle_mod, d)
+    assert not np.allclose(tr.chip[:3], tr.chid[:3])
+    R, F = oracle_mod.assemble_diffusion(mesh, xs, bct, bcv)
+    k, x = oracle_mod.eigen_sinvert(R, F)
+    phi0 = x / np.linalg.norm(x)
+    S = tr.beta[mesh.cmat] * _production(mesh, xs, phi0, k)
+    for dt in (1e-3, 0.01, 1.0):
+        Rt = oracle_mod.assemble_diffusion_transient(mesh, xs, bct, bcv, tr, dt, k)
+        b = oracle_mod.diffusion_rhs(mesh, tr, dt, S, phi0)
+        phi1 = oracle_mod.linear_solve(Rt, b)
+        assert np.linalg.norm(phi1 - phi0) <= 1e-10 * np.linalg.norm(phi0), dt
+        # the time-step operator has the steady pattern
+        assert np.array_equal(Rt.ptr, R.ptr) and np.array_equal(Rt.col, R.col)
+    # without the delayed source the step is not stationary
+    b = oracle_mod.diffusion_rhs(mesh, tr, 0.01, 0.0 * S, phi0)
+    phi1 = oracle_mod.linear_solve(Rt, b)
+    assert np.linalg.norm(phi1 - phi0) > 1e-6 * np.linalg.norm(phi0)
+
+
+def test_transient_rows_dense_decomposition(oracle_mod, tmp_path):
+    """On an 8-cell slab the time-step operator equals the steady R (pinned by
+    the golden keff values) plus diag(V / (v dt)) minus, per cell, the prompt
+    fission block (1 - beta) chi_p nuSf^T V / keff; the step's flux equals a
+    dense solve with that operator."""
+    from pampa_amd import workloads as W
+    d = W.transient_core_2d(str(tmp_path / "t"), side=3, G=2)
+    case, mesh, xs, bct, bcv, tr = _transient_setup(oracle_mod, d)
+    C, G = mesh.C, xs.G
+    R, F = oracle_mod.assemble_diffusion(mesh, xs, bct, bcv)
+    k, dt = 1.03, 0.02
+    Rt = oracle_mod.assemble_diffusion_transient(mesh, xs, bct, bcv, tr, dt, k)
+    Ad = R.scipy().toarray()
+    for i in range(C):
+        m = mesh.cmat[i]
+        V = mesh.vol[i]
+        blk = slice(i * G, (i + 1) * G)
+        Ad[blk, blk] += np.diag(V / (tr.vel[m] * dt))
+        Ad[blk, blk] -= (1.0 - tr.beta[m]) * np.outer(tr.chip[m], xs.nusf[m]) * V / k
+    assert np.allclose(Rt.scipy().toarray(), Ad, rtol=1e-14, atol=1e-14 * np.abs(Ad).max())
+    rng = np.random.default_rng(2)
+    phi0 = rng.uniform(0.5, 2.0, C * G)
+    S = rng.uniform(0.0, 1e-3, C)
+    b = oracle_mod.diffusion_rhs(mesh, tr, dt, S, phi0)
+    bd = np.array([tr.chid[mesh.cmat[l // G], l % G] * S[l // G]
+                   + mesh.vol[l // G] / (tr.vel[mesh.cmat[l // G], l % G] * dt) * phi0[l]
+                   for l in range(C * G)])
+    assert np.allclose(b, bd, rtol=1e-15, atol=0)
+    phi1 = oracle_mod.linear_solve(Rt, b)
+    assert np.allclose(phi1, np.linalg.solve(Ad, bd), rtol=1e-11, atol=0)
+    # vanishing steps leave the flux where it was
+    # (relative change of order v dt max|R|: 2e-8 at dt = 1e-15 s)
+    Rt0 = oracle_mod.assemble_diffusion_transient(mesh, xs, bct, bcv, tr, 1e-15, k)
+    phi_s = oracle_mod.linear_solve(Rt0, oracle_mod.diffusion_rhs(mesh, tr, 1e-15, S, phi0))
+    assert np.linalg.norm(phi_s - phi0) <= 1e-6 * np.linalg.norm(phi0)
+
+
+# ---- reflective boundaries: half domain against the mirrored full domain -------------
+
+def _mirror_case(root, name, solver, half, order=4):
+    """2-D Cartesian case, 1 cm cells, ny = 6; the full domain has 2 * nx_half
+    columns whose materials mirror about x = nx_half, all boundaries vacuum; the
+    half domain keeps the right half with a reflective -x boundary.  Material
+    columns and rows are asymmetric otherwise (so the mirror plane matters)."""
+    from pampa_amd import workloads as W
+    rng = W.SplitMix64(41)
+    d = root / name
+    d.mkdir()
+    mats = [W.two_group_material(rng, True), W.two_group_material(rng, True), W.two_group_material(rng, False)]
+    for k, m in enumerate(mats):
+        if solver == "sn":
+            m = dict(m)
+            m.pop("D")
+        W.write_material(str(d / f"m{k}.pmp"), **m)
+    nxh, ny = 5, 6
+    cols = [[0, 1, 1, 2, 0, 2], [1, 0, 2, 2, 1, 0], [0, 0, 1, 2, 2, 1], [2, 1, 0, 0, 1, 2],
+            [0, 2, 1, 0, 2, 1]]  # [x column][y row] material of the right half
+    if half:
+        nx = nxh
+        layout = [cols[i][j] + 1 for j in range(ny) for i in range(nxh)]
+        bcx = "bc -x reflective\nbc +x vacuum\n"
+    else:
+        nx = 2 * nxh
+        layout = [cols[abs(i - nxh) if i >= nxh else nxh - 1 - i][j] + 1 for j in range(ny) for i in range(nx)]
+        bcx = "bc -x vacuum\nbc +x vacuum\n"
+    (d / "mesh.pmp").write_text(f"dx -{nx}\n1.0\ndy -{ny}\n1.0\n{bcx}bc -y vacuum\nbc +y vacuum\n"
+                                f"materials {nx * ny}\n" + " ".join(map(str, layout)) + "\n")
+    extra = (f"   order {order}\n   mixed-face-interpolation 1.0\n" if solver == "sn" else "")
+    (d / "input.pmp").write_text("mesh cartesian mesh.pmp\n" + "".join(f"material m{k} m{k}.pmp\n" for k in range(3))
+                                 + f"solver {solver} {{\n   energy-groups 2\n{extra}}}\n")
+    return d
+
+
+@pytest.mark.parametrize("solver", ["diffusion", "sn"])
+def test_reflective_half_equals_mirrored_full(oracle_mod, tmp_path, solver):
+    """A reflective -x boundary on the half domain gives the eigenpair of the
+    mirrored full domain with vacuum everywhere: keff equal and the half's flux
+    equal to the full domain's right half.  For S_N this pins the reflected
+    direction table and the reflective branch (the incoming direction m takes
+    the outgoing flux of its reflection about x), with delta = 1 where the
+    interior face weight of uniform cells is exactly 1."""
+    full = _keff(oracle_mod, _mirror_case(tmp_path, "full", solver, False))
+    half = _keff(oracle_mod, _mirror_case(tmp_path, "half", solver, True))
+    assert abs(full.keff - half.keff) <= 1e-10 * full.keff
+    G = 2
+    pf = full.phi.reshape(6, 10, G)[:, 5:, :]
+    ph = half.phi.reshape(6, 5, G)
+    pf = pf / pf.sum()
+    ph = ph / ph.sum()
+    assert np.linalg.norm(pf - ph) <= 1e-8 * np.linalg.norm(ph)
+    # and the mirror plane matters: a vacuum -x on the half gives another keff
+    case = oracle_mod.pmp.read_input(str(tmp_path / "half" / "input.pmp"))
+    assert case is not None
+    txt = (tmp_path / "half" / "mesh.pmp").read_text().replace("bc -x reflective", "bc -x vacuum")
+    (tmp_path / "half" / "mesh.pmp").write_text(txt)
+    vac = _keff(oracle_mod, tmp_path / "half")
+    assert abs(vac.keff - half.keff) > 1e-3 * half.keff
