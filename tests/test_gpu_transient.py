@@ -39,7 +39,7 @@ def test_transient_steps_match_oracle(pk, oracle_mod, tmp_path, precision):
             dts = run.dt()
             assert len(dts) == 2
             # step 1 from a perturbed flux and a random delayed source; step 2
-            # from the flux step 1 produced (the drop-in copies it)
+            # from the flux and delayed source step 1 produced
             phi0 = phi * rng.uniform(0.8, 1.2, C * G)
             # delayed source around its equilibrium value beta P_i
             S = tr.beta[mesh.cmat] * np.sum(phi.reshape(C, G) * xs.nusf[mesh.cmat], axis=1) * mesh.vol / k
@@ -68,5 +68,10 @@ def test_transient_steps_match_oracle(pk, oracle_mod, tmp_path, precision):
                 # the second starts from the first's result
                 assert moved > (1e-4 if n == 1 else 0.0)
                 phi0 = phi1
+                # each solve leaves the equilibrium delayed source S = beta P of its
+                # flux for the next step (NeutronicSolver.cxx:95-102)
+                P = run.field("production-rate")
+                S = tr.beta[mesh.cmat] * P
+                assert np.array_equal(run.field("delayed-source"), S)
     finally:
         os.chdir(cwd)
