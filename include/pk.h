@@ -145,6 +145,18 @@ int pk_sn_assemble(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const doub
 int pk_normalize_flux(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double keff, pk_vec q,
                       pk_vec P, double* total_power);
 
+/* pk_normalize_flux without allocation or library scratch: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and results as pk_normalize_flux; the reductions' partials,
+ * scalars and counters live in a fixed 16.9 KB device array per GPU (resolved
+ * by pk_initialize, which must have run), and the call allocates nothing.
+ * Cells x groups must stay below 2^31.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (from several host threads)
+ * race; the counters are cleared at the start of each call. */
+int pk_normalize_flux_static(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double keff, pk_vec q,
+                             pk_vec P, double* total_power);
+
 /* ---- eigen solver (petsc::create(EPS&) petsc.cxx:175-204, EPSSetInitialSpace,
  *      petsc::solve(EPS) petsc.cxx:428-488, EPSGetEigenpair) -------------
  * Solves R x = lambda F x for the eigenvalue of largest 1/lambda (keff) by

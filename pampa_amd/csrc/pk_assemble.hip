@@ -481,6 +481,29 @@ __global__ void sn_fill_kernel(MeshView M, XSView X, SnRefView Q, const int* rpt
    }
 }
 
+// counters that must start at zero (a fill kernel, not a memset node: the
+// call may sit inside a captured graph)
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = 0u;
+}
+
+// The two kernels of the power normalisation with reduction scratch at `scr`
+// (partials, scalars, tickets at the offsets of the static-scratch layout).
+static int normalize_flux_launch(pk_mesh_s* mesh, pk_xs_s* xs, pk_vec_s* phi, double power, double keff,
+                                 pk_vec_s* q, pk_vec_s* P, double* partials, unsigned* ticket, double* scalars) {
+   Ctx& c = ctx();
+   int g = grid_for((long)mesh->C * xs->G, kBlock, kMaxRedBlocks);
+   power_kernel<<<g, kBlock, 0, c.stream>>>(view(mesh), view(xs), phi->d.p, partials, ticket, scalars);
+   PK_HIP(hipGetLastError());
+   int g2 = grid_for(mesh->C, kBlock, kMaxRedBlocks);
+   scale_power_kernel<<<g2, kBlock, 0, c.stream>>>(view(mesh), view(xs), phi->d.p, scalars, power, 1.0 / keff,
+                                                   q ? q->d.p : nullptr, P ? P->d.p : nullptr, partials, ticket,
+                                                   scalars + 1);
+   PK_HIP(hipGetLastError());
+   return 0;
+}
+
 }  // namespace pk
 
 using namespace pk;
@@ -784,20 +807,36 @@ int pk_normalize_flux(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double k
                       pk_vec P, double* total_power) {
    PK_REQUIRE(mesh && xs && phi && phi->n == (long)mesh->C * xs->G, "wrong arguments");
    Ctx& c = ctx();
-   int g = grid_for((long)mesh->C * xs->G, kBlock, kMaxRedBlocks);
-   power_kernel<<<g, kBlock, 0, c.stream>>>(view(mesh), view(xs), phi->d.p, c.partials, c.ticket,
-                                            c.scalars);
-   PK_HIP(hipGetLastError());
-   int g2 = grid_for(mesh->C, kBlock, kMaxRedBlocks);
-   scale_power_kernel<<<g2, kBlock, 0, c.stream>>>(view(mesh), view(xs), phi->d.p, c.scalars, power,
-                                                   1.0 / keff, q ? q->d.p : nullptr,
-                                                   P ? P->d.p : nullptr, c.partials, c.ticket,
-                                                   c.scalars + 1);
-   PK_HIP(hipGetLastError());
+   PK_CALL(normalize_flux_launch(mesh, xs, phi, power, keff, q, P, c.partials, c.ticket, c.scalars));
    PK_HIP(hipMemcpyAsync(c.host_scalars, c.scalars, 3 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));
    if (total_power) *total_power = c.host_scalars[2];
    PK_REQUIRE(c.host_scalars[1] == 0.0, "negative values in the scalar-flux solution");
+   return 0;
+}
+
+int pk_normalize_flux_static(pk_mesh mesh, pk_xs xs, pk_vec phi, double power, double keff, pk_vec q,
+                             pk_vec P, double* total_power) {
+   Ctx& c = ctx();
+   PK_REQUIRE(c.stream, "pk_initialize must run before the allocation-free entry points");
+   double* scr = static_scratch();
+   PK_REQUIRE(scr, "no static scratch on this device");
+   PK_REQUIRE(mesh && xs && phi && phi->n == (long)mesh->C * xs->G, "wrong arguments");
+   PK_REQUIRE(q == nullptr || q->n == mesh->C, "wrong size of the power vector");
+   PK_REQUIRE(P == nullptr || P->n == mesh->C, "wrong size of the production vector");
+   // the reductions use at most kMaxRedBlocks workgroups x 2 values of partials;
+   // the kernels index cells and rows with 32-bit integers
+   PK_REQUIRE((long)mesh->C * xs->G <= 0x7fffffffL,
+              "flux larger than the static-scratch path supports (cells x groups must be below 2^31)");
+   unsigned* tickets = (unsigned*)(scr + kScratchTickets);
+   zero_u32_kernel<<<1, 64, 0, c.stream>>>(tickets, 16);
+   PK_HIP(hipGetLastError());
+   PK_CALL(normalize_flux_launch(mesh, xs, phi, power, keff, q, P, scr, tickets, scr + kScratchScalars));
+   double h[3] = {0.0, 0.0, 0.0};
+   PK_HIP(hipMemcpyAsync(h, scr + kScratchScalars, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   if (total_power) *total_power = h[2];
+   PK_REQUIRE(h[1] == 0.0, "negative values in the scalar-flux solution");
    return 0;
 }
 

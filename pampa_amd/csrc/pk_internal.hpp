@@ -93,6 +93,18 @@ void prof_add(const char* name, long launches, double seconds, double bytes);
 // Grow the reduction scratch to at least `count` doubles (setup time only).
 int reserve_partials(size_t count);
 
+// Static scratch of the allocation-free entry points: kScratchDoubles doubles
+// (16.9 KB) per device, a file-scope __device__ array whose device address
+// pk_initialize resolves per device.  Layout: reduction partials of up to
+// kMaxRedBlocks workgroups x 2 values, then 8 scalars, then 16 ticket
+// counters (unsigned, in the last doubles).  static_scratch() is the current
+// device's copy (nullptr before pk_initialize).
+constexpr int kScratchPartials = kMaxRedBlocks * 2;
+constexpr int kScratchScalars = kScratchPartials;       // 8 doubles
+constexpr int kScratchTickets = kScratchPartials + 8;   // 16 unsigned (8 doubles)
+constexpr int kScratchDoubles = kScratchPartials + 64;
+double* static_scratch();
+
 inline int grid_for(long n, int block = kBlock, int cap = 1 << 30) {
    long g = (n + block - 1) / block;
    if (g < 1) g = 1;

@@ -91,6 +91,20 @@ void set_error(const char* file, int line, const char* func, const std::string& 
 
 Ctx& ctx() { return g_ctx; }
 
+// Fixed device scratch of the allocation-free entry points (each device that
+// loads the library holds one copy until the process ends).  Host code never
+// uses this name as an address: it is a host-side stand-in.  pk_initialize
+// resolves the copy of the current device with hipGetSymbolAddress and keeps
+// that address per device; kernels receive it as an argument.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+double* static_scratch() {
+   const int d = g_ctx.device;
+   return d >= 0 && d < kMaxDevices ? g_scratch_dev[d] : nullptr;
+}
+
 int ensure_init() {
    if (g_ctx.stream) return 0;
    return pk_initialize(-1);
@@ -216,6 +230,12 @@ int pk_initialize(int device) {
    PK_HIP(hipMalloc(&c.scalars, sizeof(double) * 256));
    PK_HIP(hipMemsetAsync(c.scalars, 0, sizeof(double) * 256, c.stream));
    PK_HIP(hipHostMalloc(&c.host_scalars, sizeof(double) * 256, hipHostMallocDefault));
+   if (device < kMaxDevices && !g_scratch_dev[device]) {
+      void* sp = nullptr;
+      PK_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+      PK_REQUIRE(sp != nullptr, "no device address for the static scratch");
+      g_scratch_dev[device] = (double*)sp;
+   }
    PK_HIP(hipStreamSynchronize(c.stream));
    return 0;
 }
