@@ -26,11 +26,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per gmres_spmv_dot launch on the quarter-core workload: two rocprofv3 --pmc
-# passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --roofline-pass`, each scaled by
-# what the same counter reads for stream_copy_kernel moving 2^28 bytes each way
-# (fetch x2.000, write x1.000): 3095.6 KiB x 2.000 + 274.7 KiB = 6.621 MB.
-SPMV_DOT_TRAFFIC_BYTES = 6.621e6
+# HBM bytes per gmres_spmv_dot launch (all launches of the captured cycles, idle ones
+# included) on the quarter-core workload: two rocprofv3 --pmc passes (FETCH_SIZE, then
+# WRITE_SIZE) over `bench.py --roofline-pass`, each scaled by what the same counter reads for
+# stream_copy_kernel moving 2^28 bytes each way (fetch x2.000, write x1.000):
+# 1911.2 KiB x 2.000 + 160.5 KiB = 4.079 MB.
+SPMV_DOT_TRAFFIC_BYTES = 4.079e6
+# the same counter passes over `bench.py --batched-pass` (256 instances in one mixed pool),
+# HBM bytes per launch of each GMRES step kernel
+BATCHED_TRAFFIC_BYTES = {"gmres_spmv_dot": 508.7e6, "gmres_orth": 729.5e6, "gmres_poly": 952.2e6}
 METRIC = ("eigen solves/s + p50/p99 latency, mixed vs FP64 at |dk|<=1pcm; sweep/SpMV HBM GB/s")
 
 
@@ -126,6 +130,33 @@ def cpu_baseline(d, target_seconds=15.0):
         os.chdir(cwd)
 
 
+def _cpu_worker(args):
+    d, seconds = args
+    import oracle
+    from threadpoolctl import threadpool_limits
+    os.chdir(d)
+    with threadpool_limits(1):
+        case = oracle.pmp.read_input("input.pmp")
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds or n == 0:
+            oracle.solve_diffusion_case(case)
+            n += 1
+        return n, time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(d, seconds=10.0, procs=None):
+    """The same oracle solves on every host core at once: `procs` processes
+    (the host's CPU share, at most 16), each running cold solves of the same
+    workload for `seconds`, one thread each.  Aggregate solves/s of the host."""
+    import multiprocessing as mp
+    procs = procs or min(16, len(os.sched_getaffinity(0)))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, [(d, seconds)] * procs)
+    total = sum(n / t for n, t in res)
+    return total, procs, sum(n for n, _ in res)
+
+
 def sn_solve_numbers(pa, d, reps=3):
     """Cold k-eigenvalue solves of the S6 hexagonal lattice through the drop-in
     C-ABI (sweeps inside Anderson-accelerated source iteration), FP64 and mixed
@@ -209,6 +240,53 @@ def _oracle_request_sn(oracle, W, path, seed, rel):
     return k, phiw
 
 
+def _oracle_request_job(args):
+    path, seed, rel = args
+    import oracle
+    from pampa_amd import workloads as W
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):
+        k, x = _oracle_request(oracle, W, path, seed, rel)
+    return k, x
+
+
+FIXTURE = os.path.join(HERE, "tests", "golden", "batched_classes.npz")
+
+
+def _class_accuracy(W, res, paths, sample, rows):
+    """Per size class of the batched family: the served keff and flux against
+    the FP64 oracle -- solved here in parallel processes (1 thread each) for the
+    classes it finishes in about a minute, from the committed oracle fixture
+    (keff, every 64th flux entry) for the four largest."""
+    import multiprocessing as mp
+    fx = np.load(FIXTURE) if os.path.exists(FIXTURE) else None
+    direct, out, skipped = [], [], []
+    for r in sample:
+        key = f"inst{r}_keff"
+        if fx is not None and key in fx.files:
+            st = int(fx["stride"])
+            k_o, x_o = float(fx[key]), fx[f"inst{r}_phi"]
+            xs = np.asarray(res["flux"][r])[::st]
+            out.append({"instance": int(r), "rows": int(rows[r]), "source": "fixture (every 64th entry)",
+                        "dk_pcm": abs(res["keff"][r] - k_o) * 1e5,
+                        "flux_l2": float(np.linalg.norm(_shape(xs) - _shape(x_o)) / np.linalg.norm(_shape(x_o)))})
+        elif _class_of(W, r) in BIG_CLASSES:
+            skipped.append(int(r))
+        else:
+            direct.append(r)
+    procs = min(16, len(os.sched_getaffinity(0)), max(1, len(direct)))
+    with mp.get_context("spawn").Pool(procs) as pool:
+        sols = pool.map(_oracle_request_job, [(paths[r], 0, 0.0) for r in direct])
+    for r, (k_o, x_o) in zip(direct, sols):
+        out.append({"instance": int(r), "rows": int(rows[r]), "source": "oracle",
+                    "dk_pcm": abs(res["keff"][r] - k_o) * 1e5,
+                    "flux_l2": float(np.linalg.norm(_shape(res["flux"][r]) - _shape(x_o)) /
+                                     np.linalg.norm(_shape(x_o)))})
+    out.sort(key=lambda a: a["rows"])
+    return {"max_dk_pcm": max(a["dk_pcm"] for a in out), "max_flux_l2": max(a["flux_l2"] for a in out),
+            "classes": len(out), "per_class": out, "skipped_no_fixture": skipped}
+
+
 def _accuracy(oracle, W, res, paths, cid, seeds, rel, sample, sn_cases=()):
     out = []
     for r in sample:
@@ -269,6 +347,70 @@ def batched_spmv(pa, paths, reps=20):
     return spmv
 
 
+def step_kernel_profile(pa):
+    """Per-kernel launches, device seconds (HIP events recorded by each launch)
+    and algorithmic bytes of the GMRES step kernels from the library profiler;
+    the dominant kernel's roofline."""
+    prof = {}
+    for name in ("gmres_spmv_dot", "gmres_orth", "gmres_poly"):
+        cnt, sec, byt = pa.profile_get(name)
+        if cnt:
+            prof[name] = dict(launches=cnt, seconds=sec, bytes=byt, avg_us=sec / cnt * 1e6,
+                              bytes_per_launch=byt / cnt, GBs=byt / sec / 1e9)
+    roof = None
+    if prof:
+        name, dv = max(prof.items(), key=lambda kv: kv[1]["seconds"])
+        ach = dv["bytes"] / dv["seconds"] / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "avg_launch_us": dv["avg_us"],
+                "bytes_per_launch": dv["bytes_per_launch"], "launches": dv["launches"]}
+    return prof, roof
+
+
+def batched_profile_traffic(prof, roof):
+    """Counter-pass HBM bytes per launch of the batched pool's step kernels."""
+    if roof:
+        roof["traffic"] = BATCHED_TRAFFIC_BYTES.get(roof["kernel"])
+    for name, dv in prof.items():
+        dv["traffic_per_launch"] = BATCHED_TRAFFIC_BYTES.get(name)
+    return prof, roof
+
+
+def batched_profile(pa, paths):
+    """Profiled pass of the batched family in one mixed pool: the same ticks as
+    the timed pass, launched one by one with events (every step launch of the
+    captured cycle counted, idle ones included)."""
+    n = len(paths)
+    log(f"[bench] batched: {n} instances in one mixed pool, profiled pass")
+    pa.profile(True)
+    _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), np.arange(n), np.zeros(n, np.uint64), 0.0, False)
+    prof, roof = step_kernel_profile(pa)
+    pa.profile(False)
+    return batched_profile_traffic(prof, roof)
+
+
+# size classes whose oracle solve takes minutes here: checked against the committed fixture
+BIG_CLASSES = {(128, 16), (256, 8), (192, 16), (256, 16)}
+
+
+def _class_of(W, i):
+    rng = W.SplitMix64(3_000_000 + i)
+    side = W.BATCH_SIDES[rng.choice(len(W.BATCH_SIDES))]
+    return side, W.BATCH_GROUPS[rng.choice(len(W.BATCH_GROUPS))]
+
+
+def _class_sample(W, tmp, n, rows):
+    """One instance of every (side, groups) class present among the first n
+    instances of the batched family (the first of each class)."""
+    seen, out = set(), []
+    for i in range(n):
+        c = _class_of(W, i)
+        if c not in seen:
+            seen.add(c)
+            out.append(i)
+    return out
+
+
 def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
     """Config-3 family: n_inst independent instances (sides and group counts
     drawn by seed, cross sections perturbed +-5% per instance), all admitted at
@@ -278,21 +420,14 @@ def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
     n = n_inst
     cid = np.arange(n)
     seeds = np.zeros(n, np.uint64)
-    small = [int(i) for i in np.argsort(rows, kind="stable")[:4]]
-    log(f"[bench] batched: {n} instances ({rows.sum()} rows) in one mixed pool, profiled pass")
-    pa.profile(True)
-    _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), cid, seeds, 0.0, False)
-    prof = {}
-    for name in ("gmres_spmv_dot", "gmres_orth"):
-        cnt, sec, byt = pa.profile_get(name)
-        if cnt:
-            prof[name] = dict(launches=cnt, seconds=sec, bytes=byt)
-    pa.profile(False)
+    small = _class_sample(W, tmp, n, rows)
+    prof, roof = batched_profile(pa, paths)
     log("[bench] batched: timed pass")
     res, st = _serve(pa, paths, pa.PK_PREC_MIXED, n, np.zeros(n), cid, seeds, 0.0, False, keep=small)
     spmv = batched_spmv(pa, paths)
     mixed_all = n / res["latency"].max()
-    acc = _accuracy(oracle, W, res, paths, cid, seeds, 0.0, small)
+    log(f"[bench] batched: accuracy of {len(small)} size classes against the oracle")
+    acc = _class_accuracy(W, res, paths, small, rows)
     k = fp64_sample
     log(f"[bench] batched: FP64 drop-in, one at a time, on {k} of them")
     r64, _ = _serve(pa, paths[:k], pa.PK_PREC_FP64, 1, np.zeros(k), np.arange(k), np.zeros(k, np.uint64), 0.0,
@@ -301,13 +436,6 @@ def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
                     0.0, False)
     fp64_rate = k / r64["latency"].max()
     mixed_rate = k / rmx["latency"].max()
-    roof = None
-    if prof:
-        name, dv = max(prof.items(), key=lambda kv: kv[1]["seconds"])
-        ach = dv["bytes"] / dv["seconds"] / 1e9
-        roof = {"bound": "hbm", "kernel": name, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "avg_launch_us": dv["seconds"] / dv["launches"] * 1e6,
-                "bytes_per_launch": dv["bytes"] / dv["launches"], "launches": dv["launches"]}
     return {"workload": f"{n} instances of the batched family (sides {list(W.BATCH_SIDES)}, groups "
                         f"{list(W.BATCH_GROUPS)}, +-5% cross sections), {int(rows.sum())} rows, all "
                         f"admitted at once",
@@ -337,7 +465,7 @@ def _sn_bases(W, tmp, n, seed=5):
 
 
 def serving_numbers(pa, W, oracle, tmp, n_base=16, n_sn=4, n_req=300, rate=200.0, rel=0.02, seed=11,
-                    sn_share=0.2):
+                    sn_share=0.2, accuracy=True):
     """Open-loop Poisson stream of seeded +-2% perturbations of previously seen
     base cases, warm-started: 80 % diffusion requests on n_base cases of the
     batched family, 20 % S_N requests on n_sn hexagonal-lattice cases.  The
@@ -375,8 +503,8 @@ def serving_numbers(pa, W, oracle, tmp, n_base=16, n_sn=4, n_req=300, rate=200.0
                      "sn_p50_ms": float(np.percentile(lat[is_sn], 50) * 1e3) if is_sn.any() else None,
                      "sn_p99_ms": float(np.percentile(lat[is_sn], 99) * 1e3) if is_sn.any() else None,
                      "max_in_flight": st["max_live"], "converged": int(res["converged"].sum()),
-                     "accuracy_vs_fp64_oracle": _accuracy(oracle, W, res, all_paths, cid, seeds, rel,
-                                                          small + small_sn, sn_cases)}
+                     "accuracy_vs_fp64_oracle": (_accuracy(oracle, W, res, all_paths, cid, seeds, rel,
+                                                           small + small_sn, sn_cases) if accuracy else None)}
     out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64_dropin"]["solves_per_s"]
     return out
 
@@ -400,6 +528,15 @@ def main():
     ap.add_argument("--roofline-pass", action="store_true",
                     help="only the profiled solves of the headline workload (for a kernel trace whose "
                          "launches are exactly the ones the roofline figure averages)")
+    ap.add_argument("--batched-pass", action="store_true",
+                    help="only the profiled pass of the batched family in one mixed pool (per-kernel "
+                         "rates), then the copy kernel that calibrates the traffic counters")
+    ap.add_argument("--sweep-pass", action="store_true",
+                    help="only the S_N sweep timing (FP32 and FP64) and the copy kernel")
+    ap.add_argument("--serving-rates", type=str, default=None,
+                    help="comma-separated Poisson rates (req/s): the serving stream at each rate for "
+                         "--rate-window seconds of arrivals, mixed vs FP64")
+    ap.add_argument("--rate-window", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -414,6 +551,37 @@ def main():
     log(f"[bench] workload quarter core 170x170 G=2 at {d1}")
     if args.roofline_pass:
         print(json.dumps(roofline_pass(pa, d1)))
+        return
+    if args.batched_pass:
+        paths, rows = _family(W, os.path.join(tmp, "family"), args.batch)
+        prof, roof = batched_profile(pa, paths)
+        n = 1 << 26
+        sec = pa.stream_copy(n, 20)
+        print(json.dumps({"profiled_kernels": prof, "roofline": roof,
+                          "stream_copy": {"bytes": 8.0 * n, "us": sec * 1e6, "GBs": 8.0 * n / sec / 1e9}}))
+        return
+    if args.sweep_pass:
+        d2 = W.hex_sn_3d(os.path.join(tmp, "hex-s6"))
+        out = sweep_numbers(pa, d2)
+        n = 1 << 26
+        sec = pa.stream_copy(n, 20)
+        out["stream_copy"] = {"bytes": 8.0 * n, "us": sec * 1e6, "GBs": 8.0 * n / sec / 1e9}
+        print(json.dumps(out))
+        return
+    if args.serving_rates:
+        import oracle
+        oracle.lib()
+        rates = [float(r) for r in args.serving_rates.split(",")]
+        out = {"window_s": args.rate_window, "rates": {}}
+        for rate in rates:
+            n_req = max(20, int(round(rate * args.rate_window)))
+            log(f"[bench] serving at {rate:g} req/s: {n_req} requests")
+            out["rates"][f"{rate:g}"] = serving_numbers(pa, W, oracle, os.path.join(tmp, f"family-{rate:g}"),
+                                                        n_req=n_req, rate=rate, accuracy=False)
+            r = out["rates"][f"{rate:g}"]
+            log(f"[bench]   mixed {r['mixed']['solves_per_s']:.1f} solves/s p50 {r['mixed']['p50_ms']:.0f} "
+                f"p99 {r['mixed']['p99_ms']:.0f} ms; fp64 {r['fp64_dropin']['solves_per_s']:.1f} solves/s")
+        print(json.dumps(out))
         return
     if args.sweep_only:
         d2 = W.hex_sn_3d(os.path.join(tmp, "hex-s6"))
@@ -448,10 +616,15 @@ def main():
         cpu = None
     else:
         times, ref = cpu_baseline(d1)
+        log("[bench] CPU baseline on all host cores")
+        allc, procs, nsol = cpu_baseline_all_cores(d1)
         cpu = {"value": len(times) / sum(times), "unit": "solves/s", "cores": 1, "kind": "port",
                "sample": f"{len(times)} cold solves of the same 170x170 G=2 quarter core "
                          f"(assembly + sparse LU + shift-invert Arnoldi + normalisation), "
-                         f"1 thread, host has {os.cpu_count()} logical CPUs"}
+                         f"1 thread, host has {os.cpu_count()} logical CPUs",
+               "all_cores": {"value": allc, "unit": "solves/s", "cores": procs, "kind": "port",
+                             "sample": f"{nsol} cold solves of the same workload in {procs} processes at "
+                                       f"once (1 thread each, ~10 s), aggregate rate"}}
 
     acc, lat = {}, {}
     for prec, (ls, k, phi, st, prof, spmv) in results.items():
@@ -477,7 +650,9 @@ def main():
                 "traffic": SPMV_DOT_TRAFFIC_BYTES if dname == "gmres_spmv_dot" else None,
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": avg_b,
                 "launches_per_solve": dv["launches"],
-                "note": "latency-bound: the 170x170 operator (4 MB) is cache resident"}
+                "note": "latency-bound: the 170x170 operator (4 MB) is cache resident; every launch of "
+                        "the captured cycles counted, idle ones included (HIP events recorded by each "
+                        "launch, same kernels as the graph)"}
     spmv_s, spmv_b = results["mixed"][5]
     spmv = {"kernel": "spmv_fp32", "us": spmv_s * 1e6, "GBs": spmv_b / spmv_s / 1e9,
             "frac_hbm": spmv_b / spmv_s / 1e9 / HBM_PEAK_GBS, "bytes": spmv_b}

@@ -446,10 +446,65 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, BSell A, const
 // the epilogue reads is loaded by every workgroup into local memory before the
 // reduction (one value per thread; the last workgroup then has it at hand),
 // the reduced values stay in local memory, and the stores run in parallel.
+// Profiling of the GMRES step kernels: every workgroup stores its first and
+// last device wall-clock reading at its own place (stamps[2 wg], no atomics);
+// a one-workgroup kernel then takes the launch's span (earliest start, latest
+// end) into a record.  stamps == null outside profiling.
+__device__ __forceinline__ unsigned long long stamp_begin(const unsigned long long* stamps) {
+   return stamps ? (unsigned long long)wall_clock64() : 0ull;
+}
+__device__ __forceinline__ void stamp_end(unsigned long long* stamps, unsigned long long t0) {
+   if (!stamps) return;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      stamps[2 * (long)blockIdx.x] = t0;
+      stamps[2 * (long)blockIdx.x + 1] = (unsigned long long)wall_clock64();
+   }
+}
+
+__global__ void __launch_bounds__(256) stamp_span_kernel(const unsigned long long* stamps, int n,
+                                                         unsigned long long* rec) {
+   unsigned long long lo = ~0ull, hi = 0ull;
+   for (int i = threadIdx.x; i < n; i += 256) {
+      lo = min(lo, stamps[2 * (long)i]);
+      hi = max(hi, stamps[2 * (long)i + 1]);
+   }
+   __shared__ unsigned long long slo[256], shi[256];
+   slo[threadIdx.x] = lo;
+   shi[threadIdx.x] = hi;
+   __syncthreads();
+   for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+         slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + w]);
+         shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + w]);
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      rec[0] = slo[0];
+      rec[1] = shi[0];
+   }
+}
+
+// Activity of a slot in the GMRES step kernels, counted by the slot's first
+// workgroup when profiling (act != null): [0] full SpMV/dot launches,
+// [1] sum of j + 1 over them (basis vectors read), [2] norm-only launches,
+// [3] orthogonalisation launches, [4] sum of j + 4 over them, [5] second
+// preconditioner terms.  The host turns them into algorithmic bytes.
+constexpr int kAct = 6;
+constexpr int kRecs = 1 << 14;  // launch spans held on the device before a read-back
+__device__ __forceinline__ void count_act(const BLayout& L, int b, long long* act, int k0, long long v0, int k1,
+                                          long long v1) {
+   if (act && blockIdx.x == (unsigned)L.wg_first[b] && threadIdx.x == 0) {
+      act[(long)b * kAct + k0] += v0;
+      if (k1 >= 0) act[(long)b * kAct + k1] += v1;
+   }
+}
+
 template <class T>
 __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
                                               T* w, const T* V, int j, double* partials, unsigned* tickets, unsigned* gtickets,
-                                              const BState& S) {
+                                              const BState& S, long long* act) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
    const int m = S.m;
@@ -458,6 +513,10 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const int mlen = L.mlen[b];
    const bool edge = j == mlen;
+   if (edge)
+      count_act(L, b, act, 2, 1, -1, 0);
+   else
+      count_act(L, b, act, 0, 1, 1, j + 1);
    const double vj = (double)V[(long)j * N + row];
    __shared__ double hc[kBMaxM + 2], cc[kBMaxM], ss[kBMaxM], sc[4];
    __shared__ int stop_sh;
@@ -552,8 +611,11 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
 template <class T>
 __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSell A, const T* aval,
                                                           const T* z, T* w, const T* V, int j,
-                                                          double* partials, unsigned* tickets, unsigned* gtickets, BState S) {
-   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, gtickets, S);
+                                                          double* partials, unsigned* tickets, unsigned* gtickets, BState S,
+                                                          long long* act, unsigned long long* stamps) {
+   const unsigned long long t0 = stamp_begin(stamps);
+   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, gtickets, S, act);
+   stamp_end(stamps, t0);
 }
 
 
@@ -561,9 +623,10 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
 // its norm is reduced by the next step (spmv_dot_step)
 template <class T>
 __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* w, T* V,
-                                          T* z, const T* binv, int j, const BState& S) {
+                                          T* z, const T* binv, int j, const BState& S, long long* act) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
+   count_act(L, b, act, 3, 1, 4, j + 4);
    __shared__ T tl[kRows], xl[kRows];
    const int m = S.m;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
@@ -589,61 +652,11 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell&
 
 template <class T>
 __global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, BSell A, const T* aval, const T* w, T* V,
-                                                      T* z, const T* binv, int j, BState S) {
-   orth_step<T>(N, L, A, aval, w, V, z, binv, j, S);
-}
-
-// The GMRES steps with their execution span recorded for timing: every
-// workgroup folds its first and last device wall-clock reading into clk[0]
-// (earliest start) and clk[1] (latest end).  Same arithmetic as the untimed
-// kernels.
-__device__ __forceinline__ void record_span(unsigned long long t0, unsigned long long* clk) {
-   __syncthreads();
-   if (threadIdx.x == 0) {
-      atomicMin(&clk[0], t0);
-      atomicMax(&clk[1], (unsigned long long)wall_clock64());
-   }
-}
-
-template <class T>
-__global__ void __launch_bounds__(kRows) bspmv_dot_timed_kernel(long N, BLayout L, BSell A, const T* aval,
-                                                                const T* z, T* w, const T* V, int j,
-                                                                double* partials, unsigned* tickets, unsigned* gtickets, BState S,
-                                                                unsigned long long* clk) {
-   const unsigned long long t0 = wall_clock64();
-   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, gtickets, S);
-   record_span(t0, clk);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kRows) bpoly_timed_kernel(BLayout L, BSell A, const T* aval, const T* v,
-                                                            const T* z0, const T* binv, T* z, BState S,
-                                                            unsigned long long* clk) {
-   const unsigned long long t0 = wall_clock64();
-   const int b = slot_of_wg(L);
-   if (gmres_on(S, b)) {
-      __shared__ T tl[kRows], xl[kRows];
-      const long row = blockIdx.x * (long)kRows + threadIdx.x;
-      const long s = row >> 6;
-      const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
-      const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
-      z[row] = z0[row] + c;
-   }
-   record_span(t0, clk);
-}
-
-template <class T>
-__global__ void __launch_bounds__(kRows) borth_timed_kernel(long N, BLayout L, BSell A, const T* aval, const T* w,
-                                                            T* V, T* z, const T* binv, int j, BState S,
-                                                            unsigned long long* clk) {
-   const unsigned long long t0 = wall_clock64();
-   orth_step<T>(N, L, A, aval, w, V, z, binv, j, S);
-   record_span(t0, clk);
-}
-
-__global__ void clk_reset_kernel(unsigned long long* clk, int n) {
-   int i = blockIdx.x * blockDim.x + threadIdx.x;
-   if (i < n) clk[i] = (i & 1) ? 0ull : ~0ull;
+                                                      T* z, const T* binv, int j, BState S, long long* act,
+                                                      unsigned long long* stamps) {
+   const unsigned long long t0 = stamp_begin(stamps);
+   orth_step<T>(N, L, A, aval, w, V, z, binv, j, S, act);
+   stamp_end(stamps, t0);
 }
 
 // y = H^-1 g, one workgroup per slot that took steps this cycle.  H is loaded
@@ -693,10 +706,24 @@ __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) 
 // this GMRES step; mode 1: slots whose cycle took steps (the update: also
 // e += z).
 template <class T>
+__device__ __forceinline__ void poly_step(const BLayout& L, const BSell& A, const T* aval, const T* v, const T* z0,
+                                          const T* binv, T* z, const BState& S, int mode, T* e, long long* act);
+
+template <class T>
 __global__ void __launch_bounds__(kRows) bpoly_kernel(BLayout L, BSell A, const T* aval, const T* v, const T* z0,
-                                                      const T* binv, T* z, BState S, int mode, T* e) {
+                                                      const T* binv, T* z, BState S, int mode, T* e, long long* act,
+                                                      unsigned long long* stamps) {
+   const unsigned long long t0 = stamp_begin(stamps);
+   poly_step<T>(L, A, aval, v, z0, binv, z, S, mode, e, act);
+   stamp_end(stamps, t0);
+}
+
+template <class T>
+__device__ __forceinline__ void poly_step(const BLayout& L, const BSell& A, const T* aval, const T* v, const T* z0,
+                                          const T* binv, T* z, const BState& S, int mode, T* e, long long* act) {
    const int b = slot_of_wg(L);
    if (mode == 0 ? !gmres_on(S, b) : (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0)) return;
+   if (mode == 0) count_act(L, b, act, 5, 1, -1, 0);
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long s = row >> 6;
@@ -939,8 +966,13 @@ struct pk_batch_s {
          if (e) (void)hipEventDestroy(e);
       if (h_status) (void)hipHostFree(h_status);
    }
-   DBuf<unsigned long long> clk;  // execution spans of a profiled step's kernels
-   double clock_hz = 0.0;           // device wall clock
+   DBuf<long long> act;  // per-slot activity of the GMRES step kernels (profiling)
+   // profiled launch spans: per-workgroup stamps of the current launch, and
+   // records (start, end) of launches not yet read back, with their kernel
+   DBuf<unsigned long long> stamps, recs;
+   std::vector<int> rec_kind;
+   int nrec = 0;
+   double clock_hz = 0.0;
    double tick_seconds = 0.0;
    long ticks = 0;
    // fixed-batch use (pk_batch_create): members and optional initial vectors
@@ -1221,9 +1253,55 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    return 0;
 }
 
+static const char* const kStepKernel[3] = {"gmres_spmv_dot", "gmres_orth", "gmres_poly"};
+
+// Profiling: the spans recorded so far into the profiler (launches, seconds).
+static int flush_spans(pk_batch_s* P) {
+   if (P->nrec == 0) return 0;
+   Ctx& c = ctx();
+   std::vector<unsigned long long> h(2 * (size_t)P->nrec);
+   PK_HIP(hipMemcpyAsync(h.data(), P->recs.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
+                         c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   for (int r = 0; r < P->nrec; r++)
+      prof_add(kStepKernel[P->rec_kind[r]], 1, (double)(h[2 * r + 1] - h[2 * r]) / P->clock_hz, 0.0);
+   P->nrec = 0;
+   return 0;
+}
+
+// Profiling: the slots' GMRES activity since the last fold turned into the
+// step kernels' algorithmic bytes (value and column of every stored entry of
+// the inner operator, the basis vectors each step reads and writes, the
+// block inverses), added to the profiler's records; counters cleared.
+static int fold_activity(pk_batch_s* P) {
+   if (!profiling() || !P->act.p) return 0;
+   PK_CALL(flush_spans(P));
+   Ctx& c = ctx();
+   std::vector<long long> h((size_t)P->max_slots * kAct);
+   PK_HIP(hipMemcpyAsync(h.data(), P->act.p, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipMemsetAsync(P->act.p, 0, sizeof(long long) * h.size(), c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   const double s = P->precision == PK_PREC_MIXED ? 4.0 : 8.0;
+   double bdot = 0.0, borth = 0.0, bpoly = 0.0;
+   for (int b = 0; b < P->max_slots; b++) {
+      const long long* a = &h[(size_t)b * kAct];
+      if (!P->slots[b].used) continue;
+      const double n = (double)P->slots[b].n, nnz = (double)P->slots[b].nnz, bsn = n * P->slots[b].pbs;
+      const double ent = nnz * (s + 4.0);
+      bdot += a[0] * (ent + 2.0 * n * s) + (a[1] + a[2]) * n * s;
+      borth += a[3] * (bsn * s + (P->opts.sweep ? ent : 0.0)) + a[4] * n * s;
+      bpoly += a[5] * (ent + 3.0 * n * s + bsn * s);
+   }
+   prof_add("gmres_spmv_dot", 0, 0.0, bdot);
+   prof_add("gmres_orth", 0, 0.0, borth);
+   if (P->opts.poly) prof_add("gmres_poly", 0, 0.0, bpoly);
+   return 0;
+}
+
 static int pool_release(pk_batch_s* P, int sidx) {
    Ctx& c = ctx();
    PK_REQUIRE(sidx >= 0 && sidx < P->max_slots && P->slots[sidx].used, "no such slot");
+   PK_CALL(fold_activity(P));
    Slot& S = P->slots[sidx];
    for (long u = S.unit0; u < S.unit0 + S.units; u++) P->h_unit_slot[u] = -1;
    const int fr = kFree;
@@ -1254,8 +1332,8 @@ Inner<float>& inner_of<float>(pk_batch_s* P) {
 }
 
 // Launches one tick on the stream for row units [0, units).  With `prof`,
-// GMRES kernels are bracketed by profiler events and the cycle stops issuing
-// steps once no slot is still iterating (the arithmetic is the same).
+// the GMRES step kernels carry profiler events and activity counters (the
+// same kernels, arguments and launches as the captured graph).
 template <class T>
 static int launch_tick(pk_batch_s* P, long units, bool prof) {
    Ctx& c = ctx();
@@ -1289,74 +1367,43 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    T* zfirst = o.poly ? I.z0.p : I.z.p;
    bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, P->r.p, I.rhs.p, I.e.p, I.V.p, zfirst, I.binv.p, o,
                                                  part, tk, gtk, S);
-   const double s = sizeof(T);
-   std::vector<int> hf, ph;
    // steps 0 .. m-1, then the column past the last (j = m: the norm of V_m
-   // that finishes column m-1 of the slots still iterating)
+   // that finishes column m-1 of the slots still iterating).  With `prof`
+   // the step kernels are launched with events that record their own start
+   // and end, and count each slot's activity (algorithmic bytes, summed by the
+   // host afterwards); every step is issued, as in the captured graph.
+   long long* act = prof ? P->act.p : nullptr;
+   unsigned long long* st = prof ? P->stamps.p : nullptr;
+   // the span of a profiled launch into the next record
+   auto span = [&](int which) -> int {
+      if (!prof) return 0;
+      if (P->nrec == kRecs) PK_CALL(flush_spans(P));
+      stamp_span_kernel<<<1, 256, 0, c.stream>>>(P->stamps.p, nwg, P->recs.p + 2 * (long)P->nrec);
+      P->rec_kind[P->nrec++] = which;
+      return 0;
+   };
    for (int j = 0; j <= m; j++) {
       const bool full = j < m;
-      if (prof) {
-         // issue only steps in which some slot iterates; bytes of those slots
-         hf.resize(4 * (size_t)ns);
-         ph.resize(ns);
-         PK_HIP(hipMemcpyAsync(hf.data(), S.flags, sizeof(int) * 4 * ns, hipMemcpyDeviceToHost, c.stream));
-         PK_HIP(hipMemcpyAsync(ph.data(), S.phase, sizeof(int) * ns, hipMemcpyDeviceToHost, c.stream));
-         PK_HIP(hipStreamSynchronize(c.stream));
-         double n = 0.0, nnz = 0.0, bsn = 0.0, nedge = 0.0;
-         for (int b = 0; b < ns; b++)
-            if (P->slots[b].used && ph[b] == kInner && !hf[4 * b]) {
-               if (j == P->slots[b].mlen) {
-                  nedge += P->slots[b].n;
-                  continue;
-               }
-               n += P->slots[b].n;
-               nnz += (double)P->slots[b].nnz;
-               bsn += (double)P->slots[b].n * P->slots[b].pbs;
-            }
-         if (n == 0.0 && nedge == 0.0) break;
-         // algorithmic bytes: spmv_dot is an SpMV (value and column of every
-         // stored entry, z gathered once, w written) plus j+1 basis vectors
-         // (the column past a slot's last reads V_j only); orth reads w and j+1
-         // basis vectors, writes V[j+1] and z, reads the block inverses and A
-         // (the preconditioner's workgroup-block sweep).
-         // The kernels record their execution span on the device clock.
-         clk_reset_kernel<<<1, 64, 0, c.stream>>>(P->clk.p, 6);
-         if (o.poly && full)
-            bpoly_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p,
-                                                               I.z.p, S, P->clk.p + 4);
-         bspmv_dot_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk,
-                                                                S, P->clk.p);
-         if (full)
-            borth_timed_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S,
-                                                               P->clk.p + 2);
-         unsigned long long h[6];
-         PK_HIP(hipMemcpyAsync(h, P->clk.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
-         PK_HIP(hipStreamSynchronize(c.stream));
-         if (n > 0.0) {
-            prof_add("gmres_spmv_dot", 1, (double)(h[1] - h[0]) / P->clock_hz,
-                     nnz * (s + 4.0) + 2.0 * n * s + (j + 1.0) * n * s + nedge * s);
-            prof_add("gmres_orth", 1, (double)(h[3] - h[2]) / P->clock_hz,
-                     n * s * (j + 4.0) + bsn * s + nnz * (s + 4.0));
-            if (o.poly)
-               prof_add("gmres_poly", 1, (double)(h[5] - h[4]) / P->clock_hz,
-                        nnz * (s + 4.0) + 3.0 * n * s + bsn * s);
-         } else {
-            prof_add("gmres_norm", 1, (double)(h[1] - h[0]) / P->clock_hz, nedge * s);
-         }
-      } else {
-         if (o.poly && full)
-            bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
-                                                         S, 0, nullptr);
-         bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S);
-         if (full)
-            borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S);
+      if (o.poly && full) {
+         bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
+                                                      S, 0, nullptr, act, st);
+         PK_CALL(span(2));
+      }
+      bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S,
+                                                       act, st);
+      PK_CALL(span(0));
+      if (full) {
+         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S, act,
+                                                      st);
+         PK_CALL(span(1));
       }
    }
    btrisolve_kernel<<<ns, kRows, 0, c.stream>>>(ns, S);
    if (o.poly) {
       // e += M^-1 u: u and S u, then the second term, then the sum
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
-      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1, I.e.p);
+      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1, I.e.p,
+                                                   nullptr, nullptr);
    } else {
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 1, nullptr,
                                                      nullptr);
@@ -1384,11 +1431,17 @@ template <class T>
 static int run_tick(pk_batch_s* P, long units) {
    Ctx& c = ctx();
    if (profiling()) {
-      if (!P->clk.p) {
+      if (!P->act.p) {
          int khz = 0;
          PK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device));
-         PK_REQUIRE(khz > 0 && !P->clk.alloc(6), "no device clock for profiling");
+         PK_REQUIRE(khz > 0, "no device wall clock for profiling");
          P->clock_hz = khz * 1e3;
+         PK_REQUIRE(!P->act.alloc((size_t)P->max_slots * kAct) && !P->stamps.alloc(2 * (size_t)P->max_units) &&
+                        !P->recs.alloc(2 * (size_t)kRecs),
+                    "device allocation failed");
+         P->rec_kind.assign(kRecs, 0);
+         P->nrec = 0;
+         PK_HIP(hipMemsetAsync(P->act.p, 0, sizeof(long long) * P->max_slots * kAct, c.stream));
       }
       return launch_tick<T>(P, units, true);
    }
@@ -1521,7 +1574,7 @@ static int fixed_run(pk_batch_s* P) {
          PK_HIP(hipStreamSynchronize(c.stream));
          P->tick_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
          PK_REQUIRE(herr == 0, "singular diagonal block in the block-Jacobi preconditioner");
-         return 0;
+         return fold_activity(P);
       }
    }
 }

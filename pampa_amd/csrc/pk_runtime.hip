@@ -38,8 +38,12 @@ static hipEvent_t take_event() {
 
 bool profiling() { return g_prof; }
 
+static void prof_resolve();
+
 bool prof_events(const char* name, double bytes, hipEvent_t* e0, hipEvent_t* e1) {
    if (!g_prof) return false;
+   // resolve in batches so that long profiled runs reuse their events
+   if (g_pending.size() >= 8192) prof_resolve();
    ProfRec r{name, bytes, take_event(), take_event()};
    g_pending.push_back(r);
    *e0 = r.e0;
@@ -66,6 +70,7 @@ static void prof_resolve() {
    (void)hipStreamSynchronize(g_ctx.stream);
    for (auto& r : g_pending) {
       float ms = 0.f;
+      (void)hipEventSynchronize(r.e1);  // launches on other streams (sweep handles)
       (void)hipEventElapsedTime(&ms, r.e0, r.e1);
       ProfAcc* acc = nullptr;
       for (auto& p : g_prof_acc)

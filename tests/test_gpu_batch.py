@@ -156,3 +156,39 @@ def test_pool_continuous_batching(pk, members, precision):
         assert np.array_equal(x, _MIX[precision][b][1]), members[b][0]
     print(f"   pool: {len(got)} solves, {waits} admission waits, ticks={pool.capacity()['ticks']}",
           flush=True)
+
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "batched_classes.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(FIXTURE), reason="oracle fixture of the largest classes not generated")
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_largest_size_classes_against_oracle_fixture(pk, tmp_path, precision):
+    """The batched family's four largest size classes (128x128 and 192x192 at 16
+    groups, 256x256 at 8 and 16 groups: 262,144 to 1,048,576 rows) served
+    together in one pool, against the FP64 oracle's keff and every 64th entry
+    of its power-normalised flux (tests/golden/make_fixtures.py)."""
+    from pampa_amd import workloads as W
+    fx = np.load(FIXTURE)
+    insts = sorted(int(k[4:-5]) for k in fx.files if k.endswith("_keff"))
+    st = int(fx["stride"])
+    paths = []
+    for i in insts:
+        d = str(tmp_path / f"inst-{i}")
+        W.batched_instance(d, i)
+        paths.append(os.path.join(d, "input.pmp"))
+    prec = pk.PK_PREC_MIXED if precision == "mixed" else pk.PK_PREC_FP64
+    srv = pk.Server(budget_bytes=64e9, max_slots=len(paths), precision=prec)
+    for p in paths:
+        srv.add_case(p)
+    n = len(paths)
+    res = srv.run(np.zeros(n), np.arange(n), np.zeros(n, np.uint64), rel=0.0, warm=False, keep_flux=list(range(n)))
+    srv.close()
+    for r, i in enumerate(insts):
+        dk = abs(res["keff"][r] - float(fx[f"inst{i}_keff"])) * 1e5
+        xs = np.asarray(res["flux"][r])[::st]
+        xo = fx[f"inst{i}_phi"]
+        l2 = np.linalg.norm(_shape(xs) - _shape(xo)) / np.linalg.norm(_shape(xo))
+        print(f"[case] instance {i} {precision}: dk={dk:.2e} pcm flux L2 (every {st}th entry) {l2:.2e}", flush=True)
+        assert dk <= DK_PCM
+        assert l2 <= FLUX_L2

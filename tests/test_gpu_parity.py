@@ -232,3 +232,36 @@ def test_sn_tiny_matches_assembled_operator(oracle_mod, tmp_path):
     finally:
         os.chdir(cwd)
     assert abs(k - k_as) * 1e5 <= DK_PCM
+
+
+LATTICE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hex_s6_lattice.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(LATTICE), reason="oracle fixture of the S6 lattice not generated")
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_full_s6_lattice_against_oracle_fixture(tmp_path, precision):
+    """The full hexagonal S6 lattice (317,760 prisms, 4 groups, 61,009,920 angular
+    unknowns) through the drop-in against the FP64 oracle's source iteration:
+    keff and every 64th entry of the scalar-flux shape
+    (tests/golden/make_fixtures.py)."""
+    import pampa_amd
+    from pampa_amd import workloads as W
+    fx = np.load(LATTICE)
+    st = int(fx["stride"])
+    d = W.hex_sn_3d(str(tmp_path / "hex-s6"))
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        with pampa_amd.DropIn("input.pmp", "-silent", "-pk_precision", precision) as run:
+            run.solve()
+            k, _ = run.result()
+            phi = run.field("scalar-flux")
+    finally:
+        os.chdir(cwd)
+    shape = lambda v: np.asarray(v, np.float64) / np.sum(v)  # noqa: E731
+    xs, xo = shape(phi[::st]), shape(fx["phiw"])
+    dk = abs(k - float(fx["keff"])) * 1e5
+    l2 = np.linalg.norm(xs - xo) / np.linalg.norm(xo)
+    print(f"[case] S6 lattice {precision}: dk={dk:.2e} pcm flux L2 (every {st}th entry) {l2:.2e}", flush=True)
+    assert dk <= DK_PCM
+    assert l2 <= FLUX_L2
