@@ -158,12 +158,107 @@ def build_unstructured(m: pmp.UnstructuredInput) -> Mesh:
                 bcs=dict(m.bcs), **a)
 
 
+def build_partitioned(m: pmp.PartitionedInput) -> Mesh:
+    """The mesh data as read (PartitionedMesh.hxx: nothing to build), one
+    domain: no face may lead to a ghost cell."""
+    C = m.C
+    fmax = int(m.nfaces.max())
+    a = _alloc(C, fmax)
+    a["vol"][:] = m.vol
+    a["cen"][:] = m.cen[:C]
+    a["cmat"][:] = m.materials[:C]
+    a["nfaces"][:] = m.nfaces
+    f = 0
+    for i in range(C):
+        n = int(m.nfaces[i])
+        if len(m.farea[i]) != n or len(m.fnb[i]) != n:
+            raise pmp.PmpError("wrong number of face values for a cell")
+        a["farea"][i, :n] = m.farea[i]
+        a["fnb"][i, :n] = m.fnb[i]
+        a["fcen"][i, :n] = m.fcen[f:f + n]
+        a["fnorm"][i, :n] = m.fnorm[f:f + n]
+        f += n
+    if np.any(a["fnb"] >= C):
+        raise pmp.PmpError("a face leads to a ghost cell")
+    mesh = Mesh(C=C, fmax=fmax, num_dims=m.num_dims, boundaries=list(m.boundaries), bcs=dict(m.bcs), **a)
+    mesh.partitioned = True
+    return mesh
+
+
+def split_materials(mesh: Mesh, materials: list):
+    """Mesh::splitMaterials (Mesh.cxx:9-60): every connected region of a split
+    material becomes a material of its own, <name>-<k>, appended to the list;
+    a boundary-condition material's regions name boundaries of their own."""
+    import copy
+    subs = {}
+    for i in range(mesh.C):
+        im = int(mesh.cmat[i])
+        if not materials[im].split:
+            continue
+        im2 = len(materials)
+        mesh.cmat[i] = im2
+        stack = [i]
+        while stack:
+            c = stack.pop()
+            for f in range(mesh.nfaces[c]):
+                c2 = int(mesh.fnb[c, f])
+                if c2 >= 0 and mesh.cmat[c2] == im:
+                    mesh.cmat[c2] = im2
+                    stack.append(c2)
+        sub = copy.copy(materials[im])
+        sub.name = f"{materials[im].name}-{subs.get(im, 0)}"
+        subs[im] = subs.get(im, 0) + 1
+        sub.split = False
+        if sub.bc:
+            mesh.boundaries.append(sub.name)
+        materials.append(sub)
+
+
+def remove_bc_materials(mesh: Mesh, materials: list) -> Mesh:
+    """Mesh::removeBCMaterials (Mesh.cxx:63-204): cells of boundary-condition
+    materials leave the mesh and their faces become faces of the boundary named
+    after the material."""
+    if getattr(mesh, "partitioned", False):
+        raise pmp.PmpError("unable to remove materials from a partitioned mesh")
+    keep = np.array([not materials[int(m)].bc for m in mesh.cmat])
+    new = np.full(mesh.C, -1, np.int64)
+    new[keep] = np.arange(int(keep.sum()))
+    fnb = mesh.fnb.copy()
+    for i in range(mesh.C):
+        for f in range(mesh.nfaces[i]):
+            c2 = int(fnb[i, f])
+            if c2 >= 0:
+                m2 = materials[int(mesh.cmat[c2])]
+                fnb[i, f] = new[c2] if keep[c2] else -mesh.boundaries.index(m2.name) - 1
+    sel = lambda a: np.ascontiguousarray(a[keep])  # noqa: E731
+    nf = sel(mesh.nfaces)
+    return Mesh(C=int(keep.sum()), fmax=int(nf.max()), num_dims=mesh.num_dims, vol=sel(mesh.vol),
+                cen=sel(mesh.cen), cmat=sel(mesh.cmat), nfaces=nf, farea=sel(mesh.farea), fcen=sel(mesh.fcen),
+                fnorm=sel(mesh.fnorm), fnb=np.ascontiguousarray(fnb[keep], np.int32),
+                boundaries=list(mesh.boundaries), bcs=dict(mesh.bcs))
+
+
 def build_mesh(case: pmp.Case) -> Mesh:
+    """The case's mesh, with the edits the parser makes before the first solver
+    (Parser.cxx:107-130): split materials, then boundary-condition materials
+    removed.  Split materials are appended to case.materials (once)."""
     if case.mesh_type == "cartesian":
-        return build_cartesian(case.mesh)
-    if case.mesh_type == "unstructured":
-        return build_unstructured(case.mesh)
-    raise pmp.PmpError("wrong mesh type")
+        mesh = build_cartesian(case.mesh)
+    elif case.mesh_type == "unstructured":
+        mesh = build_unstructured(case.mesh)
+    elif case.mesh_type == "partitioned":
+        mesh = build_partitioned(case.mesh)
+    else:
+        raise pmp.PmpError("wrong mesh type")
+    if any(m.split for m in case.materials):
+        if getattr(case, "_split_done", None) is None:
+            case._split_done = len(case.materials)
+        else:
+            del case.materials[case._split_done:]
+        split_materials(mesh, case.materials)
+    if any(m.bc for m in case.materials):
+        mesh = remove_bc_materials(mesh, case.materials)
+    return mesh
 
 
 # --------------------------------------------------------------------------

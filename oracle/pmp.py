@@ -430,6 +430,95 @@ def _read_solver_block(rd: Lines, s: SolverInput, boundaries: list):
 
 
 @dataclass
+class PartitionedInput:
+    """Mesh data of one domain (PartitionedMesh.cxx:4-263)."""
+    C: int = 0
+    ghost: int = 0
+    points: np.ndarray | None = None     # (np, 3)
+    cpoints: list = field(default_factory=list)
+    vol: np.ndarray | None = None
+    cen: np.ndarray | None = None        # (C + ghost, 3)
+    materials: np.ndarray | None = None  # 0-based, C + ghost
+    nfaces: np.ndarray | None = None
+    farea: list = field(default_factory=list)     # ragged per cell
+    fcen: np.ndarray | None = None       # (F, 3)
+    fnorm: np.ndarray | None = None      # (F, 3)
+    fnb: list = field(default_factory=list)       # ragged per cell
+    boundaries: list = field(default_factory=list)
+    bcs: dict = field(default_factory=dict)
+    num_dims: int = 1
+
+
+def read_partitioned(path: str) -> PartitionedInput:
+    """PartitionedMesh::read (PartitionedMesh.cxx:4-263)."""
+    rd = Lines(path)
+    m = PartitionedInput()
+    F = 0
+
+    def ragged(n, conv):
+        rows = []
+        for _ in range(n):
+            line = rd.next()
+            if not line:
+                raise PmpError("missing data")
+            rows.append([conv(v) for v in line])
+        return rows
+
+    while True:
+        line = rd.next()
+        if not line:
+            break
+        k = line[0]
+        if k == "points":
+            n = int(line[1])
+            m.points = np.array(ragged(n, float))
+        elif k == "cells":
+            m.C, m.ghost = int(line[1]), int(line[2])
+            if m.ghost < 1:
+                raise PmpError("wrong number of ghost cells")
+        elif k == "cell-points":
+            m.cpoints = ragged(m.C, int)
+        elif k == "cell-volumes":
+            m.vol = np.array(rd.floats(m.C))
+        elif k == "cell-centroids":
+            m.cen = np.array(ragged(m.C + m.ghost, float))
+            c = m.cen[:m.C]
+            m.num_dims = 1 + int(np.any(c[1:, 1] != c[0, 1])) + int(np.any(c[1:, 2] != c[0, 2]))
+        elif k == "cell-materials":
+            m.materials = np.array(rd.ints(m.C + m.ghost)) - 1
+        elif k in ("cell-nodal-indices",):
+            rd.ints(m.C)
+        elif k == "cell-global-indices":
+            rd.ints(m.C + m.ghost)
+        elif k == "faces":
+            m.nfaces = np.array(rd.ints(m.C), np.int32)
+            F = int(m.nfaces.sum())
+        elif k == "face-areas":
+            if int(line[2]) != F:
+                raise PmpError("wrong number of face areas")
+            m.farea = ragged(m.C, float)
+        elif k in ("face-centroids", "face-normals"):
+            if int(line[2]) != 3 * F:
+                raise PmpError("wrong number of face coordinates")
+            a = np.array(ragged(F, float))
+            if k == "face-centroids":
+                m.fcen = a
+            else:
+                m.fnorm = a
+        elif k == "face-neighbors":
+            if int(line[2]) != F:
+                raise PmpError("wrong number of face neighbors")
+            m.fnb = ragged(m.C, int)
+        elif k == "boundary":
+            m.boundaries.append(line[1])
+        elif k == "bc":
+            m.bcs[m.boundaries.index(line[1])] = _read_bc(line, 2, rd)
+        else:
+            raise PmpError(f"unrecognized keyword '{k}'")
+    return m
+
+
+@dataclass
 class Case:
     mesh_type: str
     mesh: object
@@ -462,6 +551,8 @@ def read_input(path: str) -> Case:
                 mesh = read_cartesian(resolve(line[2]))
             elif mesh_type == "unstructured":
                 mesh = read_unstructured(resolve(line[2]))
+            elif mesh_type == "partitioned":
+                mesh = read_partitioned(resolve(line[2]))
             else:
                 raise PmpError("wrong mesh type")
         elif k == "material":
@@ -470,6 +561,9 @@ def read_input(path: str) -> Case:
                 _read_material_block(rd, mat)
             else:
                 _read_material_block(Lines(resolve(line[2])), mat)
+            if mat.bc:
+                # a boundary named after the material (Parser.cxx:93)
+                mesh.boundaries.append(mat.name)
             materials.append(mat)
         elif k == "solver":
             solver = SolverInput(line[1])

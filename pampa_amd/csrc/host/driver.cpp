@@ -91,7 +91,7 @@ int Driver::parse(const std::string& filename, std::vector<double>& dt) {
          else if (line[1] == "unstructured")
             mesh.reset(new UnstructuredExtrudedMesh());
          else if (line[1] == "partitioned")
-            PAMPA_CHECK(true, "partitioned meshes are not supported (single-domain build)");
+            mesh.reset(new PartitionedMesh());
          else
             PAMPA_CHECK(true, "wrong mesh type");
          PAMPA_CHECK(mesh->read(line[2], base), "unable to read the mesh from " + line[2]);
@@ -106,11 +106,28 @@ int Driver::parse(const std::string& filename, std::vector<double>& dt) {
             PAMPA_CHECK(m.read(in), "unable to read the material from " + filename);
          else
             PAMPA_CHECK(m.read(line[2], base), "unable to read the material from " + line[2]);
-         PAMPA_CHECK(m.bc || m.split, "boundary-condition and split materials are not supported");
+         // a boundary-condition material names a boundary of the mesh (Parser.cxx:93)
+         if (m.bc) {
+            PAMPA_CHECK(!mesh, "missing mesh definition before a boundary-condition material");
+            mesh->addBoundary(m.name);
+         }
          materials.push_back(m);
       } else if (k == "solver") {
          PAMPA_CHECK(line.size() < 3, "wrong number of arguments for keyword '" + k + "'");
          PAMPA_CHECK(!mesh, "missing mesh definition before the solver");
+         if (!mesh_ready) {
+            // split materials, then boundary-condition materials out of the mesh
+            // (Parser.cxx:107-130)
+            bool split = false, bc = false;
+            for (const auto& m : materials) {
+               split |= m.split;
+               bc |= m.bc;
+            }
+            if (split) PAMPA_CHECK(split_materials(*mesh, materials), "unable to split materials in the mesh");
+            if (bc)
+               PAMPA_CHECK(remove_bc_materials(*mesh, materials),
+                           "unable to remove boundary-condition materials from the mesh");
+         }
          mesh_ready = true;
          std::unique_ptr<Solver> s;
          if (line[1] == "diffusion")
@@ -146,8 +163,7 @@ int Driver::parse(const std::string& filename, std::vector<double>& dt) {
          if (line[1] == "verbose") {
             PAMPA_CHECK(input::read(options.verbose_solver, line[2]), "wrong switch for verbose PETSc output");
          } else if (line[1] == "dump") {
-            bool d = false;
-            PAMPA_CHECK(input::read(d, line[2]), "wrong switch to write the solution in PETSc format");
+            PAMPA_CHECK(input::read(options.dump, line[2]), "wrong switch to write the solution in PETSc format");
          } else {
             PAMPA_CHECK(setOption(line[1], line[2]), "wrong PETSc/SLEPc option");
          }
@@ -188,7 +204,7 @@ int Driver::initialize(int argc, char* argv[], std::vector<double>& dt) {
       else if (s == "-petsc_verbose")
          options.verbose_solver = true;
       else if (s == "-petsc_dump")
-         continue;
+         options.dump = true;
       else if (s.size() > 1 && s[0] == '-' && a + 1 < argc && argv[a + 1][0] != '-')
          PAMPA_CHECK(setOption(s, argv[++a]), "wrong command-line option");
       else if (s.size() > 1 && s[0] == '-')

@@ -28,6 +28,10 @@ class Mesh {
    std::vector<std::string> boundaries;
    std::vector<BoundaryCondition> bcs;       // indexed by boundary + 1
 
+   bool partitioned = false;                 // read from a partitioned-mesh data file
+   // a boundary named after a boundary-condition material (Parser.cxx:93)
+   void addBoundary(const std::string& name);
+
    int findBoundary(const std::string& name) const {
       for (size_t b = 0; b < boundaries.size(); b++)
          if (boundaries[b] == name) return (int)b;
@@ -64,5 +68,28 @@ class UnstructuredExtrudedMesh : public Mesh {
    int default_boundary = -1;
    std::vector<int> mat_in;
 };
+
+// Mesh data of one domain in the plain-text format of PartitionedMesh.cxx:4-263
+// (points, cells with ghost and global counts, cell points, volumes,
+// centroids, materials, nodal and global indices, faces, face areas,
+// centroids, normals and neighbours, boundaries and conditions).
+class PartitionedMesh : public Mesh {
+  public:
+   PartitionedMesh() { partitioned = true; }
+   int read(const std::string& filename, const std::string& base_dir) override;
+   int build() override;
+
+  private:
+   int num_ghost = 0, num_cells_global = 0;
+   std::vector<int> global;
+};
+
+struct Material;
+// Each connected region of a split material becomes a material of its own,
+// named <material>-<k> (Mesh.cxx:9-60).
+int split_materials(Mesh& mesh, std::vector<Material>& materials);
+// Cells of boundary-condition materials removed; their faces become faces of
+// the boundary named after the material (Mesh.cxx:63-204).
+int remove_bc_materials(Mesh& mesh, const std::vector<Material>& materials);
 
 }  // namespace pampa

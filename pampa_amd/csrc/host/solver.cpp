@@ -1,5 +1,9 @@
 #include "solver.hpp"
 
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+
 #include <algorithm>
 #include <cmath>
 
@@ -152,8 +156,45 @@ int NeutronicSolver::output(const std::string& path, int n, bool write_mesh) {
    output::print("Power", power_total, true, 3);
    if (write_mesh) PAMPA_CHECK(mesh->writeVTK(path + "/output", n), "unable to write the mesh");
    PAMPA_CHECK(writeVTK(path, n), "unable to write the solution in .vtk format");
+   PAMPA_CHECK(writePETSc(n), "unable to write the solution in PETSc format");
    return 0;
 }
+
+// A vector in PETSc's binary format, as VecView on a binary viewer writes it:
+// big-endian int32 class id 1211214 and length, then the values as big-endian
+// IEEE doubles; the viewer's companion <file>.info is written empty.  File
+// <prefix>_<n>.ptc in the working directory (petsc.cxx:491-511).
+static int write_petsc_vec(const std::string& prefix, int n, pk_vec v) {
+   if (!options.dump) return 0;
+   const long N = pk_vec_size(v);
+   std::vector<double> h(N);
+   PK_CHECK(pk_vec_get_values(v, h.data()));
+   const std::string fname = prefix + "_" + std::to_string(n) + ".ptc";
+   std::ofstream f(fname, std::ios::binary);
+   PAMPA_CHECK(!f.is_open(), "unable to open " + fname);
+   auto be32 = [&](uint32_t x) {
+      unsigned char b[4] = {(unsigned char)(x >> 24), (unsigned char)(x >> 16), (unsigned char)(x >> 8),
+                            (unsigned char)x};
+      f.write((const char*)b, 4);
+   };
+   be32(1211214u);
+   be32((uint32_t)N);
+   std::vector<unsigned char> buf(8 * (size_t)N);
+   for (long i = 0; i < N; i++) {
+      uint64_t x;
+      std::memcpy(&x, &h[i], 8);
+      for (int k = 0; k < 8; k++) buf[8 * i + k] = (unsigned char)(x >> (56 - 8 * k));
+   }
+   f.write((const char*)buf.data(), (std::streamsize)buf.size());
+   PAMPA_CHECK(!f.good(), "unable to write " + fname);
+   std::ofstream info(fname + ".info");
+   PAMPA_CHECK(!info.is_open(), "unable to open " + fname + ".info");
+   return 0;
+}
+
+int DiffusionSolver::writePETSc(int n) const { return write_petsc_vec("scalar_flux", n, phi); }
+
+int SNSolver::writePETSc(int n) const { return write_petsc_vec("angular_flux", n, psi); }
 
 int NeutronicSolver::finalize() {
    pk_vec* vs[] = {&phi, &phi0, &T, &S, &q, &P};

@@ -26,6 +26,7 @@ struct SolverOptions {
    double sn_tol_k = 1e-9, sn_tol_phi = 1e-8;
    int sn_max_outer = 100000;
    bool verbose_solver = false;
+   bool dump = false;  // -petsc_dump / `petsc dump 1`: solution vectors in PETSc binary format
 };
 extern SolverOptions options;
 
@@ -97,6 +98,8 @@ class NeutronicSolver : public Solver {
    virtual int readKeyword(const std::vector<std::string>& line, std::istream& in);
    virtual int build() = 0;
    virtual int writeVTK(const std::string& path, int n) = 0;
+   // the solution vector in PETSc's binary format (petsc.cxx:491-511), with -petsc_dump
+   virtual int writePETSc(int n) const = 0;
    int uploadMaterials(bool diffusion);
    int powerAndProduction(bool normalize);  // NeutronicSolver.cxx:46-116
 
@@ -132,6 +135,7 @@ class DiffusionSolver : public NeutronicSolver {
   protected:
    int build() override;
    int writeVTK(const std::string& path, int n) override;
+   int writePETSc(int n) const override;
 
   private:
    pk_mat R = nullptr, F = nullptr;
@@ -156,6 +160,7 @@ class SNSolver : public NeutronicSolver {
    int readKeyword(const std::vector<std::string>& line, std::istream& in) override;
    int build() override;
    int writeVTK(const std::string& path, int n) override;
+   int writePETSc(int n) const override;
 
   private:
    int order = -1;

@@ -63,3 +63,30 @@ def test_c_abi_fields_and_power():
                 run.field("no-such-field")
     finally:
         os.chdir(cwd)
+
+
+def test_petsc_dump(tmp_path):
+    """-petsc_dump writes the solution vector as VecView writes it to a PETSc
+    binary viewer (petsc.cxx:491-511): scalar_flux_<n>.ptc in the working
+    directory, big-endian int32 class id 1211214 and length, then the values
+    as big-endian doubles, beside an (empty) .info file."""
+    import shutil
+    import pampa_amd
+    from pampa_amd import workloads as W
+    d = W.slab_1d(str(tmp_path / "slab"))
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        with pampa_amd.DropIn("input.pmp", "-silent", "-petsc_dump") as run:
+            run.solve()
+            phi = run.field("scalar-flux")
+        # written by the solve's output step
+        raw = open(os.path.join(d, "scalar_flux_0.ptc"), "rb").read()
+    finally:
+        os.chdir(cwd)
+    hdr = np.frombuffer(raw[:8], dtype=">i4")
+    assert hdr[0] == 1211214 and hdr[1] == len(phi)
+    vals = np.frombuffer(raw[8:], dtype=">f8")
+    assert np.array_equal(vals, phi)
+    assert os.path.exists(os.path.join(d, "scalar_flux_0.ptc.info"))
+    shutil.rmtree(d)

@@ -1,10 +1,7 @@
 """VTK output of the drop-in (`-vtk`): the legacy ASCII file the reference's
 vtk.cxx:28-170 writes (header, points, cells, cell types, then the materials and
 the solution as cell scalars), read back and checked against the mesh sizes and
-the fields the C-ABI returns for the same solve.
-
-Not yet run on a GPU: kept under a name that pytest does not collect until it
-has passed there once."""
+the fields the C-ABI returns for the same solve."""
 import os
 import shutil
 

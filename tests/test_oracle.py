@@ -394,3 +394,85 @@ def test_reflective_half_equals_mirrored_full(oracle_mod, tmp_path, solver):
     (tmp_path / "half" / "mesh.pmp").write_text(txt)
     vac = _keff(oracle_mod, tmp_path / "half")
     assert abs(vac.keff - half.keff) > 1e-3 * half.keff
+
+
+# ---- input formats: partitioned mesh data, boundary-condition and split materials ----
+
+def _ops(oracle_mod, d):
+    cwd = os.getcwd()
+    os.chdir(d)
+    try:
+        return oracle_mod.solve_input("input.pmp", return_ops=True)
+    finally:
+        os.chdir(cwd)
+
+
+def _partitioned_case(oracle_mod, tmp_path, fmt):
+    """The reference's cartesian-diffusion quarter core written as partitioned
+    mesh data of one domain, with the same materials and solver."""
+    import meshdata
+    src = os.path.join(REF, "pwr-iaea-benchmark", "cartesian-diffusion")
+    case = oracle_mod.pmp.read_input(os.path.join(src, "input.pmp"))
+    mesh = oracle_mod.build_mesh(case)
+    d = tmp_path / ("part-" + fmt.__name__)
+    d.mkdir()
+    meshdata.write_partitioned(str(d / "mesh.pmp"), mesh, fmt)
+    text = open(os.path.join(src, "input.pmp")).read().replace("mesh cartesian mesh.pmp", "mesh partitioned mesh.pmp")
+    text = text.replace("../materials/", os.path.join(REF, "pwr-iaea-benchmark", "materials") + "/")
+    (d / "input.pmp").write_text(text)
+    return str(d), src
+
+
+def test_partitioned_mesh_data_equals_cartesian(oracle_mod, tmp_path, golden_keff):
+    """Mesh data written exactly (PartitionedMesh.cxx format) gives the Cartesian
+    mesh's operators bit for bit; written as the reference's writer does (3
+    decimals, Mesh.cxx:413-414) it still gives the golden keff of the
+    reference's partitioned rerun (check_ref.txt:93)."""
+    import meshdata
+    d, src = _partitioned_case(oracle_mod, tmp_path, repr)
+    a, b = _ops(oracle_mod, src), _ops(oracle_mod, d)
+    for X, Y in ((a.R, b.R), (a.F, b.F)):
+        assert np.array_equal(X.ptr, Y.ptr) and np.array_equal(X.col, Y.col) and np.array_equal(X.val, Y.val)
+    assert abs(a.keff - b.keff) < 1e-13
+    d3, _ = _partitioned_case(oracle_mod, tmp_path, meshdata.fixed3)
+    c = _ops(oracle_mod, d3)
+    assert f"{c.keff:.6f}" == f"{golden_keff['pwr-iaea-benchmark/cartesian-diffusion']:.6f}"
+
+
+def test_boundary_condition_material_equals_boundary(oracle_mod, tmp_path):
+    """A column and a row of a boundary-condition material outside a quarter
+    core, with vacuum on that material's boundary, give the operators of the
+    core alone with vacuum on +x/+y (Mesh.cxx:63-204): same cells in the same
+    order, the removed cells' faces turned into boundary faces."""
+    import meshdata
+    from pampa_amd import workloads as W
+    core = W.quarter_core_2d(str(tmp_path / "core"), seed=4, side=24)
+    wall = meshdata.with_wall(core, str(tmp_path / "wall"))
+    a, b = _ops(oracle_mod, core), _ops(oracle_mod, wall)
+    for X, Y in ((a.R, b.R), (a.F, b.F)):
+        assert np.array_equal(X.ptr, Y.ptr) and np.array_equal(X.col, Y.col) and np.array_equal(X.val, Y.val)
+    assert abs(a.keff - b.keff) < 1e-13
+
+
+def test_split_material_regions(oracle_mod, tmp_path):
+    """A split material (Mesh.cxx:9-60) becomes one material per connected
+    region -- as many as scipy's 4-connected labelling of the Cartesian map
+    finds -- with the parent's data, so the operators and keff are unchanged."""
+    import scipy.ndimage as ndi
+    from pampa_amd import workloads as W
+    d = W.quarter_core_2d(str(tmp_path / "core"), seed=4, side=34)
+    before = _ops(oracle_mod, d)
+    with open(os.path.join(d, "fuel-absorber.pmp"), "a") as fh:
+        fh.write("split 1\n")
+    case = oracle_mod.pmp.read_input(os.path.join(d, "input.pmp"))
+    mesh = oracle_mod.build_mesh(case)
+    grid = np.array([int(v) for v in open(os.path.join(d, "mesh.pmp")).read().split("materials")[1].split()[1:]])
+    _, nreg = ndi.label(grid.reshape(34, 34) == 3)
+    assert nreg > 1
+    assert len(case.materials) == 4 + nreg
+    assert sorted(set(mesh.cmat.tolist())) == [0, 1, 3] + list(range(4, 4 + nreg))
+    assert [m.name for m in case.materials[4:]] == [f"fuel-absorber-{k}" for k in range(nreg)]
+    after = _ops(oracle_mod, d)
+    for X, Y in ((before.R, after.R), (before.F, after.F)):
+        assert np.array_equal(X.ptr, Y.ptr) and np.array_equal(X.col, Y.col) and np.array_equal(X.val, Y.val)
+    assert before.keff == after.keff
