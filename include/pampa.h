@@ -49,6 +49,12 @@ void pampa_ext_get_result(double* keff, double* power, int* error);
  * is that step's operator (DiffusionSolver.cxx:134-175). */
 void pampa_ext_get_operator(const char name[], long* n, long* nnz, int* row_ptr, int* col_idx,
                             double* val, int* error);
+/* Device address (HIP device memory of the library's GPU) and length of a
+ * field of the main solver, for a coupled code that exchanges fields on the
+ * same GPU without host copies (the fields of pampa_get_field/set_field,
+ * Solver.cxx:18-43).  Writes through it take effect at the next solve; the
+ * address holds until pampa_finalize. */
+void pampa_ext_field_device(const char name[], void** dev, long* n, int* error);
 /* Outer iterations, inner iterations (Krylov steps or sweeps), final residual
  * and device seconds of the last solve. */
 void pampa_ext_get_stats(int* outer, long* inner, double* residual, double* seconds, int* error);

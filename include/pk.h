@@ -102,8 +102,23 @@ int pk_mesh_destroy(pk_mesh* mesh);
  * D may be NULL (SN). */
 int pk_xs_create(int num_mat, int G, const double* sigt, const double* sigs, const double* nusf,
                  const double* kapf, const double* chi, const double* D, pk_xs* xs);
+/* Temperature-dependent tables (FeedbackNuclearData.hxx:8-150): material m
+ * owns num_sets[m] consecutive sets tabulated at increasing temperatures
+ * temps[] (one set: constant data); every table is [sets][G] (sigs
+ * [sets][G][G]).  Coefficients are evaluated at each cell's temperature
+ * (pk_xs_set_temperature) by linear interpolation between the two sets around
+ * it, the end set outside the tabulated range.  pk_xs_create is the one-set
+ * case.  The transient tables of pk_diffusion_assemble_transient (velocity,
+ * prompt and delayed spectra) are then per set as well. */
+int pk_xs_create_sets(int num_mat, int G, const int* num_sets, const double* temps, const double* sigt,
+                      const double* sigs, const double* nusf, const double* kapf, const double* chi,
+                      const double* D, pk_xs* xs);
+/* The cell temperatures the tables are evaluated at, a vector of one value per
+ * cell read at each assembly and normalisation (the solver's "temperature"
+ * field, which a coupled code sets through pampa_set_field); NULL: 0 K. */
+int pk_xs_set_temperature(pk_xs xs, pk_vec T);
 int pk_xs_destroy(pk_xs* xs);
-/* new values for the same table sizes (a perturbed copy of a material set) */
+/* new values for the same table sizes (a perturbed copy of a material set; [sets][G]) */
 int pk_xs_set(pk_xs xs, const double* sigt, const double* sigs, const double* nusf, const double* kapf,
               const double* chi, const double* D);
 /* R and F of DiffusionSolver::buildMatrices (DiffusionSolver.cxx:88-320, n = 0)
@@ -118,7 +133,8 @@ int pk_diffusion_refill(pk_mesh mesh, pk_xs xs, pk_mat R, pk_mat F);
 /* Transient rows (DiffusionSolver.cxx:134-148,168-175): R gains
  * V/(v dt) on the diagonal and the prompt fission -(1-beta) chi_p nuSf V / keff;
  * b = chi_d S + V/(v dt) phi0.  velocity/chip/chid are [num_mat][G], beta is
- * [num_mat]; S [num_cells]; phi0 and b are pk_vec of size num_cells*G. */
+ * [num_mat]; S [num_cells]; phi0 and b are pk_vec of size num_cells*G.  With
+ * temperature-dependent tables, velocity/chip/chid are [sets][G]. */
 int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const int* bc_type,
                                     const double* bc_value, const double* velocity, double dt,
                                     const double* chip, const double* chid, const double* beta,

@@ -58,6 +58,7 @@ def _declare(L):
     L.orc_unst_build.argtypes = ([_I, _P, _I, _P, _P, _I, _P, _I, _P, _P, _P, _I, _P, _I]
                                  + [_P] * 8)
     L.orc_diff_assemble.argtypes = [_I, _I] + [_P] * 8 + [_I] + [_P] * 13
+    L.orc_diff_assemble_sets.argtypes = [_I, _I] + [_P] * 8 + [_I] + [_P] * 17
     L.orc_diff_assemble_transient.argtypes = ([_I, _I] + [_P] * 8 + [_I] + [_P] * 7 + [_D, _P, _P, _D]
                                               + [_P] * 3)
     L.orc_diff_rhs.argtypes = [_I, _I, _P, _P, _P, _P, _D, _P, _P, _P]
@@ -275,30 +276,70 @@ class XS:
     chi: np.ndarray    # effective spectrum
     D: np.ndarray
     beta: np.ndarray   # (nm,)
+    # temperature sets (None: one set per material, tables indexed by material)
+    set0: np.ndarray | None = None
+    nset: np.ndarray | None = None
+    tref: np.ndarray | None = None
 
 
 def xs_tables(materials, G: int, diffusion: bool) -> XS:
+    """Per-material tables; with temperature-dependent materials the tables
+    are per set (material m's sets from set0[m], nset[m] of them, at
+    temperatures tref)."""
     nm = len(materials)
-    t = {k: np.zeros((nm, G)) for k in ("sigt", "nusf", "kapf", "chi", "D")}
-    sigs = np.zeros((nm, G, G))
+    sets = [mat.sets if mat.sets else [mat.nd] for mat in materials]
+    ns = sum(len(x) for x in sets)
+    t = {k: np.zeros((ns, G)) for k in ("sigt", "nusf", "kapf", "chi", "D")}
+    sigs = np.zeros((ns, G, G))
     beta = np.zeros(nm)
+    set0, nset, tref = np.zeros(nm, np.int32), np.zeros(nm, np.int32), np.zeros(ns)
+    k = 0
     for i, mat in enumerate(materials):
-        nd = mat.nd
-        if nd is None:
-            raise pmp.PmpError("missing nuclear data")
-        if nd.G != G:
-            raise pmp.PmpError("wrong number of energy groups")
-        t["sigt"][i] = nd.sigma_total
-        t["nusf"][i] = nd.nu_sigma_fission
-        t["kapf"][i] = nd.kappa_sigma_fission
-        t["chi"][i] = nd.chi_effective
-        if diffusion:
-            if nd.diffusion_coefficient is None:
-                raise pmp.PmpError("missing diffusion coefficients")
-            t["D"][i] = nd.diffusion_coefficient
-        sigs[i] = nd.sigma_scattering
+        set0[i], nset[i] = k, len(sets[i])
         beta[i] = mat.beta_total
-    return XS(G=G, sigs=sigs, beta=beta, **t)
+        for j, nd in enumerate(sets[i]):
+            if nd is None:
+                raise pmp.PmpError("missing nuclear data")
+            if nd.G != G:
+                raise pmp.PmpError("wrong number of energy groups")
+            t["sigt"][k] = nd.sigma_total
+            t["nusf"][k] = nd.nu_sigma_fission
+            t["kapf"][k] = nd.kappa_sigma_fission
+            t["chi"][k] = nd.chi_effective
+            if diffusion:
+                if nd.diffusion_coefficient is None:
+                    raise pmp.PmpError("missing diffusion coefficients")
+                t["D"][k] = nd.diffusion_coefficient
+            sigs[k] = nd.sigma_scattering
+            tref[k] = mat.temperatures[j] if mat.sets else 0.0
+            k += 1
+    if ns == nm:
+        return XS(G=G, sigs=sigs, beta=beta, **t)
+    return XS(G=G, sigs=sigs, beta=beta, set0=set0, nset=nset, tref=tref, **t)
+
+
+def cell_values(xs: XS, mesh, name: str, T=None) -> np.ndarray:
+    """(C, G) values of table `name` for every cell, at the cells'
+    temperatures T (0 when None) for temperature-dependent materials
+    (the interpolation of orc_sets in oracle.c)."""
+    tab = getattr(xs, name)
+    if xs.nset is None:
+        return tab[mesh.cmat]
+    T = np.zeros(mesh.C) if T is None else np.asarray(T, np.float64)
+    out = np.zeros((mesh.C, xs.G))
+    for i in range(mesh.C):
+        m = int(mesh.cmat[i])
+        s0, n = int(xs.set0[m]), int(xs.nset[m])
+        Tr = xs.tref[s0:s0 + n]
+        if n == 1 or T[i] <= Tr[0]:
+            out[i] = tab[s0]
+        elif T[i] > Tr[-1]:
+            out[i] = tab[s0 + n - 1]
+        else:
+            k = int(np.searchsorted(Tr, T[i], side="left"))
+            f = (T[i] - Tr[k - 1]) / (Tr[k] - Tr[k - 1])
+            out[i] = (1.0 - f) * tab[s0 + k - 1] + f * tab[s0 + k]
+    return out
 
 
 def bc_table(mesh: Mesh, solver_bcs: dict):
@@ -330,8 +371,10 @@ class CSR:
         return sp.csr_matrix((self.val, self.col, self.ptr), shape=(self.n, self.n))
 
 
-def assemble_diffusion(mesh: Mesh, xs: XS, bct, bcv):
-    """R and F of DiffusionSolver::buildMatrices (DiffusionSolver.cxx:88-320), n = 0."""
+def assemble_diffusion(mesh: Mesh, xs: XS, bct, bcv, T=None):
+    """R and F of DiffusionSolver::buildMatrices (DiffusionSolver.cxx:88-320), n = 0;
+    temperature-dependent materials evaluated at the cell temperatures T
+    (0 when None)."""
     L = lib()
     G = xs.G
     N = mesh.C * G
@@ -340,13 +383,18 @@ def assemble_diffusion(mesh: Mesh, xs: XS, bct, bcv):
     args = [mesh.C, mesh.fmax, _p(mesh.nfaces), _p(mesh.vol), _p(mesh.cen), _p(mesh.cmat),
             _p(mesh.farea), _p(mesh.fcen), _p(mesh.fnorm), _p(mesh.fnb), G, _p(xs.sigt),
             _p(xs.sigs), _p(xs.nusf), _p(xs.chi), _p(xs.D), _p(bct), _p(bcv)]
-    if L.orc_diff_assemble(*args, _p(r_ptr), None, None, _p(f_ptr), None, None):
+    fn = L.orc_diff_assemble
+    if xs.nset is not None:
+        Tc = np.zeros(mesh.C) if T is None else np.ascontiguousarray(T, np.float64)
+        args += [_p(xs.set0), _p(xs.nset), _p(xs.tref), _p(Tc)]
+        fn = L.orc_diff_assemble_sets
+    if fn(*args, _p(r_ptr), None, None, _p(f_ptr), None, None):
         raise pmp.PmpError("boundary condition not implemented")
     r_col = np.zeros(r_ptr[-1], np.int32)
     r_val = np.zeros(r_ptr[-1])
     f_col = np.zeros(f_ptr[-1], np.int32)
     f_val = np.zeros(f_ptr[-1])
-    L.orc_diff_assemble(*args, _p(r_ptr), _p(r_col), _p(r_val), _p(f_ptr), _p(f_col), _p(f_val))
+    fn(*args, _p(r_ptr), _p(r_col), _p(r_val), _p(f_ptr), _p(f_col), _p(f_val))
     return CSR(N, r_ptr, r_col, r_val), CSR(N, f_ptr, f_col, f_val)
 
 
@@ -437,13 +485,13 @@ def eigen_sinvert(R: CSR, F: CSR, tol=1e-13, v0=None, permc="MMD_AT_PLUS_A"):
     return k, x
 
 
-def normalize_flux(mesh: Mesh, xs: XS, phi: np.ndarray, power: float):
+def normalize_flux(mesh: Mesh, xs: XS, phi: np.ndarray, power: float, T=None):
     """NeutronicSolver::normalizeScalarFlux + calculatePowerAndProductionRate
     (NeutronicSolver.cxx:46-116): phi scaled so sum(phi kappaSf V) = power;
     returns (phi, q, P, total_power)."""
     G = xs.G
     ph = phi.reshape(mesh.C, G)
-    kap = xs.kapf[mesh.cmat]
+    kap = cell_values(xs, mesh, "kapf", T)
     p0 = float(np.sum(ph * kap * mesh.vol[:, None]))
     ph = ph * (power / p0)
     if np.any(ph < 0.0):
@@ -462,14 +510,14 @@ class Result:
     F: CSR | None = None
 
 
-def solve_diffusion_case(case: pmp.Case, return_ops=False) -> Result:
+def solve_diffusion_case(case: pmp.Case, return_ops=False, T=None) -> Result:
     mesh = build_mesh(case)
     s = case.solver
     xs = xs_tables(case.materials, s.G, diffusion=True)
     bct, bcv = bc_table(mesh, s.bcs)
-    R, F = assemble_diffusion(mesh, xs, bct, bcv)
+    R, F = assemble_diffusion(mesh, xs, bct, bcv, T)
     k, x = eigen_sinvert(R, F)
-    phi, q, p = normalize_flux(mesh, xs, x, s.power)
+    phi, q, p = normalize_flux(mesh, xs, x, s.power, T)
     return Result(k, phi, q, p, R if return_ops else None, F if return_ops else None)
 
 

@@ -366,8 +366,56 @@ static int aij_merge(int n, int* col, double* val, int add) {
 /* Boundary types, as in the reference's BC::Type (utils.hxx:103-105). */
 enum { BC_NONE = 0, BC_VACUUM = 1, BC_REFLECTIVE = 2, BC_ROBIN = 3 };
 
+/* Temperature-dependent data (FeedbackNuclearData.hxx:62-150): material m
+ * owns nset[m] tables from set set0[m], tabulated at increasing temperatures
+ * tref[]; a coefficient at temperature T is the first set at or below the
+ * first temperature, the last set above the last one, and between two,
+ * (1 - f) lower + f upper with f = (T - T1) / (T2 - T1).  (The reference takes
+ * the pair around lower_bound(T), which is the same except at T equal to the
+ * first temperature, where it reads before the first set.)  A material with
+ * one set returns that set's value. */
+typedef struct {
+   const int* set0;
+   const int* nset;
+   const double* tref;
+   const double* T; /* per cell */
+} orc_sets;
+
+typedef struct {
+   int i1, i2, one;
+   double f;
+} orc_at;
+
+static orc_at sets_at(const orc_sets* S, int m, double T) {
+   orc_at a = {m, m, 1, 0.0};
+   if (!S) return a;
+   const int n = S->nset[m], s0 = S->set0[m];
+   const double* Tr = S->tref + s0;
+   a.i1 = a.i2 = s0;
+   if (n == 1) return a;
+   a.one = 0;
+   if (T <= Tr[0]) return a;
+   if (T > Tr[n - 1]) {
+      a.i1 = a.i2 = s0 + n - 1;
+      return a;
+   }
+   int k = 1;
+   while (Tr[k] < T) k++;
+   a.i1 = s0 + k - 1;
+   a.i2 = s0 + k;
+   a.f = (T - Tr[k - 1]) / (Tr[k] - Tr[k - 1]);
+   return a;
+}
+
+static double xs_val(orc_at a, const double* tab, int ld, int g) {
+   if (a.one) return tab[a.i1 * ld + g];
+   return (1.0 - a.f) * tab[a.i1 * ld + g] + a.f * tab[a.i2 * ld + g];
+}
+
 /* Material tables: sigt[m*G+g], sigs[(m*G+from)*G+to], nusf[m*G+g],
- * chi[m*G+g] (effective spectrum), D[m*G+g].  Boundary table indexed by
+ * chi[m*G+g] (effective spectrum), D[m*G+g] (per set with temperature sets,
+ * evaluated at each cell's temperature; the neighbour's diffusion coefficient
+ * at the cell's own temperature, as DiffusionSolver.cxx:275 does).  Boundary table indexed by
  * bc id = boundary_index + 1: bct[id], bcv[id] (Robin f at t).
  * Two-pass: with r_col == NULL only r_ptr/f_ptr are filled.
  * Time-dependent rows (n > 0, DiffusionSolver.cxx:134-148,156-175) when
@@ -380,8 +428,8 @@ static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, cons
                      const int* fnb, int G, const double* sigt, const double* sigs,
                      const double* nusf, const double* chi, const double* D, const int* bct,
                      const double* bcv, const double* vel, double dt, const double* chip,
-                     const double* beta, double keff, int* r_ptr, int* r_col, double* r_val,
-                     int* f_ptr, int* f_col, double* f_val) {
+                     const double* beta, double keff, const orc_sets* S, int* r_ptr, int* r_col,
+                     double* r_val, int* f_ptr, int* f_col, double* f_val) {
    int cap = G + fmax;
    int* cols = malloc(sizeof(int) * cap);
    double* vals = malloc(sizeof(double) * cap);
@@ -397,14 +445,16 @@ static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, cons
          int l = i * G + g;
          int n = 0;
          cols[n] = l;
-         vals[n] = sigt[m * G + g] * V;
+         const double Ti = S && S->T ? S->T[i] : 0.0;
+         const orc_at xa = sets_at(S, m, Ti);
+         vals[n] = xs_val(xa, sigt, G, g) * V;
          if (vel) {
-            double d = V / (vel[m * G + g] * dt);
+            double d = V / (xs_val(xa, vel, G, g) * dt);
             vals[0] += d;
          }
          n = 1;
          for (int g2 = 0; g2 < G; g2++) {
-            double s = -sigs[(m * G + g2) * G + g] * V;
+            double s = -xs_val(xa, sigs, G * G, g2 * G + g) * V;
             int at = 0;
             if (g2 == g)
                vals[0] += s;
@@ -415,7 +465,7 @@ static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, cons
                n++;
             }
             if (vel) {
-               double pf = -(1.0 - beta[m]) * chip[m * G + g] * nusf[m * G + g2] * V / keff;
+               double pf = -(1.0 - beta[m]) * xs_val(xa, chip, G, g) * xs_val(xa, nusf, G, g2) * V / keff;
                vals[at] += pf;
             }
          }
@@ -427,7 +477,7 @@ static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, cons
                int t = bct[id];
                if (t == BC_VACUUM) {
                   double w = slf(cen + 3 * i, fcen + 3 * s, fnorm + 3 * s);
-                  vals[0] += w * D[m * G + g] * farea[s];
+                  vals[0] += w * xs_val(xa, D, G, g) * farea[s];
                } else if (t == BC_REFLECTIVE) {
                } else if (t == BC_ROBIN) {
                   vals[0] -= bcv[id] * farea[s];
@@ -439,12 +489,12 @@ static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, cons
                double r;
                if (m2 == m) {
                   double w = slf(cen + 3 * i, cen + 3 * i2, fnorm + 3 * s);
-                  r = -w * D[m * G + g] * farea[s];
+                  r = -w * xs_val(xa, D, G, g) * farea[s];
                } else {
                   double w1 = slf(cen + 3 * i, fcen + 3 * s, fnorm + 3 * s);
-                  w1 *= D[m * G + g] * farea[s];
+                  w1 *= xs_val(xa, D, G, g) * farea[s];
                   double w2 = slf(cen + 3 * i2, fcen + 3 * s, fnorm + 3 * s);
-                  w2 *= -D[m2 * G + g] * farea[s];
+                  w2 *= -xs_val(sets_at(S, m2, Ti), D, G, g) * farea[s];
                   r = -(w1 * w2) / (w1 + w2);
                }
                vals[0] -= r;
@@ -463,7 +513,7 @@ static int diff_rows(int C, int fmax, const int* nfaces, const double* vol, cons
             if (f_col)
                for (int g2 = 0; g2 < G; g2++) {
                   f_col[f_ptr[l] + g2] = i * G + g2;
-                  f_val[f_ptr[l] + g2] = chi[m * G + g] * nusf[m * G + g2] * V;
+                  f_val[f_ptr[l] + g2] = xs_val(xa, chi, G, g) * xs_val(xa, nusf, G, g2) * V;
                }
          }
       }
@@ -481,7 +531,19 @@ int orc_diff_assemble(int C, int fmax, const int* nfaces, const double* vol, con
                       const double* bcv, int* r_ptr, int* r_col, double* r_val, int* f_ptr,
                       int* f_col, double* f_val) {
    return diff_rows(C, fmax, nfaces, vol, cen, cmat, farea, fcen, fnorm, fnb, G, sigt, sigs, nusf, chi,
-                    D, bct, bcv, NULL, 0.0, NULL, NULL, 0.0, r_ptr, r_col, r_val, f_ptr, f_col, f_val);
+                    D, bct, bcv, NULL, 0.0, NULL, NULL, 0.0, NULL, r_ptr, r_col, r_val, f_ptr, f_col, f_val);
+}
+
+/* Steady rows with temperature sets (tables per set, T per cell). */
+int orc_diff_assemble_sets(int C, int fmax, const int* nfaces, const double* vol, const double* cen,
+                           const int* cmat, const double* farea, const double* fcen, const double* fnorm,
+                           const int* fnb, int G, const double* sigt, const double* sigs, const double* nusf,
+                           const double* chi, const double* D, const int* bct, const double* bcv,
+                           const int* set0, const int* nset, const double* tref, const double* T,
+                           int* r_ptr, int* r_col, double* r_val, int* f_ptr, int* f_col, double* f_val) {
+   orc_sets S = {set0, nset, tref, T};
+   return diff_rows(C, fmax, nfaces, vol, cen, cmat, farea, fcen, fnorm, fnb, G, sigt, sigs, nusf, chi,
+                    D, bct, bcv, NULL, 0.0, NULL, NULL, 0.0, &S, r_ptr, r_col, r_val, f_ptr, f_col, f_val);
 }
 
 /* Time-step rows (n > 0): R only, with vel[m*G+g], chip[m*G+g], beta[m]
@@ -495,7 +557,7 @@ int orc_diff_assemble_transient(int C, int fmax, const int* nfaces, const double
                                 const double* beta, double keff, int* r_ptr, int* r_col,
                                 double* r_val) {
    return diff_rows(C, fmax, nfaces, vol, cen, cmat, farea, fcen, fnorm, fnb, G, sigt, sigs, nusf, NULL,
-                    D, bct, bcv, vel, dt, chip, beta, keff, r_ptr, r_col, r_val, NULL, NULL, NULL);
+                    D, bct, bcv, vel, dt, chip, beta, keff, NULL, r_ptr, r_col, r_val, NULL, NULL, NULL);
 }
 
 /* Right-hand side of a time step (DiffusionSolver.cxx:134-143):

@@ -202,6 +202,10 @@ class Material:
     fuel: bool = False
     bc: bool = False
     split: bool = False
+    # temperature-dependent data (nuclear-data-set, FeedbackNuclearData.cxx:4-53):
+    # one data set per temperature; nd is the first
+    temperatures: list = field(default_factory=list)
+    sets: list = field(default_factory=list)
 
 
 def _read_material_block(rd: Lines, mat: Material):
@@ -213,6 +217,20 @@ def _read_material_block(rd: Lines, mat: Material):
         k = line[0]
         if k == "nuclear-data":
             mat.nd = _read_nuclear_data(rd)
+        elif k == "nuclear-data-set":
+            while True:
+                ln = rd.next()
+                if not ln or ln[0] == "}":
+                    break
+                if ln[0] == "temperature":
+                    mat.temperatures = rd.floats(int(ln[1]))
+                elif ln[0] == "nuclear-data":
+                    mat.sets.append(_read_nuclear_data(rd))
+                else:
+                    raise PmpError(f"unrecognized keyword '{ln[0]}'")
+            if not mat.sets or len(mat.sets) != len(mat.temperatures):
+                raise PmpError("one nuclear-data block per temperature is needed")
+            mat.nd = mat.sets[0]
         elif k == "precursor-data":
             beta_total = 0.0
             ng = 0
@@ -240,8 +258,8 @@ def _read_material_block(rd: Lines, mat: Material):
             mat.split = bool(int(line[1]))
         else:
             raise PmpError(f"unrecognized keyword '{k}'")
-    if mat.nd is not None:
-        mat.nd.check(mat.beta_total)
+    for nd in (mat.sets if mat.sets else [mat.nd] if mat.nd is not None else []):
+        nd.check(mat.beta_total)
 
 
 @dataclass

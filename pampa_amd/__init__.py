@@ -96,6 +96,7 @@ def _declare(L):
         "pampa_ext_get_result": [_P, _P, _IP], "pampa_ext_get_operator": [ct.c_char_p, _P, _P, _P, _P, _P, _IP],
         "pampa_ext_get_stats": [_P, _P, _P, _P, _IP], "pampa_ext_get_sizes": [_P, _P, _P, _IP],
         "pampa_ext_time_kernel": [_I, _I, _P, _P, _IP], "pampa_ext_cold_start": [_IP],
+        "pampa_ext_field_device": [ct.c_char_p, _PP, ct.POINTER(ct.c_long), _IP],
     }
     for name, args in {"pk_profile_enable": [_I], "pk_profile_reset": [],
                        "pk_profile_get": [ct.c_char_p, _P, _P, _P]}.items():
@@ -589,6 +590,14 @@ class DropIn:
         o, i, r, s, err = ct.c_int(), ct.c_long(), ct.c_double(), ct.c_double(), ct.c_int()
         self.L.pampa_ext_get_stats(ct.byref(o), ct.byref(i), ct.byref(r), ct.byref(s), ct.byref(err))
         return dict(outer=o.value, inner=i.value, residual=r.value, seconds=s.value)
+
+    def field_device(self, name: str):
+        """(device address, length) of a field (pampa_ext_field_device)."""
+        dev, n, err = ct.c_void_p(), ct.c_long(), ct.c_int()
+        self.L.pampa_ext_field_device(name.encode(), ct.byref(dev), ct.byref(n), ct.byref(err))
+        if err.value:
+            raise PkError(f"pampa_ext_field_device({name}) failed")
+        return dev.value, n.value
 
     def cold_start(self):
         """The next solve starts cold (pampa_ext_cold_start)."""

@@ -78,7 +78,29 @@ int Material::read(std::istream& in) {
          PAMPA_CHECK(line.size() != 2 || line[1] != "{", "missing opening '{' for constant nuclear data");
          PAMPA_CHECK(read_nuclear_data(*this, in), "unable to read the constant nuclear data");
       } else if (k == "nuclear-data-set") {
-         PAMPA_CHECK(true, "temperature-dependent nuclear data is not supported");
+         PAMPA_CHECK(line.size() != 2 || line[1] != "{", "missing opening '{' for nuclear data");
+         while (true) {
+            auto l2 = input::next_line(in);
+            if (l2.empty() || l2[0] == "}") break;
+            if (l2[0] == "temperature") {
+               PAMPA_CHECK(l2.size() != 2, "wrong number of arguments for keyword 'temperature'");
+               int n = 0;
+               PAMPA_CHECK(input::read(n, 0, INT_MAX, l2[1]), "wrong number of temperatures");
+               PAMPA_CHECK(input::read(temperatures, n, 0.0, DBL_MAX, in), "wrong temperature data");
+            } else if (l2[0] == "nuclear-data") {
+               PAMPA_CHECK(l2.size() != 2 || l2[1] != "{", "missing opening '{' for nuclear data");
+               Material set;
+               PAMPA_CHECK(read_nuclear_data(set, in), "unable to read the nuclear data");
+               sets.push_back(set);
+            } else {
+               PAMPA_CHECK(true, "unrecognized keyword '" + l2[0] + "'");
+            }
+         }
+         PAMPA_CHECK(sets.empty() || sets.size() != temperatures.size(),
+                     "one nuclear-data block per temperature is needed");
+         for (size_t k = 1; k < temperatures.size(); k++)
+            PAMPA_CHECK(!(temperatures[k] > temperatures[k - 1]), "temperatures must increase");
+         has_nuclear_data = true;
       } else if (k == "precursor-data") {
          PAMPA_CHECK(line.size() != 2 || line[1] != "{", "missing opening '{' for precursor data");
          PAMPA_CHECK(read_precursor_data(*this, in), "unable to read the precursor data");
@@ -98,7 +120,34 @@ int Material::read(std::istream& in) {
          PAMPA_CHECK(true, "unrecognized keyword '" + k + "'");
       }
    }
-   if (has_nuclear_data) PAMPA_CHECK(derive(), "wrong nuclear data");
+   if (!sets.empty()) {
+      // every set derived with this material's delayed fraction; the material's
+      // own fields take the first set
+      for (auto& st : sets) {
+         st.beta_total = beta_total;
+         PAMPA_CHECK(st.derive(), "wrong nuclear data");
+      }
+      const std::vector<double> temps = temperatures;
+      std::vector<Material> all = sets;
+      const std::string nm = name;
+      const bool f = fuel, b = bc, sp = split;
+      const double bt = beta_total;
+      const int np = num_precursor_groups;
+      const std::vector<double> lam = lambda, bet = beta;
+      *this = all[0];
+      name = nm;
+      fuel = f;
+      bc = b;
+      split = sp;
+      beta_total = bt;
+      num_precursor_groups = np;
+      lambda = lam;
+      beta = bet;
+      temperatures = temps;
+      sets = all;
+   } else if (has_nuclear_data) {
+      PAMPA_CHECK(derive(), "wrong nuclear data");
+   }
    return 0;
 }
 
@@ -141,6 +190,7 @@ int Material::derive() {
 
 int Material::check(int num_groups, bool diffusion, bool transient) const {
    PAMPA_CHECK(!has_nuclear_data, "missing nuclear data");
+   for (const auto& st : sets) PAMPA_CHECK(st.check(num_groups, diffusion, transient), "wrong nuclear data");
    PAMPA_CHECK(G != num_groups, "wrong number of energy groups");
    PAMPA_CHECK(sigma_total.empty(), "missing total cross sections");
    PAMPA_CHECK(sigma_scattering.empty(), "missing scattering cross sections");

@@ -18,6 +18,11 @@ struct Material {
    std::vector<double> lambda, beta;
    double beta_total = 0.0;
    bool fuel = false, bc = false, split = false;
+   // temperature-dependent data (nuclear-data-set, FeedbackNuclearData.cxx:4-53):
+   // one constant data set per tabulated temperature; the material's own
+   // fields hold the first set
+   std::vector<double> temperatures;
+   std::vector<Material> sets;
 
    int read(std::istream& in);
    int read(const std::string& filename, const std::string& base_dir);

@@ -99,6 +99,11 @@ void pampa_ext_cold_start(int* error) {
    *error = s ? s->coldStart() : 1;
 }
 
+void pampa_ext_field_device(const char name[], void** dev, long* n, int* error) {
+   pampa::Solver* s = g_driver.mainSolver();
+   *error = s ? s->fieldDevice(std::string(name), dev, n) : 1;
+}
+
 void pampa_ext_get_sizes(int* num_cells, int* num_groups, int* num_directions, int* error) {
    pampa::Solver* s = g_driver.mainSolver();
    *error = s ? 0 : 1;

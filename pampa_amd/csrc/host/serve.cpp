@@ -277,6 +277,10 @@ void pampa_serve_add_case(pampa_server S, const char* input, int* case_id, long*
       return;
    }
    if (c->solver->materialTables(true, c->base)) return;
+   if (c->base.feedback()) {
+      report_error(__FILE__, __LINE__, __func__, "served cases need constant cross sections per material");
+      return;
+   }
    const int G = c->solver->groups();
    c->n = c->solver->numCells() * (long)G;
    if (pk_xs_create(c->base.nm, G, c->base.st.data(), c->base.ss.data(), c->base.nf.data(), c->base.kf.data(),

@@ -36,6 +36,17 @@ int Solver::setField(const double* v, const std::string& fname) {
    PAMPA_CHECK(true, "unable to find field '" + fname + "'");
 }
 
+int Solver::fieldDevice(const std::string& fname, void** dev, long* n) const {
+   for (const auto& f : fields)
+      if (f.name == fname) {
+         PAMPA_CHECK(!f.vec || !*f.vec, "field '" + fname + "' is not allocated");
+         PK_CHECK(pk_vec_device_ptr(*f.vec, dev));
+         *n = pk_vec_size(*f.vec);
+         return 0;
+      }
+   PAMPA_CHECK(true, "unable to find field '" + fname + "'");
+}
+
 int Solver::getOperator(const std::string&, long*, long*, int*, int*, double*) const {
    PAMPA_CHECK(true, "no operator available for solver '" + name + "'");
 }
@@ -99,14 +110,30 @@ int NeutronicSolver::materialTables(bool diffusion, XSTables& t) const {
    t = XSTables();
    t.nm = nm;
    t.G = G;
-   for (const auto& m : materials) {
-      PAMPA_CHECK(m.check(G, diffusion, transient), "wrong nuclear data in material '" + m.name + "'");
+   auto add = [&](const Material& m) {
       t.st.insert(t.st.end(), m.sigma_total.begin(), m.sigma_total.end());
       t.ss.insert(t.ss.end(), m.sigma_scattering.begin(), m.sigma_scattering.end());
       t.nf.insert(t.nf.end(), m.nu_sigma_fission.begin(), m.nu_sigma_fission.end());
       t.kf.insert(t.kf.end(), m.kappa_sigma_fission.begin(), m.kappa_sigma_fission.end());
       t.ch.insert(t.ch.end(), m.chi_effective.begin(), m.chi_effective.end());
       if (diffusion) t.D.insert(t.D.end(), m.diffusion_coefficient.begin(), m.diffusion_coefficient.end());
+      if (transient) {
+         t.vel.insert(t.vel.end(), m.velocity.begin(), m.velocity.end());
+         t.chip.insert(t.chip.end(), m.chi_prompt.begin(), m.chi_prompt.end());
+         t.chid.insert(t.chid.end(), m.chi_delayed.begin(), m.chi_delayed.end());
+      }
+   };
+   for (const auto& m : materials) {
+      PAMPA_CHECK(m.check(G, diffusion, transient), "wrong nuclear data in material '" + m.name + "'");
+      if (m.sets.empty()) {
+         t.nsets.push_back(1);
+         t.temps.push_back(0.0);
+         add(m);
+      } else {
+         t.nsets.push_back((int)m.sets.size());
+         t.temps.insert(t.temps.end(), m.temperatures.begin(), m.temperatures.end());
+         for (const auto& st : m.sets) add(st);
+      }
    }
    return 0;
 }
@@ -116,8 +143,9 @@ int NeutronicSolver::uploadMaterials(bool diffusion) {
    PAMPA_CHECK(materialTables(diffusion, t), "wrong material data");
    for (int i = 0; i < mesh->num_cells; i++)
       PAMPA_CHECK(mesh->mat[i] < 0 || mesh->mat[i] >= t.nm, "wrong material index in the mesh");
-   PK_CHECK(pk_xs_create(t.nm, G, t.st.data(), t.ss.data(), t.nf.data(), t.kf.data(), t.ch.data(),
-                         diffusion ? t.D.data() : nullptr, &dxs));
+   PK_CHECK(pk_xs_create_sets(t.nm, G, t.nsets.data(), t.temps.data(), t.st.data(), t.ss.data(), t.nf.data(),
+                              t.kf.data(), t.ch.data(), diffusion ? t.D.data() : nullptr, &dxs));
+   feedback = t.feedback();
    return 0;
 }
 
@@ -230,6 +258,8 @@ int DiffusionSolver::build() {
    PK_CHECK(pk_vec_create(C, &q));
    PK_CHECK(pk_vec_create(C, &P));
    PK_CHECK(pk_vec_create(C * G, &b));
+   // temperature-dependent cross sections are evaluated at this field
+   PK_CHECK(pk_xs_set_temperature(dxs, T));
    fields = {{"temperature", &T, true, false},   {"delayed-source", &S, true, false},
              {"scalar-flux", &phi, false, false}, {"power", &q, false, true},
              {"production-rate", &P, false, true}};
@@ -275,13 +305,13 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       PAMPA_CHECK(powerAndProduction(true), "unable to normalize the scalar flux");
    } else {
       // transient rows and KSP solve (DiffusionSolver.cxx:94-148,168-175,344)
-      std::vector<double> vel, chip, chid, beta;
-      for (const auto& m : materials) {
-         vel.insert(vel.end(), m.velocity.begin(), m.velocity.end());
-         chip.insert(chip.end(), m.chi_prompt.begin(), m.chi_prompt.end());
-         chid.insert(chid.end(), m.chi_delayed.begin(), m.chi_delayed.end());
-         beta.push_back(m.beta_total);
-      }
+      XSTables t;
+      PAMPA_CHECK(materialTables(true, t), "wrong material data");
+      std::vector<double>& vel = t.vel;
+      std::vector<double>& chip = t.chip;
+      std::vector<double>& chid = t.chid;
+      std::vector<double> beta;
+      for (const auto& m : materials) beta.push_back(m.beta_total);
       PK_CHECK(pk_mat_destroy(&R));
       PK_CHECK(pk_mat_destroy(&F));
       PK_CHECK(pk_diffusion_assemble_transient(dmesh, dxs, nb, bc_type.data(), bc_value.data(),
@@ -458,6 +488,7 @@ int SNSolver::build() {
    PK_CHECK(pk_vec_create(C, &S));
    PK_CHECK(pk_vec_create(C, &q));
    PK_CHECK(pk_vec_create(C, &P));
+   PK_CHECK(pk_xs_set_temperature(dxs, T));
    fields = {{"temperature", &T, true, false},    {"delayed-source", &S, true, false},
              {"scalar-flux", &phi, false, false},  {"angular-flux", &psi, false, false},
              {"power", &q, false, true},           {"production-rate", &P, false, true}};
@@ -468,6 +499,7 @@ int SNSolver::build() {
    bool all_vacuum = true;
    for (size_t f = 0; f < mesh->fnb.size(); f++)
       if (mesh->fnb[f] < 0 && bc_type[-mesh->fnb[f]] != PK_BC_VACUUM) all_vacuum = false;
+   PAMPA_CHECK(feedback, "temperature-dependent nuclear data is supported by the diffusion solver only");
    sweepable = (delta == 1.0) && !ls && all_vacuum;
    if (sweepable) {
       PK_CHECK(pk_sn_create(dmesh, dxs, M, dirs.data(), wts.data(), delta, (int)bc_type.size(),

@@ -30,10 +30,16 @@ struct SolverOptions {
 };
 extern SolverOptions options;
 
-// Per-material tables in the layout of pk_xs_create: [nm][G] and sigs [nm][G][G].
+// Per-material tables in the layout of pk_xs_create_sets: material m owns
+// nsets[m] sets at temperatures temps (one set for constant data); tables
+// [sets][G] and sigs [sets][G][G]; the transient tables likewise.
 struct XSTables {
    int nm = 0, G = 0;
+   std::vector<int> nsets;
+   std::vector<double> temps;
    std::vector<double> st, ss, nf, kf, ch, D;
+   std::vector<double> vel, chip, chid;
+   bool feedback() const { return (int)temps.size() > nm; }
 };
 
 struct Field {
@@ -56,6 +62,8 @@ class Solver {
    virtual int finalize() = 0;
    int getField(double* v, const std::string& name) const;
    int setField(const double* v, const std::string& name);
+   // device address and length of a field (for coupled codes on the same GPU)
+   int fieldDevice(const std::string& name, void** dev, long* n) const;
    // solution summary used by the C-ABI extensions and tests
    virtual double getKeff() const { return -1.0; }
    virtual double getPower() const { return 0.0; }
@@ -101,6 +109,7 @@ class NeutronicSolver : public Solver {
    // the solution vector in PETSc's binary format (petsc.cxx:491-511), with -petsc_dump
    virtual int writePETSc(int n) const = 0;
    int uploadMaterials(bool diffusion);
+   bool feedback = false;  // some material has temperature-dependent data
    int powerAndProduction(bool normalize);  // NeutronicSolver.cxx:46-116
 
    Mesh* mesh;

@@ -50,14 +50,50 @@ struct XSView {
    const double* kapf;
    const double* chi;
    const double* D;
+   const int* set0;
+   const int* nset;
+   const double* tref;
+   const double* T;  // cell temperatures (null: 0)
 };
+
+// Interpolation between the sets of material m at temperature T
+// (FeedbackNuclearData.hxx:62-90): below the first or above the last
+// tabulated temperature the end set; between two, linear with weight f on the
+// upper one.  One set: its values as they are.
+struct XSAt {
+   int i1, i2;
+   double f;
+   bool one;
+};
+
+__device__ __forceinline__ XSAt xs_at(const XSView& X, int m, double T) {
+#pragma clang fp contract(off)
+   const int n = X.nset[m], s0 = X.set0[m];
+   if (n == 1) return XSAt{s0, s0, 0.0, true};
+   const double* Tr = X.tref + s0;
+   if (T <= Tr[0]) return XSAt{s0, s0, 0.0, false};
+   if (T > Tr[n - 1]) return XSAt{s0 + n - 1, s0 + n - 1, 0.0, false};
+   int k = 1;  // first tabulated temperature >= T
+   while (Tr[k] < T) k++;
+   return XSAt{s0 + k - 1, s0 + k, (T - Tr[k - 1]) / (Tr[k] - Tr[k - 1]), false};
+}
+
+// value g of a [sets][G] table (or entry (from, to) of sigs with ld = G, g = from * G + to)
+__device__ __forceinline__ double xv(const XSAt& a, const double* tab, int ld, int g) {
+#pragma clang fp contract(off)
+   if (a.one) return tab[(long)a.i1 * ld + g];
+   return (1.0 - a.f) * tab[(long)a.i1 * ld + g] + a.f * tab[(long)a.i2 * ld + g];
+}
+
+__device__ __forceinline__ double cell_T(const XSView& X, int i) { return X.T ? X.T[i] : 0.0; }
 
 MeshView view(pk_mesh_s* m) {
    return MeshView{m->C, m->fptr.p, m->vol.p, m->cen.p, m->mat.p, m->farea.p, m->fcen.p, m->fnorm.p,
                    m->fnb.p};
 }
 XSView view(pk_xs_s* x) {
-   return XSView{x->G, x->sigt.p, x->sigs.p, x->nusf.p, x->kapf.p, x->chi.p, x->D.p};
+   return XSView{x->G,      x->sigt.p, x->sigs.p, x->nusf.p, x->kapf.p, x->chi.p,
+                 x->D.p,    x->set0.p, x->nset.p, x->tref.p, x->T};
 }
 
 // Distinct physical neighbours of a cell in face order (first occurrence).
@@ -92,9 +128,13 @@ __global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const dou
    int m = M.mat[i];
    double V = M.vol[i];
    const double* ci = M.cen + 3 * (long)i;
+   // every coefficient at this cell's temperature, the neighbour's diffusion
+   // coefficient included (DiffusionSolver.cxx:275 evaluates it at T_i)
+   const double Ti = cell_T(X, i);
+   const XSAt xa = xs_at(X, m, Ti);
    // diagonal: total reaction, then self-scattering (DiffusionSolver.cxx:130-131,157-158)
-   double diag = X.sigt[m * G + g] * V;
-   if (vel) diag += V / (vel[m * G + g] * dt);
+   double diag = xv(xa, X.sigt, G, g) * V;
+   if (vel) diag += V / (xv(xa, vel, G, g) * dt);
    // neighbour entries in face order; a repeated neighbour keeps the last value
    int ncol[kMaxFaces];
    double nval[kMaxFaces];
@@ -103,8 +143,8 @@ __global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const dou
    // the diagonal receives the self terms in g2 order
    for (int g2 = 0; g2 < G; g2++) {
       if (g2 == g) {
-         diag += -X.sigs[(m * G + g2) * G + g] * V;
-         if (vel) diag += -(1.0 - beta[m]) * chip[m * G + g] * X.nusf[m * G + g2] * V / keff;
+         diag += -xv(xa, X.sigs, G * G, g2 * G + g) * V;
+         if (vel) diag += -(1.0 - beta[m]) * xv(xa, chip, G, g) * xv(xa, X.nusf, G, g2) * V / keff;
       }
    }
    for (int f = M.fptr[i]; f < M.fptr[i + 1]; f++) {
@@ -115,7 +155,7 @@ __global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const dou
          int t = bct[id];
          if (t == PK_BC_VACUUM) {
             double w = slf(ci, M.fcen + 3 * (long)f, M.fnorm + 3 * (long)f);
-            diag += w * X.D[m * G + g] * A;
+            diag += w * xv(xa, X.D, G, g) * A;
          } else if (t == PK_BC_REFLECTIVE) {
          } else if (t == PK_BC_ROBIN) {
             diag -= bcv[id] * A;
@@ -127,12 +167,12 @@ __global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const dou
          double r;
          if (m2 == m) {
             double w = slf(ci, M.cen + 3 * (long)i2, M.fnorm + 3 * (long)f);
-            r = -w * X.D[m * G + g] * A;
+            r = -w * xv(xa, X.D, G, g) * A;
          } else {
             double w1 = slf(ci, M.fcen + 3 * (long)f, M.fnorm + 3 * (long)f);
-            w1 *= X.D[m * G + g] * A;
+            w1 *= xv(xa, X.D, G, g) * A;
             double w2 = slf(M.cen + 3 * (long)i2, M.fcen + 3 * (long)f, M.fnorm + 3 * (long)f);
-            w2 *= -X.D[m2 * G + g] * A;
+            w2 *= -xv(xs_at(X, m2, Ti), X.D, G, g) * A;
             r = -(w1 * w2) / (w1 + w2);
          }
          diag -= r;
@@ -176,8 +216,8 @@ __global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const dou
       if (g2 == g) {
          rval[p] = diag;
       } else {
-         double v = -X.sigs[(m * G + g2) * G + g] * V;
-         if (vel) v += -(1.0 - beta[m]) * chip[m * G + g] * X.nusf[m * G + g2] * V / keff;
+         double v = -xv(xa, X.sigs, G * G, g2 * G + g) * V;
+         if (vel) v += -(1.0 - beta[m]) * xv(xa, chip, G, g) * xv(xa, X.nusf, G, g2) * V / keff;
          rval[p] = v;
       }
    }
@@ -189,21 +229,22 @@ __global__ void diff_fill_kernel(MeshView M, XSView X, const int* bct, const dou
       long q = l * G;
       for (int g2 = 0; g2 < G; g2++) {
          fcol[q + g2] = i * G + g2;
-         fval[q + g2] = X.chi[m * G + g] * X.nusf[m * G + g2] * V;
+         fval[q + g2] = xv(xa, X.chi, G, g) * xv(xa, X.nusf, G, g2) * V;
       }
    }
 }
 
 // b = chi_d S + V/(v dt) phi0 (DiffusionSolver.cxx:134-148)
-__global__ void diff_rhs_kernel(MeshView M, int G, const double* chid, const double* vel, double dt,
+__global__ void diff_rhs_kernel(MeshView M, XSView X, const double* chid, const double* vel, double dt,
                                 const double* S, const double* phi0, double* b) {
 #pragma clang fp contract(off)
+   const int G = X.G;
    long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (l >= (long)M.C * G) return;
    int i = (int)(l / G), g = (int)(l % G);
-   int m = M.mat[i];
-   double v = chid[m * G + g] * S[i];
-   double d = M.vol[i] / (vel[m * G + g] * dt);
+   const XSAt a = xs_at(X, M.mat[i], cell_T(X, i));
+   double v = xv(a, chid, G, g) * S[i];
+   double d = M.vol[i] / (xv(a, vel, G, g) * dt);
    v += d * phi0[l];
    b[l] = v;
 }
@@ -244,7 +285,7 @@ __global__ void __launch_bounds__(kBlock) power_kernel(MeshView M, XSView X, con
    for (long l = blockIdx.x * (long)kBlock + threadIdx.x; l < (long)M.C * G;
         l += (long)gridDim.x * kBlock) {
       int i = (int)(l / G), g = (int)(l % G);
-      s += phi[l] * X.kapf[M.mat[i] * G + g] * M.vol[i];
+      s += phi[l] * xv(xs_at(X, M.mat[i], cell_T(X, i)), X.kapf, G, g) * M.vol[i];
    }
    double v[1] = {s};
    block_reduce_finish<1>(v, partials, ticket, out);
@@ -260,14 +301,14 @@ __global__ void __launch_bounds__(kBlock) scale_power_kernel(MeshView M, XSView 
    double f = power >= 0.0 ? power / p0[0] : 1.0;  // power < 0: rates only, no scaling
    double neg = 0.0, tot = 0.0;
    for (long i = blockIdx.x * (long)kBlock + threadIdx.x; i < M.C; i += (long)gridDim.x * kBlock) {
-      int m = M.mat[i];
+      const XSAt a = xs_at(X, M.mat[i], cell_T(X, i));
       double qi = 0.0, Pi = 0.0;
       for (int g = 0; g < G; g++) {
          double v = phi[i * G + g] * f;
          phi[i * G + g] = v;
          if (v < 0.0) neg += 1.0;
-         qi += v * X.kapf[m * G + g] * M.vol[i];
-         Pi += v * X.nusf[m * G + g] * M.vol[i] * inv_keff;
+         qi += v * xv(a, X.kapf, G, g) * M.vol[i];
+         Pi += v * xv(a, X.nusf, G, g) * M.vol[i] * inv_keff;
       }
       if (q) q[i] = qi;
       if (P) P[i] = Pi;
@@ -339,8 +380,9 @@ __global__ void sn_fill_kernel(MeshView M, XSView X, SnRefView Q, const int* rpt
    const double V = M.vol[i];
    const double* om = Q.dirs + 3 * m;
    const double* ci = M.cen + 3 * (long)i;
-   double diag = X.sigt[mt * G + g] * V;
-   diag += -X.sigs[(mt * G + g) * G + g] * Q.wts[m] * V;
+   const XSAt xa = xs_at(X, mt, cell_T(X, i));
+   double diag = xv(xa, X.sigt, G, g) * V;
+   diag += -xv(xa, X.sigs, G * G, g * G + g) * Q.wts[m] * V;
    int ncol[kMaxFaces];
    double nval[kMaxFaces];
    int nn = 0;
@@ -456,7 +498,7 @@ __global__ void sn_fill_kernel(MeshView M, XSView X, SnRefView Q, const int* rpt
    }
    const long c0 = (long)i * GM;
    for (int g2 = 0; g2 < G; g2++) {
-      const double sv = -X.sigs[(mt * G + g2) * G + g];
+      const double sv = -xv(xa, X.sigs, G * G, g2 * G + g);
       for (int m2 = 0; m2 < Md; m2++, p++) {
          rcol[p] = (int)(c0 + g2 * Md + m2);
          double v = (g2 == g && m2 == m) ? diag : sv * Q.wts[m2] * V;
@@ -472,11 +514,11 @@ __global__ void sn_fill_kernel(MeshView M, XSView X, SnRefView Q, const int* rpt
    }
    if (fcol) {
       const long q = l * GM;
-      const double cn = X.chi[mt * G + g];
+      const double cn = xv(xa, X.chi, G, g);
       for (int g2 = 0; g2 < G; g2++)
          for (int m2 = 0; m2 < Md; m2++) {
             fcol[q + g2 * Md + m2] = (int)(c0 + g2 * Md + m2);
-            fval[q + g2 * Md + m2] = cn * X.nusf[mt * G + g2] * Q.wts[m2] * V;
+            fval[q + g2 * Md + m2] = cn * xv(xa, X.nusf, G, g2) * Q.wts[m2] * V;
          }
    }
 }
@@ -575,12 +617,40 @@ int pk_mesh_destroy(pk_mesh* mesh) {
 
 int pk_xs_create(int num_mat, int G, const double* sigt, const double* sigs, const double* nusf,
                  const double* kapf, const double* chi, const double* D, pk_xs* xs) {
+   return pk_xs_create_sets(num_mat, G, nullptr, nullptr, sigt, sigs, nusf, kapf, chi, D, xs);
+}
+
+int pk_xs_create_sets(int num_mat, int G, const int* num_sets, const double* temps, const double* sigt,
+                      const double* sigs, const double* nusf, const double* kapf, const double* chi,
+                      const double* D, pk_xs* xs) {
    PK_CALL(ensure_init());
    PK_REQUIRE(num_mat > 0 && G > 0, "wrong cross-section table size");
+   std::vector<int> set0(num_mat), nset(num_mat, 1);
+   int ns = 0;
+   for (int m = 0; m < num_mat; m++) {
+      if (num_sets) nset[m] = num_sets[m];
+      PK_REQUIRE(nset[m] >= 1, "a material needs at least one cross-section set");
+      set0[m] = ns;
+      ns += nset[m];
+   }
+   std::vector<double> tref(ns, 0.0);
+   if (temps) {
+      tref.assign(temps, temps + ns);
+      for (int m = 0; m < num_mat; m++)
+         for (int k = 1; k < nset[m]; k++)
+            PK_REQUIRE(tref[set0[m] + k] > tref[set0[m] + k - 1], "set temperatures must increase");
+   }
    auto* x = new pk_xs_s();
    x->nm = num_mat;
    x->G = G;
-   size_t n1 = (size_t)num_mat * G, n2 = n1 * G;
+   x->ns = ns;
+   x->feedback = ns > num_mat;
+   if (upload(x->set0, set0.data(), num_mat) || upload(x->nset, nset.data(), num_mat) ||
+       upload(x->tref, tref.data(), ns)) {
+      pk_xs_destroy(&x);
+      PK_FAIL("device allocation failed");
+   }
+   size_t n1 = (size_t)ns * G, n2 = n1 * G;
    x->h_sigt.assign(sigt, sigt + n1);
    x->h_sigs.assign(sigs, sigs + n2);
    x->h_nusf.assign(nusf, nusf + n1);
@@ -633,7 +703,7 @@ static int diffusion_assemble_impl(pk_mesh mesh, pk_xs xs, int num_bc, const int
    PK_REQUIRE(!upload(dbct, bc_type, num_bc) && !upload(dbcv, bc_value, num_bc) && !derr.alloc(1),
               "device allocation failed");
    if (vel) {
-      PK_REQUIRE(!upload(dvdt, vel, (size_t)xs->nm * G) && !upload(dchip, chip, (size_t)xs->nm * G) &&
+      PK_REQUIRE(!upload(dvdt, vel, (size_t)xs->ns * G) && !upload(dchip, chip, (size_t)xs->ns * G) &&
                      !upload(dbeta, beta, (size_t)xs->nm),
                  "device allocation failed");
    }
@@ -762,11 +832,17 @@ int pk_diffusion_refill(pk_mesh mesh, pk_xs xs, pk_mat R, pk_mat F) {
    return 0;
 }
 
+int pk_xs_set_temperature(pk_xs xs, pk_vec T) {
+   PK_REQUIRE(xs, "null argument");
+   xs->T = T ? T->d.p : nullptr;
+   return 0;
+}
+
 int pk_xs_set(pk_xs xs, const double* sigt, const double* sigs, const double* nusf, const double* kapf,
               const double* chi, const double* D) {
    PK_REQUIRE(xs && sigt && sigs && nusf && kapf && chi, "null argument");
    Ctx& c = ctx();
-   const size_t n1 = (size_t)xs->nm * xs->G, n2 = n1 * xs->G;
+   const size_t n1 = (size_t)xs->ns * xs->G, n2 = n1 * xs->G;
    xs->h_sigt.assign(sigt, sigt + n1);
    xs->h_sigs.assign(sigs, sigs + n2);
    xs->h_nusf.assign(nusf, nusf + n1);
@@ -794,10 +870,10 @@ int pk_diffusion_assemble_transient(pk_mesh mesh, pk_xs xs, int num_bc, const in
    Ctx& c = ctx();
    const int G = xs->G;
    DBuf<double> dchid, dvdt;
-   PK_REQUIRE(!upload(dchid, chid, (size_t)xs->nm * G) && !upload(dvdt, velocity, (size_t)xs->nm * G),
+   PK_REQUIRE(!upload(dchid, chid, (size_t)xs->ns * G) && !upload(dvdt, velocity, (size_t)xs->ns * G),
               "device allocation failed");
    diff_rhs_kernel<<<grid_for((long)mesh->C * G), kBlock, 0, c.stream>>>(
-       view(mesh), G, dchid.p, dvdt.p, dt, S->d.p, phi0->d.p, b->d.p);
+       view(mesh), view(xs), dchid.p, dvdt.p, dt, S->d.p, phi0->d.p, b->d.p);
    PK_HIP(hipGetLastError());
    PK_HIP(hipStreamSynchronize(c.stream));
    return 0;

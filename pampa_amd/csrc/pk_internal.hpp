@@ -210,9 +210,16 @@ struct pk_mesh_s {
    std::vector<double> h_vol, h_cen, h_farea, h_fcen, h_fnorm;
 };
 
+// Cross-section tables in sets: material m owns nset[m] sets from set0[m],
+// tabulated at increasing temperatures tref (one set: constant data).  Tables
+// are [sets][G] and sigs [sets][G from][G to]; T (device, one value per cell,
+// may be null = 0) is where temperature-dependent materials are evaluated.
 struct pk_xs_s {
-   int nm = 0, G = 0;
-   pk::DBuf<double> sigt, sigs, nusf, kapf, chi, D;
+   int nm = 0, G = 0, ns = 0;
+   bool feedback = false;  // some material has more than one set
+   pk::DBuf<double> sigt, sigs, nusf, kapf, chi, D, tref;
+   pk::DBuf<int> set0, nset;
+   const double* T = nullptr;
    std::vector<double> h_sigt, h_sigs, h_nusf, h_kapf, h_chi, h_D;
 };
 

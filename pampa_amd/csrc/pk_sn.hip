@@ -954,6 +954,7 @@ int pk_sn_create(pk_mesh mesh, pk_xs xs, int M, const double* dirs, const double
    PK_REQUIRE(mesh && xs && dirs && wts && M > 0, "null argument");
    PK_REQUIRE(delta == 1.0, "sweeps need pure upwind face interpolation (mixed-face-interpolation 1)");
    PK_REQUIRE(xs->G <= kMaxSnGroups, "too many energy groups for the sweep kernel");
+   PK_REQUIRE(!xs->feedback, "sweeps need constant cross sections per material");
    const int C = mesh->C;
    PK_REQUIRE((long)M * C < (1L << 31), "mesh too large for the sweep schedule");
    const auto& fptr = mesh->h_fptr;

@@ -14,6 +14,7 @@ Workloads:
   hex_sn_3d         hexagonal lattice of triangular prisms, 4 groups, S_N
   batched_2d        one instance of the batched diffusion family (side, groups drawn)
   transient_core_2d small quarter core with time-step data (velocities, spectra, precursors)
+  feedback_core_2d  quarter core whose fuels have temperature-dependent cross sections
 """
 from __future__ import annotations
 
@@ -286,6 +287,38 @@ def transient_core_2d(d: str, seed: int = 7, side: int = 24, G: int = 3, dt: flo
         write_material(os.path.join(d, n + ".pmp"), **m, **extra)
     with open(os.path.join(d, "input.pmp"), "a") as fh:
         fh.write(f"\ndt -{steps}\n{_fmt(dt)}\n")
+    return d
+
+
+def feedback_core_2d(d: str, seed: int = 9, side: int = 30, temps=(300.0, 600.0, 900.0)):
+    """Quarter core (as quarter_core_2d, 2 groups) whose two plain fuels carry
+    temperature-dependent data: one set per temperature in `temps`, the
+    thermal absorption growing by U[2, 6] % and the fast one by U[0.5, 2] %
+    per 300 K from the drawn values, nu-fission unchanged (a Doppler-like
+    trend).  The absorber fuel and the reflector stay constant."""
+    quarter_core_2d(d, seed=seed, side=side, G=2)
+    base = SplitMix64(seed)
+    fa = multigroup_material(base, 2, True)
+    fb = multigroup_material(base, 2, True)
+    rng = SplitMix64(seed ^ 0xFEEDBAC)
+    for name, m in (("fuel-a", fa), ("fuel-b", fb)):
+        k2, k1 = rng.uniform(0.02, 0.06), rng.uniform(0.005, 0.02)
+        blocks = []
+        for j, T in enumerate(temps):
+            x = (T - temps[0]) / 300.0
+            sa1 = (m["sigt"][0] - m["sigs"][0][1]) * (1.0 + k1 * x)
+            sa2 = m["sigt"][1] * (1.0 + k2 * x)
+            st = [sa1 + m["sigs"][0][1], sa2]
+            blocks += ["   nuclear-data {", "      energy-groups 2", "      sigma-total",
+                       "      " + " ".join(_fmt(v) for v in st), "      nu-sigma-fission",
+                       "      " + " ".join(_fmt(v) for v in m["nusf"]), "      sigma-scattering"]
+            blocks += ["      " + " ".join(_fmt(v) for v in row) for row in m["sigs"]]
+            blocks += ["      diffusion-coefficient", "      " + " ".join(_fmt(v) for v in m["D"]),
+                       "      fission-spectrum", "      " + " ".join(_fmt(v) for v in m["chi"]), "   }"]
+        lines = ["nuclear-data-set {", f"   temperature {len(temps)}", "   " + " ".join(_fmt(t) for t in temps)]
+        lines += blocks + ["}", "fuel 1"]
+        with open(os.path.join(d, name + ".pmp"), "w") as fh:
+            fh.write("\n".join(lines) + "\n")
     return d
 
 
