@@ -211,6 +211,48 @@ def sweep_numbers(pa, d):
         os.chdir(cwd)
 
 
+def contraction_numbers(pa, W, tmp, solves=True):
+    """The S_N scattering + fission source at 16 groups, a per-material 16 x 16
+    contraction per cell, on the vector kernel and on the matrix cores (FP64
+    MFMA), on the lattice geometry of the S6 workload (331 columns x 24
+    triangles x 40 layers = 317,760 cells) with 16 groups (S2: the source does
+    not depend on the directions): device time per call, algorithmic bytes
+    (moment read, source written, volume and material) and flops (2 C G^2), and
+    one cold mixed eigen solve per contraction."""
+    d = W.hex_sn_3d(os.path.join(tmp, "hex-g16"), G=16, order=2)
+    cwd = os.getcwd()
+    os.chdir(d)
+    out = {"workload": "hex lattice 331 columns x 24 triangles x 40 layers, G=16, S2"}
+    try:
+        for name, mode in (("vector", 1), ("mfma", 2)):
+            with pa.DropIn("input.pmp", "-silent", "-pk_precision", "mixed",
+                           "-pk_contraction", "vector" if mode == 1 else "mfma") as run:
+                C, G, _ = run.sizes()
+                r = {}
+                for prec, fp32 in (("fp32_out", True), ("fp64_out", False)):
+                    s, b = run.time_source(20, fp32=fp32, mode=mode)
+                    fl = 2.0 * C * G * G
+                    r[prec] = {"us": s * 1e6, "bytes": b, "GBs": b / s / 1e9, "frac_hbm": b / s / 1e9 / HBM_PEAK_GBS,
+                               "TFLOPs": fl / s / 1e12}
+                if solves:
+                    run.solve()  # records built
+                    run.cold_start()
+                    pa.lib().pk_synchronize()
+                    t0 = time.perf_counter()
+                    run.solve()
+                    pa.lib().pk_synchronize()
+                    k, _ = run.result()
+                    r["eigen_mixed"] = {"ms": 1e3 * (time.perf_counter() - t0), "keff": k,
+                                        "sweeps": run.stats()["inner"]}
+                out[name] = r
+            log(f"[bench]   source contraction G=16 {name}: {r['fp32_out']['us']:.1f} us "
+                f"({r['fp32_out']['GBs']:.0f} GB/s)")
+    finally:
+        os.chdir(cwd)
+    out["mfma_speedup"] = out["vector"]["fp32_out"]["us"] / out["mfma"]["fp32_out"]["us"]
+    return out
+
+
 def _shape(v):
     v = np.asarray(v, dtype=float)
     return v / v.sum()
@@ -563,6 +605,7 @@ def main():
     if args.sweep_pass:
         d2 = W.hex_sn_3d(os.path.join(tmp, "hex-s6"))
         out = sweep_numbers(pa, d2)
+        out["contraction_g16"] = contraction_numbers(pa, W, tmp, solves=False)
         n = 1 << 26
         sec = pa.stream_copy(n, 20)
         out["stream_copy"] = {"bytes": 8.0 * n, "us": sec * 1e6, "GBs": 8.0 * n / sec / 1e9}
@@ -665,6 +708,8 @@ def main():
         sweep["workload"] = "hex lattice 331 columns x 24 triangles x 40 layers, G=4, S6"
         log("[bench] S6 hex lattice eigen solves")
         sweep["eigen"] = sn_solve_numbers(pa, d2)
+        log("[bench] 16-group source contraction, vector kernel vs matrix cores")
+        sweep["contraction_g16"] = contraction_numbers(pa, W, tmp)
 
     batched = serving = None
     if not args.no_batched:

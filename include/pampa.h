@@ -69,6 +69,10 @@ void pampa_ext_get_sizes(int* num_cells, int* num_groups, int* num_directions, i
  * launch: the SpMV of R (diffusion) or one full transport sweep (S_N, summed
  * over its level steps).  fp32 != 0 times the FP32 variant. */
 void pampa_ext_time_kernel(int reps, int fp32, double* seconds, double* bytes, int* error);
+/* The same for the S_N solver's scattering + fission source contraction
+ * (pk_sn_time_source) with contraction mode 1 (vector kernel) or 2 (matrix
+ * cores, 16 groups); the solver keeps its own mode (-pk_contraction). */
+void pampa_ext_time_source(int reps, int fp32, int mode, double* seconds, double* bytes, int* error);
 
 /* ---- serving (no counterpart in the reference; for callers that solve many
  * perturbed copies of a few base problems as requests arrive).  A server owns

@@ -301,6 +301,20 @@ int pk_sn_set_acceleration(pk_sn S, int depth);
 int pk_sn_reset(pk_sn S);
 int pk_sn_get_stats(pk_sn S, int* outer, long* sweeps, double* seconds);
 int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_sweep);
+/* The isotropic scattering + fission source of each source iteration,
+ * q_i = V_i (sum_g2 Sigma_s(g2->g) phiw_i,g2 + chi_g sum_g2 nuSigma_f,g2 phiw_i,g2 / k)
+ * (SNSolver.cxx:417-439 moved to the right-hand side), is a per-material
+ * G x G contraction.  mode 0 (default): on the matrix cores (FP64 MFMA) when
+ * G = 16, else the vector kernel; 1: always the vector kernel; 2: always the
+ * matrix cores (G = 16 only).  The two agree to rounding. */
+int pk_sn_set_contraction(pk_sn S, int mode);
+/* That source in FP64 for a given moment phiw[cell][group] and k (host
+ * arrays), through the handle's contraction: a parity entry point. */
+int pk_sn_source_host(pk_sn S, const double* phiw, double keff, double* q);
+/* Average device time of the source kernel alone in the handle's precision
+ * (reps calls after one warm-up) and its algorithmic bytes per call.  It
+ * overwrites the handle's moment, so the next solve starts cold. */
+int pk_sn_time_source(pk_sn S, int reps, double* avg_seconds, double* bytes_per_call);
 int pk_sn_destroy(pk_sn* S);
 
 /* ---- kernel profiler (replaces the timing printed by petsc::solve with

@@ -96,6 +96,7 @@ def _declare(L):
         "pampa_ext_get_result": [_P, _P, _IP], "pampa_ext_get_operator": [ct.c_char_p, _P, _P, _P, _P, _P, _IP],
         "pampa_ext_get_stats": [_P, _P, _P, _P, _IP], "pampa_ext_get_sizes": [_P, _P, _P, _IP],
         "pampa_ext_time_kernel": [_I, _I, _P, _P, _IP], "pampa_ext_cold_start": [_IP],
+        "pampa_ext_time_source": [_I, _I, _I, _P, _P, _IP],
         "pampa_ext_field_device": [ct.c_char_p, _PP, ct.POINTER(ct.c_long), _IP],
     }
     for name, args in {"pk_profile_enable": [_I], "pk_profile_reset": [],
@@ -124,7 +125,8 @@ def _declare(L):
                        "pk_sn_solve": [_P, _P, _P, _P], "pk_sn_get_stats": [_P, _P, _P, _P],
                        "pk_sn_time_sweep": [_P, _I, _P, _P], "pk_sn_destroy": [_PP],
                        "pk_sn_set_acceleration": [_P, _I], "pk_sn_reset": [_P], "pk_sn_get_moment": [_P, _P],
-                       "pk_sn_prepare": [_P],
+                       "pk_sn_prepare": [_P], "pk_sn_set_contraction": [_P, _I],
+                       "pk_sn_source_host": [_P, _P, _D, _P], "pk_sn_time_source": [_P, _I, _P, _P],
                        "pk_sn_set_xs": [_P, _P, _P, _P, _P, _P]}.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -613,6 +615,15 @@ class DropIn:
             raise PkError("pampa_ext_time_kernel failed")
         return s.value, b.value
 
+    def time_source(self, reps=20, fp32=False, mode=1):
+        """(seconds, algorithmic bytes) of the S_N source kernel; mode 1
+        vector kernel, 2 matrix cores (16 groups)."""
+        s, b, err = ct.c_double(), ct.c_double(), ct.c_int()
+        self.L.pampa_ext_time_source(reps, int(fp32), int(mode), ct.byref(s), ct.byref(b), ct.byref(err))
+        if err.value:
+            raise PkError("pampa_ext_time_source failed")
+        return s.value, b.value
+
     def operator(self, name: str):
         n, nnz, err = ct.c_long(), ct.c_long(), ct.c_int()
         self.L.pampa_ext_get_operator(name.encode(), ct.byref(n), ct.byref(nnz), None, None, None,
@@ -690,6 +701,25 @@ class Sweep(Handle):
     def time_sweep(self, reps=10):
         t, b = ct.c_double(), ct.c_double()
         check(lib().pk_sn_time_sweep(self.h, reps, ct.byref(t), ct.byref(b)))
+        return t.value, b.value
+
+    def set_contraction(self, mode):
+        """Source contraction: 0 by group count (matrix cores at 16 groups),
+        1 vector kernel, 2 matrix cores (16 groups only)."""
+        check(lib().pk_sn_set_contraction(self.h, int(mode)))
+
+    def source(self, phiw, keff):
+        """The scattering + fission source q[C*G] (volume included) of a
+        moment phiw[C*G] at k, in FP64 through the handle's contraction."""
+        phiw = np.ascontiguousarray(phiw, np.float64)
+        q = np.zeros(self.C * self.G)
+        check(lib().pk_sn_source_host(self.h, ptr(phiw), float(keff), ptr(q)))
+        return q
+
+    def time_source(self, reps=20):
+        """(seconds per source kernel, algorithmic bytes per call)."""
+        t, b = ct.c_double(), ct.c_double()
+        check(lib().pk_sn_time_source(self.h, reps, ct.byref(t), ct.byref(b)))
         return t.value, b.value
 
     def solve(self, accel=5, tol_k=1e-9, tol_phi=1e-8, max_outer=100000, cold=True):

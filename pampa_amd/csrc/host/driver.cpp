@@ -56,6 +56,10 @@ int Driver::setOption(const std::string& name, const std::string& value) {
       PAMPA_CHECK(value != "bj" && value != "bj2" && value != "cell",
                   "wrong pk_preconditioner (bj, bj2 or cell)");
       options.preconditioner = value == "bj2" ? 1 : (value == "cell" ? 2 : 0);
+   } else if (n == "pk_contraction") {
+      PAMPA_CHECK(value != "auto" && value != "vector" && value != "mfma",
+                  "wrong pk_contraction (auto, vector or mfma)");
+      options.contraction = value == "vector" ? 1 : (value == "mfma" ? 2 : 0);
    } else if (n == "ksp_gmres_restart") {
       PAMPA_CHECK(input::read(options.gmres_restart, 1, 64, value), "wrong ksp_gmres_restart");
    } else if (n == "pk_precision") {

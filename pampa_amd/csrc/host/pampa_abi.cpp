@@ -94,6 +94,11 @@ void pampa_ext_time_kernel(int reps, int fp32, double* seconds, double* bytes, i
    *error = s ? s->timeKernel(reps, fp32, seconds, bytes) : 1;
 }
 
+void pampa_ext_time_source(int reps, int fp32, int mode, double* seconds, double* bytes, int* error) {
+   pampa::Solver* s = g_driver.mainSolver();
+   *error = s ? s->timeSource(reps, fp32, mode, seconds, bytes) : 1;
+}
+
 void pampa_ext_cold_start(int* error) {
    pampa::Solver* s = g_driver.mainSolver();
    *error = s ? s->coldStart() : 1;

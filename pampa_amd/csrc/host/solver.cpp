@@ -506,6 +506,7 @@ int SNSolver::build() {
                             bc_type.data(), &sn));
       PK_CHECK(pk_sn_set_precision(sn, options.precision));
       PK_CHECK(pk_sn_set_tolerances(sn, options.sn_tol_k, options.sn_tol_phi, options.sn_max_outer));
+      PK_CHECK(pk_sn_set_contraction(sn, options.contraction));
    }
    return 0;
 }
@@ -625,6 +626,17 @@ int SNSolver::timeKernel(int reps, int fp32, double* seconds, double* bytes) {
    PAMPA_CHECK(!sn, "timing needs the swept (matrix-free) S_N operator");
    PK_CHECK(pk_sn_set_precision(sn, fp32 ? PK_PREC_MIXED : PK_PREC_FP64));
    int rc = pk_sn_time_sweep(sn, reps, seconds, bytes);
+   PK_CHECK(pk_sn_set_precision(sn, options.precision));
+   PK_CHECK(rc);
+   return 0;
+}
+
+int SNSolver::timeSource(int reps, int fp32, int mode, double* seconds, double* bytes) {
+   PAMPA_CHECK(!sn, "timing needs the swept (matrix-free) S_N operator");
+   PK_CHECK(pk_sn_set_precision(sn, fp32 ? PK_PREC_MIXED : PK_PREC_FP64));
+   PK_CHECK(pk_sn_set_contraction(sn, mode));
+   int rc = pk_sn_time_source(sn, reps, seconds, bytes);
+   PK_CHECK(pk_sn_set_contraction(sn, options.contraction));
    PK_CHECK(pk_sn_set_precision(sn, options.precision));
    PK_CHECK(rc);
    return 0;

@@ -21,6 +21,9 @@ struct SolverOptions {
    // 0 cell-block Jacobi + workgroup-block sweep (bj), 1 with one Richardson
    // correction (bj2), 2 cell-block Jacobi alone (cell); -1: the solver's choice
    int preconditioner = -1;
+   // S_N scattering + fission source: 0 matrix cores at 16 groups, else the
+   // vector kernel (auto); 1 vector kernel; 2 matrix cores
+   int contraction = 0;
    double ksp_rtol = 1e-5, ksp_atol = 1e-50;  // PETSc KSP defaults
    int ksp_max_it = 10000;
    double sn_tol_k = 1e-9, sn_tol_phi = 1e-8;
@@ -72,6 +75,11 @@ class Solver {
    virtual int getStats(int* outer, long* inner, double* residual, double* seconds) const;
    virtual void getSizes(int* cells, int* groups, int* directions) const = 0;
    virtual int timeKernel(int reps, int fp32, double* seconds, double* bytes) = 0;
+   // the S_N source contraction alone (mode as SolverOptions::contraction)
+   virtual int timeSource(int reps, int fp32, int mode, double* seconds, double* bytes) {
+      PAMPA_CHECK(true, "the solver has no source contraction kernel");
+      return 0;
+   }
    // forget any state a steady solve starts from (other than the fields)
    virtual int coldStart() { return 0; }
 
@@ -161,6 +169,7 @@ class SNSolver : public NeutronicSolver {
                    double* val) const override;
    int numDirections() const override { return M; }
    int timeKernel(int reps, int fp32, double* seconds, double* bytes) override;
+   int timeSource(int reps, int fp32, int mode, double* seconds, double* bytes) override;
    // the sweep handle (null when the input is not sweepable)
    pk_sn sweepHandle() const { return sweepable ? sn : nullptr; }
    int coldStart() override { return sn ? pk_sn_reset(sn) : 0; }

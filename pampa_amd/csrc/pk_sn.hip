@@ -408,21 +408,126 @@ static void sn_moment(const T* psi, const int* pos, const double* wts, int M, in
    }
 }
 
-template <class T>
-__global__ void sn_source_kernel(int C, int G, const double* vol, const int* mat, const double* sigs,
-                                 const double* nusf, const double* chi, const double* phiw,
-                                 const double* kk, T* q) {
+// q[i][g] = V_i (sum_g2 Sigma_s(g2 -> g) phiw[i][g2] + chi_g / k sum_g2 nuSf_g2
+// phiw[i][g2]), one thread per cell, sums in ascending g2 (the oracle's order).
+// GT > 0: the group count at compile time, the cell's moment held in registers.
+template <class T, int GT>
+__global__ void __launch_bounds__(kBlock) sn_source_kernel(int C, int G_, const double* __restrict__ vol,
+                                                           const int* __restrict__ mat,
+                                                           const double* __restrict__ sigs,
+                                                           const double* __restrict__ nusf,
+                                                           const double* __restrict__ chi,
+                                                           const double* __restrict__ phiw,
+                                                           const double* __restrict__ kk, T* __restrict__ q) {
 #pragma clang fp contract(off)
-   int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const int G = GT > 0 ? GT : G_;
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i >= C) return;
    const int mt = mat[i];
    const double inv_k = 1.0 / kk[0];
+   constexpr int GM = GT > 0 ? GT : kMaxSnGroups;
+   double ph[GM];
+#pragma unroll
+   for (int g2 = 0; g2 < GM; g2++) ph[g2] = (GT > 0 || g2 < G) ? phiw[(long)i * G + g2] : 0.0;
    double fis = 0.0;
-   for (int g2 = 0; g2 < G; g2++) fis += nusf[mt * G + g2] * phiw[(long)i * G + g2];
+#pragma unroll
+   for (int g2 = 0; g2 < GM; g2++)
+      if (GT > 0 || g2 < G) fis += nusf[mt * G + g2] * ph[g2];
    for (int g = 0; g < G; g++) {
       double s = 0.0;
-      for (int g2 = 0; g2 < G; g2++) s += sigs[(mt * G + g2) * G + g] * phiw[(long)i * G + g2];
+#pragma unroll
+      for (int g2 = 0; g2 < GM; g2++)
+         if (GT > 0 || g2 < G) s += sigs[(mt * G + g2) * G + g] * ph[g2];
       q[(long)i * G + g] = (T)(vol[i] * (s + chi[mt * G + g] * inv_k * fis));
+   }
+}
+
+// The same source for 16 groups on the matrix cores: with the per-material
+// matrix K_m[g2][g] = Sigma_s,m(g2 -> g) + chi_m(g) nuSigma_f,m(g2) / k the
+// source of 16 cells is the product Q(16 cells x 16 groups) = Phi(16 x 16) K_m,
+// four V_MFMA_F64_16X16X4_F64 steps, then q = V Q.  A wavefront takes 16
+// consecutive cells.  Lane l supplies row l % 16 of A and column l % 16 of B
+// at inner index l / 16, and holds D rows 4 v + l / 16, v < 4, column l % 16
+// (so the 64 lanes of item v hold 4 consecutive cells' 16 groups).
+// Product step c (c < 4) pairs inner index kq with group 4 kq + c, so a lane
+// reads its cell's 4 consecutive groups 4 kq .. 4 kq + 3 once.  Cells of a
+// tile with different materials: one pass per distinct material, the other
+// rows of A zero (they add exact zeros).  Sums in another order than the
+// vector kernel, so the two agree to rounding, not bitwise.
+typedef double sn_d4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__global__ void __launch_bounds__(256) sn_source_mfma_kernel(int C, const double* __restrict__ vol,
+                                                             const int* __restrict__ mat,
+                                                             const double* __restrict__ sigs,
+                                                             const double* __restrict__ nusf,
+                                                             const double* __restrict__ chi,
+                                                             const double* __restrict__ phiw,
+                                                             const double* __restrict__ kk, T* __restrict__ q) {
+   constexpr int G = 16;
+   const int lane = threadIdx.x & 63;
+   const long c0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+   if (c0 >= C) return;  // the whole wavefront
+   const int i = lane & 15, kq = lane >> 4;
+   const long cell = c0 + i;
+   const bool valid = cell < C;
+   double a[4] = {0.0, 0.0, 0.0, 0.0};
+   int mine = -1;
+   if (valid) {
+      const double2* p = (const double2*)(phiw + cell * G + 4 * kq);
+      const double2 p0 = p[0], p1 = p[1];
+      a[0] = p0.x;
+      a[1] = p0.y;
+      a[2] = p1.x;
+      a[3] = p1.y;
+      mine = mat[cell];
+   }
+   const double inv_k = 1.0 / kk[0];
+   sn_d4 d = {0.0, 0.0, 0.0, 0.0};
+   bool pending = valid;
+   for (;;) {
+      const unsigned long long bal = __ballot(pending);
+      if (!bal) break;
+      const int m = __shfl(mine, __ffsll((long long)bal) - 1);
+      const bool take = pending && mine == m;
+      const double cm = chi[m * G + i] * inv_k;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+         const int g2 = 4 * kq + c;
+         const double b = sigs[(m * G + g2) * G + i] + cm * nusf[m * G + g2];
+         d = __builtin_amdgcn_mfma_f64_16x16x4f64(take ? a[c] : 0.0, b, d, 0, 0, 0);
+      }
+      pending = pending && !take;
+   }
+#pragma unroll
+   for (int v = 0; v < 4; v++) {
+      const long cr = c0 + 4 * v + kq;
+      if (cr < C) q[cr * G + i] = (T)(vol[cr] * d[v]);
+   }
+}
+
+// Which contraction computes the source: the matrix cores for 16 groups
+// (mode 0 = by group count, 2 = matrix cores), else the vector kernel.
+static inline bool sn_use_mfma(int mode, int G) { return G == 16 && mode != 1; }
+
+template <class T>
+static void sn_source(int mode, int C, int G, const pk_mesh_s* mesh, const pk_xs_s* xs, const double* phiw,
+                      const double* kk, T* q, hipStream_t st) {
+   if (sn_use_mfma(mode, G)) {
+      const long tiles = ((long)C + 15) / 16;
+      sn_source_mfma_kernel<T><<<(unsigned)((tiles + 3) / 4), 256, 0, st>>>(C, mesh->vol.p, mesh->mat.p, xs->sigs.p,
+                                                                          xs->nusf.p, xs->chi.p, phiw, kk, q);
+      return;
+   }
+   const double *vol = mesh->vol.p, *sigs = xs->sigs.p, *nusf = xs->nusf.p, *chi = xs->chi.p;
+   const int* mat = mesh->mat.p;
+   switch (G) {
+      case 1: sn_source_kernel<T, 1><<<grid_for(C), kBlock, 0, st>>>(C, G, vol, mat, sigs, nusf, chi, phiw, kk, q); break;
+      case 2: sn_source_kernel<T, 2><<<grid_for(C), kBlock, 0, st>>>(C, G, vol, mat, sigs, nusf, chi, phiw, kk, q); break;
+      case 4: sn_source_kernel<T, 4><<<grid_for(C), kBlock, 0, st>>>(C, G, vol, mat, sigs, nusf, chi, phiw, kk, q); break;
+      case 8: sn_source_kernel<T, 8><<<grid_for(C), kBlock, 0, st>>>(C, G, vol, mat, sigs, nusf, chi, phiw, kk, q); break;
+      case 16: sn_source_kernel<T, 16><<<grid_for(C), kBlock, 0, st>>>(C, G, vol, mat, sigs, nusf, chi, phiw, kk, q); break;
+      default: sn_source_kernel<T, 0><<<grid_for(C), kBlock, 0, st>>>(C, G, vol, mat, sigs, nusf, chi, phiw, kk, q); break;
    }
 }
 
@@ -608,6 +713,7 @@ struct pk_sn_s {
    double value_units = 0.0;
    // Anderson acceleration: depth (0 = plain source iteration) and history
    int aa_depth = 5;
+   int contraction = 0;  // source contraction: 0 by group count, 1 vector, 2 matrix cores
    // mixed precision: FP32 sweeps until the iterate's relative change falls
    // below switch64 (the FP32 round-off floor is near 1e-7), then FP64 sweeps
    bool finish64 = true;
@@ -1332,13 +1438,11 @@ int pk_sn_solve(pk_sn S, double* keff, pk_vec psi, pk_vec phiw_out) {
       PK_HIP(hipMemcpyAsync(S->old.p, S->phiw.p, sizeof(double) * CG, hipMemcpyDeviceToDevice, c.stream));
       last32 = use32;
       if (use32) {
-         sn_source_kernel<float><<<grid_for(C), kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->sigs.p,
-                                                                    xs->nusf.p, xs->chi.p, S->phiw.p, st, S->q32.p);
+         sn_source<float>(S->contraction, C, G, mesh, xs, S->phiw.p, st, S->q32.p, c.stream);
          PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
          sn_moment<float>(S->psi32.p, S->dpos.p, S->wts.p, M, C, G, S->phiw.p, c.stream);
       } else {
-         sn_source_kernel<double><<<grid_for(C), kBlock, 0, c.stream>>>(C, G, mesh->vol.p, mesh->mat.p, xs->sigs.p,
-                                                                     xs->nusf.p, xs->chi.p, S->phiw.p, st, S->q64.p);
+         sn_source<double>(S->contraction, C, G, mesh, xs, S->phiw.p, st, S->q64.p, c.stream);
          PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
          sn_moment<double>(S->psi64.p, S->dpos.p, S->wts.p, M, C, G, S->phiw.p, c.stream);
       }
@@ -1452,6 +1556,63 @@ int pk_sn_get_stats(pk_sn S, int* outer, long* sweeps, double* seconds) {
    if (outer) *outer = S->outer;
    if (sweeps) *sweeps = S->sweeps;
    if (seconds) *seconds = S->seconds;
+   return 0;
+}
+
+int pk_sn_set_contraction(pk_sn S, int mode) {
+   PK_REQUIRE(S, "null handle");
+   PK_REQUIRE(mode >= 0 && mode <= 2, "contraction mode must be 0, 1 or 2");
+   PK_REQUIRE(mode != 2 || S->G == 16, "the matrix-core source contraction takes 16 groups");
+   S->contraction = mode;
+   return 0;
+}
+
+int pk_sn_source_host(pk_sn S, const double* phiw, double keff, double* q) {
+   PK_REQUIRE(S && phiw && q, "null argument");
+   PK_REQUIRE(keff != 0.0, "keff must be nonzero");
+   auto& c = S->sc;
+   const long CG = (long)S->C * S->G;
+   DBuf<double> dphi, dq, dk;
+   PK_REQUIRE(!dphi.alloc(CG) && !dq.alloc(CG) && !dk.alloc(1), "device allocation failed");
+   PK_HIP(hipMemcpyAsync(dphi.p, phiw, sizeof(double) * CG, hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(dk.p, &keff, sizeof(double), hipMemcpyHostToDevice, c.stream));
+   sn_source<double>(S->contraction, S->C, S->G, S->mesh, S->xs, dphi.p, dk.p, dq.p, c.stream);
+   PK_HIP(hipGetLastError());
+   PK_HIP(hipMemcpyAsync(q, dq.p, sizeof(double) * CG, hipMemcpyDeviceToHost, c.stream));
+   PK_HIP(hipStreamSynchronize(c.stream));
+   return 0;
+}
+
+int pk_sn_time_source(pk_sn S, int reps, double* avg_seconds, double* bytes_per_call) {
+   PK_REQUIRE(S && reps > 0, "wrong arguments");
+   PK_CALL(sn_prepare(S));
+   auto& c = S->sc;
+   const int C = S->C, G = S->G;
+   const long CG = (long)C * G;
+   const bool f32 = S->precision == PK_PREC_MIXED;
+   sn_fill_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->phiw.p, CG, 1.0);
+   sn_fill_kernel<double><<<1, 1, 0, c.stream>>>(S->st.p, 1, 1.0);
+   hipEvent_t e0, e1;
+   PK_HIP(hipEventCreate(&e0));
+   PK_HIP(hipEventCreate(&e1));
+   for (int r = -1; r < reps; r++) {
+      if (r == 0) PK_HIP(hipEventRecord(e0, c.stream));
+      if (f32)
+         sn_source<float>(S->contraction, C, G, S->mesh, S->xs, S->phiw.p, S->st.p, S->q32.p, c.stream);
+      else
+         sn_source<double>(S->contraction, C, G, S->mesh, S->xs, S->phiw.p, S->st.p, S->q64.p, c.stream);
+   }
+   PK_HIP(hipEventRecord(e1, c.stream));
+   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipGetLastError());
+   float ms = 0.f;
+   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   if (avg_seconds) *avg_seconds = ms * 1e-3 / reps;
+   // the moment read, the source written, volume and material per cell
+   if (bytes_per_call) *bytes_per_call = (double)CG * (8.0 + (f32 ? 4.0 : 8.0)) + (double)C * 12.0;
+   S->has_state = false;  // the moment was overwritten: the next solve starts cold
    return 0;
 }
 
