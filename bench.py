@@ -453,6 +453,22 @@ def _class_sample(W, tmp, n, rows):
     return out
 
 
+def tuner_report(pa):
+    """The autotuner's choice per size class of the pools run so far: slice
+    height and SpMV workgroup, the chosen candidate's time and the 64-row /
+    256-row default's (FP32 values in mixed pools)."""
+    out = {}
+    for c in pa.tune_choices():
+        side = int(round((c["rows"] / c["block"]) ** 0.5))
+        name = f"{side}x{side}x{c['block']}" + ("" if c["precision"] == pa.PK_PREC_MIXED else "-fp64")
+        if side * side * c["block"] != c["rows"]:
+            name = f"rows{c['rows']}xG{c['block']}"
+        best = c["us"][f"{c['slice_height']}x{c['spmv_block']}"]
+        out[name] = {"slice_height": c["slice_height"], "spmv_block": c["spmv_block"], "us": round(best, 2),
+                     "us_64x256": round(c["us"]["64x256"], 2)}
+    return out
+
+
 def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
     """Config-3 family: n_inst independent instances (sides and group counts
     drawn by seed, cross sections perturbed +-5% per instance), all admitted at
@@ -487,7 +503,7 @@ def batched_numbers(pa, W, oracle, tmp, n_inst=256, fp64_sample=32):
             "mixed_batched_solves_per_s": mixed_rate, "mixed_over_fp64": mixed_rate / fp64_rate,
             "accuracy_vs_fp64_oracle": acc, "outer_mean": float(res["outer"].mean()),
             "inner_mean": float(res["inner"].mean()), "profiled_kernels": prof, "roofline": roof,
-            "spmv_all_instances": spmv}
+            "spmv_all_instances": spmv, "autotuner": tuner_report(pa)}
 
 
 def _sn_bases(W, tmp, n, seed=5):
@@ -639,7 +655,9 @@ def main():
         return
     if args.spmv_batched_only:
         paths, rows = _family(W, os.path.join(tmp, "family"), args.batch)
-        print(json.dumps(batched_spmv(pa, paths)))
+        out = batched_spmv(pa, paths)
+        out["autotuner"] = tuner_report(pa)
+        print(json.dumps(out))
         return
 
     results = {}

@@ -237,6 +237,22 @@ int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction,
 int pk_pool_admit(pk_batch P, pk_mat R, pk_mat F, pk_vec x0, int* slot);
 /* *fits = 1 when pk_pool_admit(R, F) would find room now */
 int pk_pool_fits(pk_batch P, pk_mat R, pk_mat F, int* fits);
+/* Autotuner of the pool's operator layout (no counterpart in the reference,
+ * whose AIJ storage has no tunable layout; petsc.cxx:91-111 only
+ * preallocates rows).  The first problem of each size class (rows, cell
+ * block, nonzeros, precision) admitted to any pool times the sliced-ELL SpMV
+ * of its R for slice heights 32, 64, 128 and workgroups of 128, 256, 512,
+ * 1024 rows, and keeps the fastest pair for every problem of the class (the
+ * matrix is laid out again with that height; the plain SpMV takes that
+ * workgroup).  Every choice gives the same bits.  PK_TUNE=0 disables it,
+ * PK_SLICE=32|64|128 forces a height.  pk_tune_get reports class i in the
+ * order tuned: its key, the choice, and the 12 timings in microseconds
+ * ([height 32/64/128][workgroup 128/256/512/1024], 0 = height skipped because
+ * its padding exceeds the 64-row layout's). */
+int pk_tune_count(int* n);
+int pk_tune_get(int i, long* rows, int* block, long* nnz, int* precision, int* slice_height, int* spmv_block,
+                double* us);
+int pk_tune_reset(void);
 /* device bytes one problem (R, F) holds in a pool of this precision */
 int pk_pool_footprint(pk_mat R, pk_mat F, int precision, int restart, double* bytes);
 int pk_pool_step(pk_batch P, int ticks, int* live, int* done);

@@ -74,6 +74,7 @@ def _declare(L):
         "pk_batch_set_shift": [_P, _D], "pk_batch_set_initial": [_P, _I, _P], "pk_batch_solve": [_P],
         "pk_batch_get_eigenpair": [_P, _I, _P, _P], "pk_batch_get_stats": [_P, _I, _P, _P, _P, _P],
         "pk_batch_destroy": [_PP],
+        "pk_tune_count": [_P], "pk_tune_get": [_I, _P, _P, _P, _P, _P, _P, _P], "pk_tune_reset": [],
         "pk_pool_create": [_L, _L, _L, _L, _I, _I, _I, _PP],
         "pk_pool_set_options": [_P, _D, _I, _D, _I, _D], "pk_pool_admit": [_P, _P, _P, _P, _P],
         "pk_pool_step": [_P, _I, _P, _P], "pk_pool_collect": [_P, _I, _P, _P],
@@ -378,6 +379,31 @@ class Batch(Handle):
 
 
 PK_POOL_FULL = 2
+
+
+def tune_choices():
+    """The autotuner's choices so far, one dict per size class in the order
+    tuned: rows, cell block, nonzeros, precision, slice height, SpMV workgroup
+    and the timings (us) of every (height, workgroup) candidate."""
+    L = lib()
+    n = ct.c_int()
+    check(L.pk_tune_count(ct.byref(n)))
+    out = []
+    for i in range(n.value):
+        rows, nnz = ct.c_long(), ct.c_long()
+        bs, prec, h, blk = ct.c_int(), ct.c_int(), ct.c_int(), ct.c_int()
+        us = np.zeros(12)
+        check(L.pk_tune_get(i, ct.byref(rows), ct.byref(bs), ct.byref(nnz), ct.byref(prec), ct.byref(h),
+                            ct.byref(blk), ptr(us)))
+        out.append({"rows": rows.value, "block": bs.value, "nnz": nnz.value, "precision": prec.value,
+                    "slice_height": h.value, "spmv_block": blk.value,
+                    "us": {f"{hh}x{bb}": float(us[a * 4 + q]) for a, hh in enumerate((32, 64, 128))
+                           for q, bb in enumerate((128, 256, 512, 1024))}})
+    return out
+
+
+def tune_reset():
+    check(lib().pk_tune_reset())
 
 
 class Pool(Handle):

@@ -3,8 +3,8 @@
 // problems admitted and released while others run (continuous batching).
 //
 // Layout.  The pool owns one row space (in units of 256 rows, one workgroup
-// each) and entry spaces for the SELL-64 values of R, F and the shifted
-// operator A = R - F/ks.  A problem in slot b owns a range of row units
+// each) and entry spaces for the sliced-ELL values of R, F and the shifted
+// operator A = R - F/ks (slice height per problem, from the autotuner).  A problem in slot b owns a range of row units
 // (n_b rounded up to 256, so a workgroup never holds two problems) and ranges
 // of entries; the columns of its SELL arrays are moved by its row offset, so
 // the pool is one block-diagonal system.  unit_slot[] maps a row unit to its
@@ -31,6 +31,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <map>
+#include <mutex>
+#include <string>
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
@@ -57,12 +59,28 @@ struct BLayout {
    int sweep;              // 1: S includes the workgroup-block sweep, 0: S = B^-1
 };
 
+// Sliced-ELL operator of the pool.  Each problem keeps the slice height h of
+// its matrix (32, 64 or 128 rows, picked per size class by the autotuner
+// below); the tables have one entry per 32-row group g of the pool: the
+// offset of the group's first row in its slice, the slice width and h, so
+// entry k of row r sits at off[r / 32] + r % 32 + h k.
 struct BSell {
    const long* off;
    const int* width;
+   const int* h;
    const int* col;
    const int* rlen;
 };
+
+struct SellRef {
+   long base;  // position of entry 0 of the row
+   int wd, h;  // slice width and height: entry k at base + h k
+};
+
+__device__ __forceinline__ SellRef sell_ref(const BSell& A, long row) {
+   const long g = row >> 5;
+   return SellRef{A.off[g] + (row & 31), A.width[g], A.h[g]};
+}
 
 struct BState {  // per-slot state (arrays over slots)
    int m;
@@ -110,17 +128,23 @@ __device__ __forceinline__ int slot_of_row(const BLayout& L, long row) { return 
 
 // ---- admission: place a problem's SELL arrays in the pool --------------------------
 
-__global__ void place_slices_kernel(long ns_src, long ns_dst, const long* soff, const int* swidth,
-                                    long dst_slice0, long base, long* doff, int* dwidth) {
-   long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (s >= ns_dst) return;
+// table entries of the 32-row groups of a problem from its matrix's SELL-h
+// tables: group j (rows 32 j .. 32 j + 31 of the problem) lies in source slice
+// 32 j / h at row 32 j mod h of that slice; groups past the matrix's slices
+// are empty
+__global__ void place_groups_kernel(long ns_src, int h, long ng_dst, const long* soff, const int* swidth,
+                                    long dst_g0, long base, long* doff, int* dwidth, int* dh) {
+   const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (j >= ng_dst) return;
+   const long s = 32 * j / h;
    if (s < ns_src) {
-      doff[dst_slice0 + s] = base + (soff[s] - soff[0]);
-      dwidth[dst_slice0 + s] = swidth[s];
+      doff[dst_g0 + j] = base + (soff[s] - soff[0]) + (32 * j - s * h);
+      dwidth[dst_g0 + j] = swidth[s];
    } else {
-      doff[dst_slice0 + s] = base;
-      dwidth[dst_slice0 + s] = 0;
+      doff[dst_g0 + j] = base;
+      dwidth[dst_g0 + j] = 0;
    }
+   dh[dst_g0 + j] = h;
 }
 
 __global__ void place_cols_kernel(long n, const int* scol, int shift, int* dcol) {
@@ -150,19 +174,16 @@ __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BS
    if (b < 0 || !S.rebuild[b]) return;
    const double inv_ks = S.ks[b] > 0.0 ? 1.0 / S.ks[b] : 0.0;
    const long local = row - L.row_off[b];
-   const long s = row >> 6;
-   const int lane = (int)(row & 63);
-   const long base = R.off[s] + lane;
+   const SellRef ra = sell_ref(R, row), fa = sell_ref(F, row);
    const int fb = L.fbs[b];
    const long cell = local / fb;
-   const long fbase = F.off[s] + lane;
    const int len = R.rlen[row];
-   for (int k = 0; k < R.width[s]; k++) {
-      long p = base + (long)k * 64;
+   for (int k = 0; k < ra.wd; k++) {
+      long p = ra.base + (long)k * ra.h;
       double v = rval[p];
       if (k < len && inv_ks != 0.0) {
          long c = (long)R.col[p] - L.row_off[b];
-         if (c / fb == cell) v -= fval[fbase + (c % fb) * 64] * inv_ks;
+         if (c / fb == cell) v -= fval[fa.base + (c % fb) * fa.h] * inv_ks;
       }
       aval[p] = v;
       if (aval32) aval32[p] = (float)v;
@@ -187,11 +208,10 @@ __global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* 
    }
    for (int r = 0; r < bs; r++) {
       m[r * bs + r] = 1.0;
-      long rr = row + r;
-      long s = rr >> 6;
-      int lane = (int)(rr & 63);
+      const long rr = row + r;
+      const SellRef ar = sell_ref(A, rr);
       for (int k = 0; k < A.rlen[rr]; k++) {
-         long p = A.off[s] + (long)k * 64 + lane;
+         long p = ar.base + (long)k * ar.h;
          long c = (long)A.col[p] - L.row_off[b];
          if (c / bs == cell) a[r * bs + (c % bs)] += aval[p];
       }
@@ -247,10 +267,9 @@ __global__ void __launch_bounds__(kRows) brf_kernel(BLayout L, BSell R, const do
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kOuter) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const long s = row >> 6;
-   const int lane = threadIdx.x & 63;
-   const double a = sell_row_dot<double, double>(R.col, rval, x, R.off[s] + lane, R.width[s], 0.0);
-   const double c = sell_row_dot<double, double>(F.col, fval, x, F.off[s] + lane, F.width[s], 0.0);
+   const SellRef ra = sell_ref(R, row), fa = sell_ref(F, row);
+   const double a = sell_row_dot<double, double>(R.col, rval, x, ra.base, ra.wd, 0.0, ra.h);
+   const double c = sell_row_dot<double, double>(F.col, fval, x, fa.base, fa.wd, 0.0, fa.h);
    y1[row] = a;
    y2[row] = c;
    const double xv = x[row];
@@ -378,9 +397,8 @@ __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* bin
    __syncthreads();
    const long row0 = blockIdx.x * (long)kRows;
    const long row = row0 + threadIdx.x;
-   const long s = row >> 6;
-   const long base = A.off[s] + (threadIdx.x & 63);
-   const int wd = A.width[s];
+   const SellRef ar = sell_ref(A, row);
+   const int wd = ar.wd;
    T acc = 0;
    // entries 4 at a time: their loads first, then the sum in order
    for (int k0 = 0; k0 < wd; k0 += 4) {
@@ -388,7 +406,7 @@ __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* bin
       T aa[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-         const long p = base + (long)min(k0 + u, wd - 1) * 64;
+         const long p = ar.base + (long)min(k0 + u, wd - 1) * ar.h;
          cc[u] = A.col[p];
          aa[u] = aval[p];
       }
@@ -545,9 +563,8 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
           1, [&](int) { return vj * vj; }, part, tickets + b, gtickets + L.wg_first[b], hd + j + 1,
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    } else {
-      const long s = row >> 6;
-      const long base = A.off[s] + (threadIdx.x & 63);
-      const T acc = sell_row_dot<T, T>(A.col, aval, z, base, A.width[s], (T)0);
+      const SellRef ar = sell_ref(A, row);
+      const T acc = sell_row_dot<T, T>(A.col, aval, z, ar.base, ar.wd, (T)0, ar.h);
       w[row] = acc;
       const double wr = (double)acc;
       const int nv = j + 2;
@@ -726,8 +743,8 @@ __device__ __forceinline__ void poly_step(const BLayout& L, const BSell& A, cons
    if (mode == 0) count_act(L, b, act, 5, 1, -1, 0);
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const long s = row >> 6;
-   const T az = sell_row_dot<T, T>(A.col, aval, z0, A.off[s] + (threadIdx.x & 63), A.width[s], (T)0);
+   const SellRef ar = sell_ref(A, row);
+   const T az = sell_row_dot<T, T>(A.col, aval, z0, ar.base, ar.wd, (T)0, ar.h);
    const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
    const T zr = z0[row] + c;
    z[row] = zr;
@@ -781,9 +798,8 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    if (b < 0 || S.phase[b] != kInner || (S.flags[4 * b] && S.flags[4 * b + 1] == 0)) return;
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const long s = row >> 6;
-   const long base = A.off[s] + (threadIdx.x & 63);
-   const T acc = sell_row_dot<T, T>(A.col, aval, e, base, A.width[s], (T)0);
+   const SellRef ar = sell_ref(A, row);
+   const T acc = sell_row_dot<T, T>(A.col, aval, e, ar.base, ar.wd, (T)0, ar.h);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
    z[row] = precond_apply<T>(L, b, binv, A, aval, rv, tl, xl);
@@ -819,9 +835,8 @@ __global__ void __launch_bounds__(kRows) bspmv_kernel(BLayout L, BSell A, const 
    const int b = slot_of_wg(L);
    if (b < 0) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const long s = row >> 6;
-   const long base = A.off[s] + (threadIdx.x & 63);
-   y[row] = sell_row_dot<T, T>(A.col, aval, x, base, A.width[s], (T)0);
+   const SellRef ar = sell_ref(A, row);
+   y[row] = sell_row_dot<T, T>(A.col, aval, x, ar.base, ar.wd, (T)0, ar.h);
 }
 
 __global__ void bphase_kernel(int nslots, BState S) {
@@ -937,8 +952,8 @@ struct pk_batch_s {
    // device arrays
    DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen;
    DBuf<long> row_off, binv_off;
-   DBuf<long> r_off, f_off;
-   DBuf<int> r_width, f_width, r_col, f_col, rlen;
+   DBuf<long> r_off, f_off;  // per 32-row group
+   DBuf<int> r_width, f_width, r_h, f_h, r_col, f_col, rlen;
    DBuf<double> r_val, f_val, a_val;
    DBuf<double> x, phi, y1, y2, r;
    DBuf<double> sd;
@@ -986,8 +1001,8 @@ struct pk_batch_s {
       return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p,
                      opts.sweep};
    }
-   BSell rsell() const { return BSell{r_off.p, r_width.p, r_col.p, rlen.p}; }
-   BSell fsell() const { return BSell{f_off.p, f_width.p, f_col.p, rlen.p}; }
+   BSell rsell() const { return BSell{r_off.p, r_width.p, r_h.p, r_col.p, rlen.p}; }
+   BSell fsell() const { return BSell{f_off.p, f_width.p, f_h.p, f_col.p, rlen.p}; }
 };
 
 static int preconditioner_block(int bs) {
@@ -1005,10 +1020,10 @@ static inline long blocks64(long n) { return (n + 63) / 64; }
 // device bytes of a pool of the given capacity (units of 256 rows, entries and
 // block-inverse values in units of 64)
 static double pool_bytes(long units, long rent, long fent, long binv, int slots, int m, int precision) {
-   const double rows = (double)units * kRows, sl = rows / 64.0;
+   const double rows = (double)units * kRows, ng = rows / 32.0;
    const double t = precision == PK_PREC_MIXED ? 4.0 : 8.0;
    double b = 0.0;
-   b += units * 4.0 + sl * 2 * (8.0 + 4.0) + rows * 4.0;        // unit map, slice tables, rlen
+   b += units * 4.0 + ng * 2 * (8.0 + 4.0 + 4.0) + rows * 4.0;  // unit map, group tables, rlen
    b += rent * 64.0 * (4.0 + 8.0 + 8.0) + fent * 64.0 * 12.0;    // R cols+vals, A (FP64), F
    b += rows * 8.0 * 5;                                          // x, phi, y1, y2, r
    b += rows * t * (m + 1 + 5) + binv * 64.0 * t;                // basis, z z0 w e rhs, block inverses
@@ -1020,13 +1035,14 @@ static double pool_bytes(long units, long rent, long fent, long binv, int slots,
 
 static int pool_alloc(pk_batch_s* P) {
    Ctx& c = ctx();
-   const long units = P->max_units, rows = units * kRows, sl = rows / 64;
+   const long units = P->max_units, rows = units * kRows, sl = rows / 32;
    const int S = P->max_slots, m = P->m;
    const long rent = P->max_rentries * 64, fent = P->max_fentries * 64, binv = P->max_binv * 64;
    bool bad = P->unit_slot.alloc(units) || P->wg_first.alloc(S) || P->wg_count.alloc(S) || P->nrows.alloc(S) ||
               P->mlen.alloc(S) ||
               P->pbs.alloc(S) || P->fbs.alloc(S) || P->row_off.alloc(S) || P->binv_off.alloc(S) ||
               P->r_off.alloc(sl + 1) || P->f_off.alloc(sl + 1) || P->r_width.alloc(sl) || P->f_width.alloc(sl) ||
+              P->r_h.alloc(sl) || P->f_h.alloc(sl) ||
               P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
               P->f_col.alloc(fent) || P->f_val.alloc(fent) || P->x.alloc(rows) || P->phi.alloc(rows) ||
               P->y1.alloc(rows) || P->y2.alloc(rows) || P->r.alloc(rows) ||
@@ -1093,6 +1109,8 @@ static int pool_alloc(pk_batch_s* P) {
                          c.stream));
    PK_HIP(hipMemsetAsync(P->r_width.p, 0, sizeof(int) * sl, c.stream));
    PK_HIP(hipMemsetAsync(P->f_width.p, 0, sizeof(int) * sl, c.stream));
+   PK_HIP(hipMemsetAsync(P->r_h.p, 0, sizeof(int) * sl, c.stream));  // read only where a width is set
+   PK_HIP(hipMemsetAsync(P->f_h.p, 0, sizeof(int) * sl, c.stream));
    PK_HIP(hipMemsetAsync(P->r_off.p, 0, sizeof(long) * (sl + 1), c.stream));
    PK_HIP(hipMemsetAsync(P->f_off.p, 0, sizeof(long) * (sl + 1), c.stream));
    PK_HIP(hipMemsetAsync(P->rlen.p, 0, sizeof(int) * rows, c.stream));
@@ -1131,10 +1149,95 @@ static int check_member(pk_mat_s* R, pk_mat_s* F) {
    return 0;
 }
 
+// ---- autotuner: slice height and SpMV workgroup per size class ---------------------
+//
+// The first problem of a size class (rows, cell block, nonzeros, precision)
+// admitted to a pool times the sliced-ELL SpMV of its operator (FP32 values in
+// mixed precision, the inner solve's type) for slice heights 32, 64 and 128
+// and workgroups of 128, 256, 512 and 1024 rows, 1 + 20 launches each on the
+// library stream; later problems of the class take the stored choice.  The
+// slice height is chosen from the 256-row workgroups, the shape of the pool's
+// kernels (their workgroups stay 256 rows: a workgroup is also the block of
+// the preconditioner's sweep and the unit of the segmented reductions, so its
+// size is part of the arithmetic); the workgroup size, for the plain SpMV of
+// the matrix, from the chosen height's row.  A candidate replaces the default
+// (64 rows, 256 rows) only when it is at least 3 % faster: small operators
+// time at the launch latency, where the differences are noise.  A height
+// whose padded size exceeds that of the 64-row layout is skipped (pool
+// capacities are sized with that layout).  Padding entries add exact zeros:
+// every choice gives the same bits.  PK_TUNE=0 turns the tuner off;
+// PK_SLICE=32|64|128 forces a height.
+struct TuneKey {
+   long n, nnz;
+   int bs, precision;
+   bool operator<(const TuneKey& o) const {
+      if (n != o.n) return n < o.n;
+      if (nnz != o.nnz) return nnz < o.nnz;
+      if (bs != o.bs) return bs < o.bs;
+      return precision < o.precision;
+   }
+};
+struct TuneChoice {
+   int h = 64, block = 256;
+   double us[3][4] = {};  // [height 32/64/128][workgroup 128/256/512/1024], 0 = skipped
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tune;
+static std::vector<TuneKey> g_tune_order;
+
+static int env_int(const char* name, int dflt) {
+   const char* e = getenv(name);
+   return e && *e ? atoi(e) : dflt;
+}
+
+static int tune_operator(pk_mat_s* R, int precision) {
+   const int forced = env_int("PK_SLICE", 0);
+   if (forced) {
+      PK_REQUIRE(forced == 32 || forced == 64 || forced == 128, "PK_SLICE must be 32, 64 or 128");
+      if (R->sell.h != forced) PK_CALL(build_sell(R, forced));
+      if (R->sell.padded > R->sell.pad64) PK_CALL(build_sell(R, 64));
+      return 0;
+   }
+   if (!env_int("PK_TUNE", 1)) return 0;
+   static const int hs[3] = {32, 64, 128}, blocks[4] = {128, 256, 512, 1024};
+   const TuneKey key{R->n, R->nnz, R->bs, precision};
+   std::lock_guard<std::mutex> lock(g_tune_mu);
+   auto it = g_tune.find(key);
+   if (it == g_tune.end()) {
+      const bool f32 = precision == PK_PREC_MIXED;
+      TuneChoice ch;
+      const long pad64 = R->sell.pad64;
+      for (int a = 0; a < 3; a++) {
+         if (R->sell.h != hs[a]) PK_CALL(build_sell(R, hs[a]));
+         if (R->sell.padded > pad64) continue;
+         for (int q = 0; q < 4; q++) {
+            double s = 0.0;
+            PK_CALL(time_spmv(R, f32, blocks[q], 20, &s));
+            ch.us[a][q] = s * 1e6;
+         }
+      }
+      constexpr double kGain = 0.97;
+      int ah = 1;  // 64 rows
+      for (int a = 0; a < 3; a++)
+         if (ch.us[a][1] > 0.0 && ch.us[a][1] < kGain * ch.us[1][1] && ch.us[a][1] < ch.us[ah][1]) ah = a;
+      int qb = 1;  // 256 rows
+      for (int q = 0; q < 4; q++)
+         if (ch.us[ah][q] < kGain * ch.us[ah][1] && ch.us[ah][q] < ch.us[ah][qb]) qb = q;
+      ch.h = hs[ah];
+      ch.block = blocks[qb];
+      it = g_tune.emplace(key, ch).first;
+      g_tune_order.push_back(key);
+   }
+   if (R->sell.h != it->second.h) PK_CALL(build_sell(R, it->second.h));
+   R->sell.spmv_block = it->second.block;
+   return 0;
+}
+
 // 0: admitted into *slot; 2: no room now (nothing changed); 1: error
 static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_dev, int* slot) {
    Ctx& c = ctx();
    PK_CALL(check_member(R, F));
+   PK_CALL(tune_operator(R, P->precision));
    const long n = R->n;
    const int bs = R->bs, pbs = preconditioner_block(bs);
    const long units = units_of(n);
@@ -1185,14 +1288,14 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    S.nnz = R->sell.nnz;
    S.ticket = P->admitted++;
    const long row0 = u0 * kRows, padded = units * kRows;
-   const long slice0 = row0 / 64, nsl = padded / 64;
+   const long g0 = row0 / 32, ng = padded / 32;
    // SELL arrays of R and F, columns moved by the row offset
-   place_slices_kernel<<<grid_for(nsl), kBlock, 0, c.stream>>>(R->sell.nslices, nsl, R->sell.off.p,
-                                                              R->sell.width.p, slice0, r0 * 64, P->r_off.p,
-                                                              P->r_width.p);
-   place_slices_kernel<<<grid_for(nsl), kBlock, 0, c.stream>>>(F->sell.nslices, nsl, F->sell.off.p,
-                                                              F->sell.width.p, slice0, f0 * 64, P->f_off.p,
-                                                              P->f_width.p);
+   place_groups_kernel<<<grid_for(ng), kBlock, 0, c.stream>>>(R->sell.nslices, R->sell.h, ng, R->sell.off.p,
+                                                             R->sell.width.p, g0, r0 * 64, P->r_off.p,
+                                                             P->r_width.p, P->r_h.p);
+   place_groups_kernel<<<grid_for(ng), kBlock, 0, c.stream>>>(F->sell.nslices, F->sell.h, ng, F->sell.off.p,
+                                                             F->sell.width.p, g0, f0 * 64, P->f_off.p,
+                                                             P->f_width.p, P->f_h.p);
    if (R->sell.padded) {
       place_cols_kernel<<<grid_for(R->sell.padded), kBlock, 0, c.stream>>>(R->sell.padded, R->sell.col.p,
                                                                           (int)row0, P->r_col.p + r0 * 64);
@@ -1632,7 +1735,7 @@ int pk_batch_create(int num, const pk_mat* R, const pk_mat* F, pk_batch* out) {
          PK_FAIL(msg);
       }
       units += units_of(R[b]->n);
-      re += blocks64(R[b]->sell.padded);
+      re += blocks64(R[b]->sell.pad64);
       fe += blocks64(F[b]->sell.padded);
       bi += blocks64(R[b]->n * preconditioner_block(R[b]->bs));
    }
@@ -1661,7 +1764,7 @@ int pk_batch_set_member(pk_batch P, int index, pk_mat R, pk_mat F) {
       pk_mat_s* r = b == index ? R : P->fixR[b];
       pk_mat_s* f = b == index ? F : P->fixF[b];
       units += units_of(r->n);
-      re += blocks64(r->sell.padded);
+      re += blocks64(r->sell.pad64);
       fe += blocks64(f->sell.padded);
       bi += blocks64(r->n * preconditioner_block(r->bs));
    }
@@ -1830,7 +1933,7 @@ int pk_pool_fits(pk_batch P, pk_mat R, pk_mat F, int* fits) {
    PK_REQUIRE(P && fits && P->sd.p, "wrong pool arguments");
    PK_CALL(check_member(R, F));
    const long units = units_of(R->n);
-   const long re = blocks64(R->sell.padded), fe = blocks64(F->sell.padded);
+   const long re = blocks64(R->sell.pad64), fe = blocks64(F->sell.padded);
    const long bi = blocks64(R->n * preconditioner_block(R->bs));
    bool slot = false;
    for (auto& s : P->slots) slot |= !s.used;
@@ -1843,7 +1946,7 @@ int pk_pool_footprint(pk_mat R, pk_mat F, int precision, int restart, double* by
    PK_REQUIRE(bytes && restart >= 1 && restart <= kBMaxM, "wrong arguments");
    PK_CALL(check_member(R, F));
    const long units = units_of(R->n);
-   *bytes = pool_bytes(units, blocks64(R->sell.padded), blocks64(F->sell.padded),
+   *bytes = pool_bytes(units, blocks64(R->sell.pad64), blocks64(F->sell.padded),
                        blocks64(R->n * preconditioner_block(R->bs)), 1, restart, precision);
    return 0;
 }
@@ -1913,6 +2016,38 @@ int pk_pool_capacity(pk_batch P, long* free_rows, long* free_entries, int* free_
    for (auto& s : P->slots) fs += !s.used;
    if (free_slots) *free_slots = fs;
    if (ticks) *ticks = P->ticks;
+   return 0;
+}
+
+int pk_tune_count(int* n) {
+   PK_REQUIRE(n, "null argument");
+   std::lock_guard<std::mutex> lock(g_tune_mu);
+   *n = (int)g_tune_order.size();
+   return 0;
+}
+
+int pk_tune_get(int i, long* rows, int* block, long* nnz, int* precision, int* slice_height, int* spmv_block,
+                double* us) {
+   std::lock_guard<std::mutex> lock(g_tune_mu);
+   PK_REQUIRE(i >= 0 && i < (int)g_tune_order.size(), "no such tuned class");
+   const TuneKey& k = g_tune_order[i];
+   const TuneChoice& c = g_tune.at(k);
+   if (rows) *rows = k.n;
+   if (block) *block = k.bs;
+   if (nnz) *nnz = k.nnz;
+   if (precision) *precision = k.precision;
+   if (slice_height) *slice_height = c.h;
+   if (spmv_block) *spmv_block = c.block;
+   if (us)
+      for (int a = 0; a < 3; a++)
+         for (int q = 0; q < 4; q++) us[a * 4 + q] = c.us[a][q];
+   return 0;
+}
+
+int pk_tune_reset(void) {
+   std::lock_guard<std::mutex> lock(g_tune_mu);
+   g_tune.clear();
+   g_tune_order.clear();
    return 0;
 }
 

@@ -165,7 +165,10 @@ int reduce_dot_f(const float* x, const float* y, long n, double* out_dev);
 
 // Sliced-ELL (slice height 64 = one wavefront) device copy of a CSR matrix.
 struct SellData {
+   int h = 64;          // slice height (32, 64 or 128 rows)
+   int spmv_block = 256;  // workgroup size of the plain SpMV kernel (64 .. 1024)
    long nslices = 0, padded = 0;
+   long pad64 = 0;  // padded size of the 64-row layout: no layout in use is larger (pool capacities)
    long nnz = 0;  // unpadded nonzeros (the algorithmic count)
    pk::DBuf<long> off;
    pk::DBuf<int> width, col;
@@ -227,7 +230,8 @@ struct pk_xs_s {
 namespace pk {
 int spmv(pk_mat_s* A, const double* x, double* y);
 int spmv_f(pk_mat_s* A, const float* x, float* y);
-int build_sell(pk_mat_s* A);
+int build_sell(pk_mat_s* A, int h = 64);
+int time_spmv(pk_mat_s* A, bool f32, int block, int reps, double* seconds);
 int refresh_sell_values(pk_mat_s* A);
 int finalize_device_csr(pk_mat_s* A);
 }  // namespace pk

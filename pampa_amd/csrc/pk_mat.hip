@@ -3,12 +3,13 @@
 //
 // Device formats
 //   CSR        row_ptr/col/val, the exported pattern (bit-exact parity object).
-//   SELL-64    slices of 64 consecutive rows (one wavefront), each slice padded
-//              to its longest row and stored column-major inside the slice, so
-//              lane r of a wavefront reads element k of row (64 s + r) at
-//              sell_off[s] + 64 k + r: every load of col/val is one contiguous
-//              256-byte (FP32) or 512-byte (FP64) access.  Padding entries carry
-//              column = row and value 0 and are not counted as work.
+//   SELL-h     slices of h = 32, 64 (default) or 128 consecutive rows, each
+//              slice padded to its longest row and stored column-major inside
+//              the slice, so row (h s + r) reads element k at
+//              sell_off[s] + h k + r: the rows of a wavefront read contiguous
+//              runs of col/val.  Padding entries carry column = row and value 0
+//              and are not counted as work.  The slice height is picked per
+//              size class by the autotuner of the eigen pool (pk_batch.hip).
 #include <algorithm>
 #include <numeric>
 
@@ -20,15 +21,13 @@
 
 namespace pk {
 
-constexpr int kSlice = 64;
-
 __global__ void to_f32_kernel(const double* a, float* b, long n) {
    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (i < n) b[i] = (float)a[i];
 }
 
-// width of slice s = longest row among its 64 rows, as entries x 64
-__global__ void sell_width_kernel(long n, long ns, const int* ptr, int* width, long* wl) {
+// width of slice s = longest row among its h rows, as entries x h
+__global__ void sell_width_kernel(long n, long ns, int h, const int* ptr, int* width, long* wl) {
    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (s > ns) return;
    if (s == ns) {
@@ -36,26 +35,26 @@ __global__ void sell_width_kernel(long n, long ns, const int* ptr, int* width, l
       return;
    }
    int w = 0;
-   const long r1 = min(n, (s + 1) * (long)kSlice);
-   for (long r = s * kSlice; r < r1; r++) w = max(w, ptr[r + 1] - ptr[r]);
+   const long r1 = min(n, (s + 1) * (long)h);
+   for (long r = s * h; r < r1; r++) w = max(w, ptr[r + 1] - ptr[r]);
    width[s] = w;
-   wl[s] = (long)w * kSlice;
+   wl[s] = (long)w * h;
 }
 
-// one thread per row: its entries (or padding) at off[s] + 64 k + lane;
+// one thread per row: its entries (or padding) at off[s] + h k + lane;
 // with cols == false only the values are written (same pattern)
-__global__ void sell_fill_kernel(long n, long nrows_pad, const int* ptr, const int* col, const double* val,
-                                 const long* off, const int* width, int* scol, double* sval, int* rlen,
-                                 bool cols) {
+__global__ void sell_fill_kernel(long n, long nrows_pad, int h, const int* ptr, const int* col,
+                                 const double* val, const long* off, const int* width, int* scol, double* sval,
+                                 int* rlen, bool cols) {
    long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= nrows_pad) return;
-   const long s = row / kSlice;
-   const int lane = (int)(row % kSlice);
+   const long s = row / h;
+   const int lane = (int)(row % h);
    const int w = width[s];
    const long a = row < n ? ptr[row] : 0;
    const int len = row < n ? ptr[row + 1] - ptr[row] : 0;
    for (int k = 0; k < w; k++) {
-      const long p = off[s] + (long)k * kSlice + lane;
+      const long p = off[s] + (long)k * h + lane;
       if (k < len) {
          if (cols) scol[p] = col[a + k];
          sval[p] = val[a + k];
@@ -67,19 +66,21 @@ __global__ void sell_fill_kernel(long n, long nrows_pad, const int* ptr, const i
    if (cols && row < n) rlen[row] = len;
 }
 
-int build_sell(pk_mat_s* A) {
-   // device CSR -> SELL-64: slice widths, offsets by a scan, then the fill
+int build_sell(pk_mat_s* A, int h) {
+   // device CSR -> SELL-h: slice widths, offsets by a scan, then the fill
+   PK_REQUIRE(h == 32 || h == 64 || h == 128, "slice height must be 32, 64 or 128");
    Ctx& c = ctx();
    const long n = A->n;
-   const long ns = (n + kSlice - 1) / kSlice;
+   const long ns = (n + h - 1) / h;
    SellData& S = A->sell;
+   S.h = h;
    S.nslices = ns;
    S.nnz = A->nnz;
    DBuf<long> wl;
    PK_REQUIRE(!S.off.alloc(ns + 1) && !S.width.alloc(ns ? ns : 1) && !S.rlen.alloc(n ? n : 1) &&
                   !wl.alloc(ns + 1),
               "device allocation failed");
-   sell_width_kernel<<<grid_for(ns + 1), kBlock, 0, c.stream>>>(n, ns, A->ptr.p, S.width.p, wl.p);
+   sell_width_kernel<<<grid_for(ns + 1), kBlock, 0, c.stream>>>(n, ns, h, A->ptr.p, S.width.p, wl.p);
    PK_HIP(hipGetLastError());
    size_t tmp_bytes = 0;
    PK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, wl.p, S.off.p, (int)(ns + 1), c.stream));
@@ -90,11 +91,12 @@ int build_sell(pk_mat_s* A) {
    PK_HIP(hipMemcpyAsync(&tot, S.off.p + ns, sizeof(long), hipMemcpyDeviceToHost, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));
    S.padded = tot;
+   if (h == 64) S.pad64 = tot;
    PK_REQUIRE(!S.col.alloc(tot ? tot : 1) && !S.val.alloc(tot ? tot : 1), "device allocation failed");
    if (n) {
-      sell_fill_kernel<<<grid_for(ns * kSlice), kBlock, 0, c.stream>>>(n, ns * kSlice, A->ptr.p, A->col.p,
-                                                                     A->val.p, S.off.p, S.width.p, S.col.p,
-                                                                     S.val.p, S.rlen.p, true);
+      sell_fill_kernel<<<grid_for(ns * h), kBlock, 0, c.stream>>>(n, ns * h, h, A->ptr.p, A->col.p, A->val.p,
+                                                                S.off.p, S.width.p, S.col.p, S.val.p, S.rlen.p,
+                                                                true);
       PK_HIP(hipGetLastError());
    }
    PK_HIP(hipStreamSynchronize(c.stream));
@@ -107,9 +109,9 @@ int refresh_sell_values(pk_mat_s* A) {
    Ctx& c = ctx();
    SellData& S = A->sell;
    if (A->n) {
-      sell_fill_kernel<<<grid_for(S.nslices * kSlice), kBlock, 0, c.stream>>>(
-          A->n, S.nslices * kSlice, A->ptr.p, A->col.p, A->val.p, S.off.p, S.width.p, nullptr, S.val.p, nullptr,
-          false);
+      sell_fill_kernel<<<grid_for(S.nslices * S.h), kBlock, 0, c.stream>>>(
+          A->n, S.nslices * S.h, S.h, A->ptr.p, A->col.p, A->val.p, S.off.p, S.width.p, nullptr, S.val.p,
+          nullptr, false);
       PK_HIP(hipGetLastError());
    }
    if (S.val32.p) {
@@ -123,16 +125,15 @@ int refresh_sell_values(pk_mat_s* A) {
 // ---- SpMV kernels ------------------------------------------------------------
 
 template <class T>
-__global__ void __launch_bounds__(256) sell_spmv_kernel(long n, const long* __restrict__ off,
-                                                        const int* __restrict__ width,
-                                                        const int* __restrict__ col,
-                                                        const T* __restrict__ val,
-                                                        const T* __restrict__ x, T* __restrict__ y) {
-   long row = blockIdx.x * 256L + threadIdx.x;
-   long s = row >> 6;
-   int r = threadIdx.x & 63;
+__global__ void __launch_bounds__(1024) sell_spmv_kernel(long n, int h, const long* __restrict__ off,
+                                                         const int* __restrict__ width,
+                                                         const int* __restrict__ col,
+                                                         const T* __restrict__ val,
+                                                         const T* __restrict__ x, T* __restrict__ y) {
+   const long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= n) return;
-   y[row] = sell_row_dot<T, T>(col, val, x, off[s] + r, width[s], (T)0);
+   const long s = row / h;
+   y[row] = sell_row_dot<T, T>(col, val, x, off[s] + (row - s * h), width[s], (T)0, h);
 }
 
 
@@ -144,8 +145,8 @@ static double spmv_bytes(const pk_mat_s* A, double s) {
 int spmv(pk_mat_s* A, const double* x, double* y) {
    if (A->n == 0) return 0;
    SellData& S = A->sell;
-   PK_LAUNCH("spmv", spmv_bytes(A, 8.0), sell_spmv_kernel<double>, grid_for(A->n, 256), 256, ctx().stream,
-             A->n, S.off.p, S.width.p, S.col.p, S.val.p, x, y);
+   PK_LAUNCH("spmv", spmv_bytes(A, 8.0), sell_spmv_kernel<double>, grid_for(A->n, S.spmv_block), S.spmv_block,
+             ctx().stream, A->n, S.h, S.off.p, S.width.p, S.col.p, S.val.p, x, y);
    PK_HIP(hipGetLastError());
    return 0;
 }
@@ -154,9 +155,53 @@ int spmv_f(pk_mat_s* A, const float* x, float* y) {
    if (A->n == 0) return 0;
    PK_CALL(A->ensure_fp32());
    SellData& S = A->sell;
-   PK_LAUNCH("spmv_fp32", spmv_bytes(A, 4.0), sell_spmv_kernel<float>, grid_for(A->n, 256), 256, ctx().stream,
-             A->n, S.off.p, S.width.p, S.col.p, S.val32.p, x, y);
+   PK_LAUNCH("spmv_fp32", spmv_bytes(A, 4.0), sell_spmv_kernel<float>, grid_for(A->n, S.spmv_block),
+             S.spmv_block, ctx().stream, A->n, S.h, S.off.p, S.width.p, S.col.p, S.val32.p, x, y);
    PK_HIP(hipGetLastError());
+   return 0;
+}
+
+template <class T>
+__global__ void fill_kernel(T* x, long n, T v) {
+   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) x[i] = v;
+}
+
+// average device time of the plain SpMV with the matrix's current layout and a
+// workgroup of `block` rows (1 + reps launches on the library stream, x = 1)
+int time_spmv(pk_mat_s* A, bool f32, int block, int reps, double* seconds) {
+   Ctx& c = ctx();
+   SellData& S = A->sell;
+   const long n = A->n;
+   PK_REQUIRE(n > 0 && reps > 0 && block >= 64 && block <= 1024 && block % 64 == 0, "wrong arguments");
+   if (f32) PK_CALL(A->ensure_fp32());
+   const size_t t = f32 ? sizeof(float) : sizeof(double);
+   DBuf<char> x, y;
+   PK_REQUIRE(!x.alloc(n * t) && !y.alloc(n * t), "device allocation failed");
+   if (f32)
+      fill_kernel<float><<<grid_for(n), kBlock, 0, c.stream>>>((float*)x.p, n, 1.0f);
+   else
+      fill_kernel<double><<<grid_for(n), kBlock, 0, c.stream>>>((double*)x.p, n, 1.0);
+   hipEvent_t e0, e1;
+   PK_HIP(hipEventCreate(&e0));
+   PK_HIP(hipEventCreate(&e1));
+   for (int r = -1; r < reps; r++) {
+      if (r == 0) PK_HIP(hipEventRecord(e0, c.stream));
+      if (f32)
+         sell_spmv_kernel<float><<<grid_for(n, block), block, 0, c.stream>>>(
+             n, S.h, S.off.p, S.width.p, S.col.p, S.val32.p, (const float*)x.p, (float*)y.p);
+      else
+         sell_spmv_kernel<double><<<grid_for(n, block), block, 0, c.stream>>>(
+             n, S.h, S.off.p, S.width.p, S.col.p, S.val.p, (const double*)x.p, (double*)y.p);
+   }
+   PK_HIP(hipEventRecord(e1, c.stream));
+   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipGetLastError());
+   float ms = 0.f;
+   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   *seconds = ms * 1e-3 / reps;
    return 0;
 }
 

@@ -1,22 +1,25 @@
-// Row kernel of the SELL-64 layout (slices of 64 rows = one wavefront, each
-// slice stored column-major: lane r reads entry k of its row at base + 64 k).
+// Row kernel of the sliced-ELL layout (slices of h = 32, 64 or 128 rows, each
+// slice stored column-major: entry k of row r of a slice sits at the slice's
+// offset + (r mod h) + h k, so the rows of a wavefront read contiguous runs).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace pk {
 
-// sum_k a[p_k] x[col[p_k]] over the stored entries of one SELL-64 row
-// (p_k = base + 64 k, k < wd), in ascending k: four entries' loads are issued
-// before their products are summed, so several gathers are in flight per lane
+// sum_k a[p_k] x[col[p_k]] over the stored entries of one sliced-ELL row
+// (p_k = base + h k, k < wd), in ascending k: four entries' loads are issued
+// before their products are summed, so several gathers are in flight per lane.
+// Padding entries (value 0) add exact zeros, so the sum does not depend on
+// the slice height.
 template <class T, class A>
 __device__ __forceinline__ A sell_row_dot(const int* __restrict__ col, const T* __restrict__ val,
-                                          const T* __restrict__ x, long base, int wd, A acc) {
+                                          const T* __restrict__ x, long base, int wd, A acc, int h = 64) {
    int k = 0;
    for (; k + 4 <= wd; k += 4) {
-      const long p = base + (long)k * 64;
-      const int c0 = col[p], c1 = col[p + 64], c2 = col[p + 128], c3 = col[p + 192];
-      const T v0 = val[p], v1 = val[p + 64], v2 = val[p + 128], v3 = val[p + 192];
+      const long p = base + (long)k * h;
+      const int c0 = col[p], c1 = col[p + h], c2 = col[p + 2 * h], c3 = col[p + 3 * h];
+      const T v0 = val[p], v1 = val[p + h], v2 = val[p + 2 * h], v3 = val[p + 3 * h];
       const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
       acc += (A)(v0 * x0);
       acc += (A)(v1 * x1);
@@ -31,7 +34,7 @@ __device__ __forceinline__ A sell_row_dot(const int* __restrict__ col, const T* 
       T vv[3], xx[3];
 #pragma unroll
       for (int u = 0; u < 3; u++) {
-         const long p = base + (long)(k + min(u, rem - 1)) * 64;
+         const long p = base + (long)(k + min(u, rem - 1)) * h;
          cc[u] = col[p];
          vv[u] = val[p];
       }

@@ -597,6 +597,14 @@ int pk_lin_solve(pk_lin L, pk_vec b, pk_vec x) {
    PK_REQUIRE(L && b && x && b->n == L->A->n && x->n == L->A->n, "vector size mismatch");
    Ctx& c = ctx();
    long n = L->A->n;
+   // the solver's kernels read the 64-row slice layout; a matrix laid out
+   // otherwise (by the eigen pool's autotuner) is laid out again, and the
+   // solver set up again when its operator arrays moved
+   if (L->A->sell.h != 64) {
+      PK_CALL(build_sell(L->A, 64));
+      L->ready = false;
+   }
+   if (L->ready && L->g.Aval != L->A->sell.val.p) L->ready = false;
    if (!L->ready) {
       PK_REQUIRE(!L->r.alloc(n), "device allocation failed");
       // block-Jacobi blocks: the cell block when whole cells fit a workgroup,

@@ -192,3 +192,56 @@ def test_largest_size_classes_against_oracle_fixture(pk, tmp_path, precision):
         print(f"[case] instance {i} {precision}: dk={dk:.2e} pcm flux L2 (every {st}th entry) {l2:.2e}", flush=True)
         assert dk <= DK_PCM
         assert l2 <= FLUX_L2
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_slice_heights_give_the_same_bits(pk, members, monkeypatch, precision):
+    """The batch laid out with 32-, 64- and 128-row slices (PK_SLICE forces the
+    autotuner's choice) gives bit-identical eigenpairs: padding entries add
+    exact zeros and every row sums its entries in the same order."""
+    prec = pk.PK_PREC_FP64 if precision == "fp64" else pk.PK_PREC_MIXED
+    res = {}
+    for h in (64, 32, 128):
+        monkeypatch.setenv("PK_SLICE", str(h))
+        print(f"[case] batch of {len(members)} {precision}, slice height {h}", flush=True)
+        Rs, Fs = _mats(pk, members)
+        batch = pk.Batch(Rs, Fs, precision=prec)
+        batch.solve()
+        res[h] = [(k, x.get()) for k, x in (batch.eigenpair(b) for b in range(len(members)))]
+    for h in (32, 128):
+        for b, (name, _, _) in enumerate(members):
+            assert res[h][b][0] == res[64][b][0], (h, name)
+            assert np.array_equal(res[h][b][1], res[64][b][1]), (h, name)
+
+
+def test_autotuner_choices(pk, members, monkeypatch):
+    """The tuner times every (slice height, workgroup) candidate once per size
+    class; the height it keeps is no slower than 64 rows in 256-row
+    workgroups (the pool's shape), and the workgroup no slower than 256 rows
+    at that height.  A second batch of the same classes reuses the choices
+    (no new classes) and gives the same bits as the first."""
+    monkeypatch.delenv("PK_SLICE", raising=False)
+    monkeypatch.delenv("PK_TUNE", raising=False)
+    pk.tune_reset()
+    Rs, Fs = _mats(pk, members)
+    batch = pk.Batch(Rs, Fs, precision=pk.PK_PREC_MIXED)
+    batch.solve()
+    first = [(k, x.get()) for k, x in (batch.eigenpair(b) for b in range(len(members)))]
+    ch = pk.tune_choices()
+    classes = {(r.size()[0], r.size()[1], G) for r, (_, G, _) in zip(Rs, members)}
+    assert len(ch) == len(classes)
+    for c in ch:
+        print(f"   rows {c['rows']} G {c['block']}: slice height {c['slice_height']}, SpMV workgroup "
+              f"{c['spmv_block']}", flush=True)
+        assert c["slice_height"] in (32, 64, 128) and c["spmv_block"] in (128, 256, 512, 1024)
+        h, blk, us = c["slice_height"], c["spmv_block"], c["us"]
+        assert all(us[f"64x{b}"] > 0 for b in (128, 256, 512, 1024))
+        assert 0 < us[f"{h}x256"] <= us["64x256"]
+        assert us[f"{h}x{blk}"] <= us[f"{h}x256"]
+    Rs2, Fs2 = _mats(pk, members)
+    batch2 = pk.Batch(Rs2, Fs2, precision=pk.PK_PREC_MIXED)
+    batch2.solve()
+    assert len(pk.tune_choices()) == len(classes)
+    for b in range(len(members)):
+        k, x = batch2.eigenpair(b)
+        assert k == first[b][0] and np.array_equal(x.get(), first[b][1])
