@@ -189,7 +189,7 @@ __device__ __forceinline__ void sn_load_item(const SnRec<T>& R, const T* __restr
                                              SnItem<T, GT>& r) {
    r.tp = R.tp[it];
 #pragma unroll
-   for (int g = 0; g < GT; g++) r.qv[g] = qp[(long)it * G + g0 + g];
+   for (int g = 0; g < GT; g++) r.qv[g] = qp[(long)(g0 + g) * R.n + it];
 #pragma unroll
    for (int g = 0; g < GT; g++) r.den[g] = R.den[(long)(g0 + g) * R.n + it];
    if constexpr (sizeof(T) == 4) {
@@ -214,8 +214,8 @@ __device__ __forceinline__ void sn_load_item(const SnRec<T>& R, const T* __restr
 // lanes past the level's end repeat the level's last item (clamped index: a
 // wave past the end touches one cache line) and store exactly its values
 // again, so that every load and store is unconditional and the waits stay
-// precise.  The source qp and psi are in level order, so both are read and
-// written with contiguous accesses.
+// precise.  The source qp (group-major: one plane per group) and psi are in
+// level order, so both are read and written with contiguous accesses.
 template <class T, int GT, int NT, int P, int D>
 __device__ __forceinline__ void sn_level(const SnRec<T>& R, const int* lp, int l, int nl, int G, int g0, int tid,
                                          const SnItem<T, GT> (&cur)[P], SnItem<T, GT> (&pre)[P],
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(NT) sn_sweep_lds_any_kernel(int G, int lstride
          }
          T* o = psi + (long)it * G;
          for (int g = 0; g < G; g++) {
-            const T v = (qp[(long)it * G + g] - inflow[g]) / R.den[(long)g * R.n + it];
+            const T v = (qp[(long)g * R.n + it] - inflow[g]) / R.den[(long)g * R.n + it];
             o[g] = v;
             if (os >= 0) slots[(long)os * G + g] = v;
          }
@@ -357,18 +357,26 @@ __global__ void __launch_bounds__(NT) sn_sweep_lds_any_kernel(int G, int lstride
    }
 }
 
-// qp[p][.] = q[i][.] for the item (m, i) at position p: the source in the
-// sweep's level order.  An item's G values are copied as cpi chunks of type V
-// (16, 8 or 4 bytes: the widest that divides G sizeof(T)), one thread a chunk.
-template <class V>
-__global__ void __launch_bounds__(kBlock) sn_permute_kernel(const int* __restrict__ items, long n, int C, int cpi,
-                                                            const V* __restrict__ q, V* __restrict__ qp) {
-   const long l = blockIdx.x * (long)blockDim.x + threadIdx.x;
-   if (l >= n * cpi) return;
-   const long it = l / cpi;
-   const int c = (int)(l - it * cpi);
-   const int i = items[it] % C;
-   qp[l] = q[(long)i * cpi + c];
+// qp[g][p] = q[i][g] for the item (m, i) at position p: the source in the
+// sweep's level order, one plane per group, so that a workgroup sweeping
+// group g reads its items' sources as one contiguous run.  One thread per
+// item: the item's G values are read together (one gather) and written to
+// the G planes (coalesced across the threads).
+template <class T, int GT>
+__global__ void __launch_bounds__(kBlock) sn_permute_kernel(const int* __restrict__ items, long n, int C, int G_,
+                                                            const T* __restrict__ q, T* __restrict__ qp) {
+   const int G = GT > 0 ? GT : G_;
+   const long it = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (it >= n) return;
+   const long i = items[it] % C;
+   constexpr int GM = GT > 0 ? GT : kMaxSnGroups;
+   T v[GM];
+#pragma unroll
+   for (int g = 0; g < GM; g++)
+      if (GT > 0 || g < G) v[g] = q[i * G + g];
+#pragma unroll
+   for (int g = 0; g < GM; g++)
+      if (GT > 0 || g < G) qp[(long)g * n + it] = v[g];
 }
 
 // phiw[i][g] = sum_m w_m psi(m, i, g) (FP64 accumulation, m ascending); the
@@ -1007,17 +1015,26 @@ static int sn_sweep(pk_sn_s* S, const T* sigt, const T* q, T* psi) {
    T* qp = f32 ? (T*)S->qp32.p : (T*)S->qp64.p;
    {
       const long n = (long)S->M * S->C;
-      const int bytes = S->G * (int)sizeof(T);
-      const double traffic = profiling() ? (double)n * bytes * 2 + n * 4.0 : 0.0;
-      if (bytes % 16 == 0)
-         PK_LAUNCH("sn_permute", traffic, sn_permute_kernel<uint4>, grid_for(n * (bytes / 16)), kBlock, c.stream,
-                   S->ditems.p, n, S->C, bytes / 16, (const uint4*)q, (uint4*)qp);
-      else if (bytes % 8 == 0)
-         PK_LAUNCH("sn_permute", traffic, sn_permute_kernel<uint2>, grid_for(n * (bytes / 8)), kBlock, c.stream,
-                   S->ditems.p, n, S->C, bytes / 8, (const uint2*)q, (uint2*)qp);
-      else
-         PK_LAUNCH("sn_permute", traffic, sn_permute_kernel<unsigned>, grid_for(n * (bytes / 4)), kBlock, c.stream,
-                   S->ditems.p, n, S->C, bytes / 4, (const unsigned*)q, (unsigned*)qp);
+      const int G = S->G;
+      const double traffic = profiling() ? (double)n * G * sizeof(T) * 2 + n * 4.0 : 0.0;
+      switch (G) {
+         case 1:
+            PK_LAUNCH("sn_permute", traffic, (sn_permute_kernel<T, 1>), grid_for(n), kBlock, c.stream, S->ditems.p,
+                      n, S->C, G, q, qp);
+            break;
+         case 2:
+            PK_LAUNCH("sn_permute", traffic, (sn_permute_kernel<T, 2>), grid_for(n), kBlock, c.stream, S->ditems.p,
+                      n, S->C, G, q, qp);
+            break;
+         case 4:
+            PK_LAUNCH("sn_permute", traffic, (sn_permute_kernel<T, 4>), grid_for(n), kBlock, c.stream, S->ditems.p,
+                      n, S->C, G, q, qp);
+            break;
+         default:
+            PK_LAUNCH("sn_permute", traffic, (sn_permute_kernel<T, 0>), grid_for(n), kBlock, c.stream, S->ditems.p,
+                      n, S->C, G, q, qp);
+            break;
+      }
    }
    int done = -1;
    if (B.fast) {
