@@ -111,6 +111,8 @@ def cpu_baseline(d, target_seconds=15.0):
     """The oracle (FP64 CPU restatement: reference-form assembly, exact sparse LU
     + restarted Arnoldi shift-invert, power normalisation) timed on a bounded
     sample of the same workload, single thread."""
+    import scipy.linalg  # noqa: F401  (load every BLAS the solve uses, so that the limit below covers it)
+    import scipy.sparse.linalg  # noqa: F401
     import oracle
     from threadpoolctl import threadpool_limits
     cwd = os.getcwd()
@@ -132,6 +134,8 @@ def cpu_baseline(d, target_seconds=15.0):
 
 def _cpu_worker(args):
     d, seconds = args
+    import scipy.linalg  # noqa: F401  (load every BLAS the solve uses, so that the limit below covers it)
+    import scipy.sparse.linalg  # noqa: F401
     import oracle
     from threadpoolctl import threadpool_limits
     os.chdir(d)
@@ -284,6 +288,8 @@ def _oracle_request_sn(oracle, W, path, seed, rel):
 
 def _oracle_request_job(args):
     path, seed, rel = args
+    import scipy.linalg  # noqa: F401  (load every BLAS the solve uses, so that the limit below covers it)
+    import scipy.sparse.linalg  # noqa: F401
     import oracle
     from pampa_amd import workloads as W
     from threadpoolctl import threadpool_limits

@@ -277,10 +277,20 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       n0++;
    }
    if (n == 0) {
-      // DiffusionSolver::buildMatrices + EPS solve (DiffusionSolver.cxx:88-350)
-      PK_CHECK(pk_mat_destroy(&R));
-      PK_CHECK(pk_mat_destroy(&F));
-      PK_CHECK(pk_diffusion_assemble(dmesh, dxs, nb, bc_type.data(), bc_value.data(), &R, &F));
+      // DiffusionSolver::buildMatrices + EPS solve (DiffusionSolver.cxx:88-350):
+      // the values of the previous steady assembly recomputed in place when the
+      // boundary table is the same (cross sections and temperatures may differ),
+      // else a new assembly
+      if (steady_ops && R && F && bc_type == steady_bct && bc_value == steady_bcv) {
+         PK_CHECK(pk_diffusion_refill(dmesh, dxs, R, F));
+      } else {
+         PK_CHECK(pk_mat_destroy(&R));
+         PK_CHECK(pk_mat_destroy(&F));
+         PK_CHECK(pk_diffusion_assemble(dmesh, dxs, nb, bc_type.data(), bc_value.data(), &R, &F));
+         steady_ops = true;
+         steady_bct = bc_type;
+         steady_bcv = bc_value;
+      }
       // the reference recreates the EPS for every steady solve
       // (NeutronicSolver.cxx:14-19); here the solver keeps its device memory and
       // captured graphs across solves of the same problem and takes the new R, F
@@ -314,6 +324,7 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       for (const auto& m : materials) beta.push_back(m.beta_total);
       PK_CHECK(pk_mat_destroy(&R));
       PK_CHECK(pk_mat_destroy(&F));
+      steady_ops = false;
       PK_CHECK(pk_diffusion_assemble_transient(dmesh, dxs, nb, bc_type.data(), bc_value.data(),
                                                vel.data(), dt, chip.data(), chid.data(), beta.data(),
                                                keff, S, phi0, &R, b));
@@ -350,6 +361,7 @@ int DiffusionSolver::finalize() {
    PK_CHECK(pk_eig_destroy(&eps));
    PK_CHECK(pk_mat_destroy(&R));
    PK_CHECK(pk_mat_destroy(&F));
+   steady_ops = false;
    PK_CHECK(pk_vec_destroy(&b));
    return NeutronicSolver::finalize();
 }

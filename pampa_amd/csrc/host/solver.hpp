@@ -158,6 +158,11 @@ class DiffusionSolver : public NeutronicSolver {
    pk_mat R = nullptr, F = nullptr;
    pk_vec b = nullptr;
    pk_eig eps = nullptr;  // kept across steady solves of this problem
+   // R and F hold a steady assembly with this boundary table: the next steady
+   // solve recomputes their values in place (same pattern and layout)
+   bool steady_ops = false;
+   std::vector<int> steady_bct;
+   std::vector<double> steady_bcv;
 };
 
 class SNSolver : public NeutronicSolver {

@@ -82,6 +82,17 @@ __device__ __forceinline__ SellRef sell_ref(const BSell& A, long row) {
    return SellRef{A.off[g] + (row & 31), A.width[g], A.h[g]};
 }
 
+// The same for a row of the calling wavefront's own 64 rows (row = 64 w +
+// lane): the wavefront's two group entries are loaded once as uniform values
+// (scalar loads, issued early), each lane picks its half.
+__device__ __forceinline__ SellRef sell_ref_wave(const BSell& A, long row) {
+   const long g0 = 2L * __builtin_amdgcn_readfirstlane((int)(row >> 6));
+   const long off0 = A.off[g0], off1 = A.off[g0 + 1];
+   const int w0 = A.width[g0], w1 = A.width[g0 + 1], h0 = A.h[g0], h1 = A.h[g0 + 1];
+   const bool hi = (row & 32) != 0;
+   return SellRef{(hi ? off1 : off0) + (row & 31), hi ? w1 : w0, hi ? h1 : h0};
+}
+
 struct BState {  // per-slot state (arrays over slots)
    int m;
    // GMRES
@@ -174,7 +185,7 @@ __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BS
    if (b < 0 || !S.rebuild[b]) return;
    const double inv_ks = S.ks[b] > 0.0 ? 1.0 / S.ks[b] : 0.0;
    const long local = row - L.row_off[b];
-   const SellRef ra = sell_ref(R, row), fa = sell_ref(F, row);
+   const SellRef ra = sell_ref_wave(R, row), fa = sell_ref_wave(F, row);
    const int fb = L.fbs[b];
    const long cell = local / fb;
    const int len = R.rlen[row];
@@ -267,7 +278,7 @@ __global__ void __launch_bounds__(kRows) brf_kernel(BLayout L, BSell R, const do
    const int b = slot_of_wg(L);
    if (b < 0 || S.phase[b] != kOuter) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const SellRef ra = sell_ref(R, row), fa = sell_ref(F, row);
+   const SellRef ra = sell_ref_wave(R, row), fa = sell_ref_wave(F, row);
    const double a = sell_row_dot<double, double>(R.col, rval, x, ra.base, ra.wd, 0.0, ra.h);
    const double c = sell_row_dot<double, double>(F.col, fval, x, fa.base, fa.wd, 0.0, fa.h);
    y1[row] = a;
@@ -397,7 +408,7 @@ __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* bin
    __syncthreads();
    const long row0 = blockIdx.x * (long)kRows;
    const long row = row0 + threadIdx.x;
-   const SellRef ar = sell_ref(A, row);
+   const SellRef ar = sell_ref_wave(A, row);
    const int wd = ar.wd;
    T acc = 0;
    // entries 4 at a time: their loads first, then the sum in order
@@ -563,7 +574,7 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
           1, [&](int) { return vj * vj; }, part, tickets + b, gtickets + L.wg_first[b], hd + j + 1,
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    } else {
-      const SellRef ar = sell_ref(A, row);
+      const SellRef ar = sell_ref_wave(A, row);
       const T acc = sell_row_dot<T, T>(A.col, aval, z, ar.base, ar.wd, (T)0, ar.h);
       w[row] = acc;
       const double wr = (double)acc;
@@ -743,7 +754,7 @@ __device__ __forceinline__ void poly_step(const BLayout& L, const BSell& A, cons
    if (mode == 0) count_act(L, b, act, 5, 1, -1, 0);
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const SellRef ar = sell_ref(A, row);
+   const SellRef ar = sell_ref_wave(A, row);
    const T az = sell_row_dot<T, T>(A.col, aval, z0, ar.base, ar.wd, (T)0, ar.h);
    const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
    const T zr = z0[row] + c;
@@ -798,7 +809,7 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    if (b < 0 || S.phase[b] != kInner || (S.flags[4 * b] && S.flags[4 * b + 1] == 0)) return;
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const SellRef ar = sell_ref(A, row);
+   const SellRef ar = sell_ref_wave(A, row);
    const T acc = sell_row_dot<T, T>(A.col, aval, e, ar.base, ar.wd, (T)0, ar.h);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
@@ -835,7 +846,7 @@ __global__ void __launch_bounds__(kRows) bspmv_kernel(BLayout L, BSell A, const 
    const int b = slot_of_wg(L);
    if (b < 0) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const SellRef ar = sell_ref(A, row);
+   const SellRef ar = sell_ref_wave(A, row);
    y[row] = sell_row_dot<T, T>(A.col, aval, x, ar.base, ar.wd, (T)0, ar.h);
 }
 
