@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 SPMV_DOT_TRAFFIC_BYTES = 4.079e6
 # the same counter passes over `bench.py --batched-pass` (256 instances in one mixed pool),
 # HBM bytes per launch of each GMRES step kernel
-BATCHED_TRAFFIC_BYTES = {"gmres_spmv_dot": 508.7e6, "gmres_orth": 729.5e6, "gmres_poly": 952.2e6}
+BATCHED_TRAFFIC_BYTES = {"gmres_spmv_dot": 509.9e6, "gmres_orth": 726.3e6, "gmres_poly": 948.8e6}
 METRIC = ("eigen solves/s + p50/p99 latency, mixed vs FP64 at |dk|<=1pcm; sweep/SpMV HBM GB/s")
 
 
