@@ -201,71 +201,96 @@ __global__ void bshift_kernel(long N, BLayout L, BSell R, const double* rval, BS
    }
 }
 
+// Inverse of each cell block (bs x bs, bs a power of two <= 16, so a block's
+// rows are bs consecutive lanes of one wavefront): lane r of the group holds
+// row r of [A_cell | I] in registers, and Gauss-Jordan with partial pivoting
+// runs across the group's lanes (pivot search, row exchange and elimination
+// by cross-lane shuffles; the pivot is the first row of largest magnitude).
+// Launched with one wavefront per 64 rows.
 template <class T>
-__global__ void bblock_inverse_kernel(long N, BLayout L, BSell A, const double* aval,
-                                      const int* rebuild, T* binv, int* err) {
-   long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64) bblock_inverse_kernel(long N, BLayout L, BSell A, const double* aval,
+                                                            const int* rebuild, T* binv, int* err) {
+   const long row = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (row >= N) return;
    const int b = slot_of_row(L, row);
    if (b < 0 || !rebuild[b]) return;
    const long local = row - L.row_off[b];
+   if (local >= L.nrows[b]) return;  // whole blocks: nrows is a multiple of bs
    const int bs = L.pbs[b];
-   if (local >= L.nrows[b] || local % bs != 0) return;
+   const int r = (int)(local & (bs - 1));
    const long cell = local / bs;
-   double a[kBMaxBs * kBMaxBs], m[kBMaxBs * kBMaxBs];
-   for (int p = 0; p < bs * bs; p++) {
-      a[p] = 0.0;
-      m[p] = 0.0;
+   const int base = (int)(threadIdx.x & 63) - r;  // lane of the block's first row
+   double a[kBMaxBs], m[kBMaxBs];
+#pragma unroll
+   for (int q = 0; q < kBMaxBs; q++) {
+      a[q] = 0.0;
+      m[q] = q == r ? 1.0 : 0.0;
    }
-   for (int r = 0; r < bs; r++) {
-      m[r * bs + r] = 1.0;
-      const long rr = row + r;
-      const SellRef ar = sell_ref(A, rr);
-      for (int k = 0; k < A.rlen[rr]; k++) {
-         long p = ar.base + (long)k * ar.h;
-         long c = (long)A.col[p] - L.row_off[b];
-         if (c / bs == cell) a[r * bs + (c % bs)] += aval[p];
-      }
-   }
-   // Gauss-Jordan with partial pivoting
-   for (int k = 0; k < bs; k++) {
-      int piv = k;
-      double best = fabs(a[k * bs + k]);
-      for (int r = k + 1; r < bs; r++)
-         if (fabs(a[r * bs + k]) > best) {
-            best = fabs(a[r * bs + k]);
-            piv = r;
-         }
-      if (best == 0.0) {
-         *err = 1;
-         return;
-      }
-      if (piv != k)
-         for (int q = 0; q < bs; q++) {
-            double t = a[k * bs + q];
-            a[k * bs + q] = a[piv * bs + q];
-            a[piv * bs + q] = t;
-            t = m[k * bs + q];
-            m[k * bs + q] = m[piv * bs + q];
-            m[piv * bs + q] = t;
-         }
-      double inv = 1.0 / a[k * bs + k];
-      for (int q = 0; q < bs; q++) {
-         a[k * bs + q] *= inv;
-         m[k * bs + q] *= inv;
-      }
-      for (int r = 0; r < bs; r++) {
-         if (r == k) continue;
-         double f = a[r * bs + k];
-         if (f == 0.0) continue;
-         for (int q = 0; q < bs; q++) {
-            a[r * bs + q] -= f * a[k * bs + q];
-            m[r * bs + q] -= f * m[k * bs + q];
+   {
+      const SellRef ar = sell_ref(A, row);
+      for (int k = 0; k < A.rlen[row]; k++) {
+         const long p = ar.base + (long)k * ar.h;
+         const long c = (long)A.col[p] - L.row_off[b];
+         if (c / bs == cell) {
+            const int q0 = (int)(c - cell * bs);
+            const double v = aval[p];
+#pragma unroll
+            for (int q = 0; q < kBMaxBs; q++)
+               if (q == q0) a[q] += v;
          }
       }
    }
-   T* dst = binv + L.binv_off[b] + cell * bs * bs;
-   for (int p = 0; p < bs * bs; p++) dst[p] = (T)m[p];
+   bool singular = false;
+#pragma unroll
+   for (int k = 0; k < kBMaxBs; k++) {
+      if (k >= bs) break;
+      // pivot: the first row >= k of largest |a[k]|
+      double best = r >= k ? fabs(a[k]) : -1.0;
+      int piv = r;
+      for (int o = 1; o < bs; o <<= 1) {
+         const double ob = __shfl_xor(best, o, 64);
+         const int op = __shfl_xor(piv, o, 64);
+         if (ob > best || (ob == best && op < piv)) {
+            best = ob;
+            piv = op;
+         }
+      }
+      if (best == 0.0) singular = true;
+      // exchange rows k and piv, then scale row k
+      const int src = base + (r == k ? piv : (r == piv ? k : r));
+#pragma unroll
+      for (int q = 0; q < kBMaxBs; q++) {
+         a[q] = __shfl(a[q], src, 64);
+         m[q] = __shfl(m[q], src, 64);
+      }
+      if (r == k && !singular) {
+         const double inv = 1.0 / a[k];
+#pragma unroll
+         for (int q = 0; q < kBMaxBs; q++) {
+            a[q] *= inv;
+            m[q] *= inv;
+         }
+      }
+      // eliminate column k from the other rows
+      const double f = a[k];
+#pragma unroll
+      for (int q = 0; q < kBMaxBs; q++) {
+         const double pa = __shfl(a[q], base + k, 64);
+         const double pm = __shfl(m[q], base + k, 64);
+         if (r != k && f != 0.0) {
+            a[q] -= f * pa;
+            m[q] -= f * pm;
+         }
+      }
+   }
+   if (singular) {
+      if (r == 0) *err = 1;
+      return;
+   }
+   T* dst = binv + L.binv_off[b] + cell * bs * bs + (long)r * bs;
+#pragma unroll
+   for (int q = 0; q < kBMaxBs; q++)
+      if (q < bs) dst[q] = (T)m[q];
 }
 
 // ---- outer step ------------------------------------------------------------------------
