@@ -71,6 +71,25 @@ def test_dot_norm_deterministic(pk):
         assert abs(nr.value - np.linalg.norm(a)) <= 1e-12 * np.linalg.norm(a)
 
 
+def test_grid_reduction_visibility(pk):
+    """The cross-workgroup reductions publish each workgroup's partials with
+    relaxed system-scope stores, wait for them, then take a ticket; the last
+    workgroup reads them with system-scope loads.  Forty reductions in a row
+    over 1,024 workgroups (the reduction grid's cap), each of new vectors (a
+    partial read before it is visible would be the previous reduction's
+    value), all equal the host's FP64 sums."""
+    import ctypes as ct
+    L = pk.lib()
+    rng = np.random.default_rng(11)
+    n = 4096 * 256
+    for it in range(40):
+        a, b = rng.normal(size=n), rng.uniform(0.5, 1.5, size=n)
+        A, B = pk.Vec(n, a), pk.Vec(n, b)
+        r = ct.c_double()
+        pk.check(L.pk_vec_dot(A.h, B.h, ct.byref(r)))
+        assert abs(r.value - a @ b) <= 1e-12 * (np.abs(a) @ b), it
+
+
 def test_linear_solver(pk):
     """GMRES + cell-block Jacobi on a 2-group diffusion-like operator."""
     import ctypes as ct
