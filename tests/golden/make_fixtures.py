@@ -11,9 +11,12 @@ inside a test run (tests/golden/*.npz), in this build container:
                        normalised) of the oracle's FP64 source iteration.
 
 The eigen solves are the oracle's (oracle/__init__.py): exact sparse LU plus
-restarted Arnoldi (shift-invert) for the diffusion instances, except 256x256x16
-(1,048,576 rows) whose LU does not finish here: its shift-invert Arnoldi applies
-R^-1 by FP64 GMRES to a 1e-13 relative residual instead.  Run:
+restarted Arnoldi (shift-invert) for the diffusion instances; for 256x256x16
+(1,048,576 rows), whose LU with SuperLU's own column orderings does not
+finish here, the same shift-invert Arnoldi on an LU of R with its cells in
+nested-dissection order (recursive bisection of the cell grid, separator
+lines last, a cell's groups together).  The file is rewritten after every
+instance.  Run:
     python tests/golden/make_fixtures.py [batched] [lattice]
 """
 import os
@@ -44,28 +47,46 @@ def _instance_of(side, G):
     raise RuntimeError("class not drawn")
 
 
-def _sinvert_iterative(R, F, G, tol=1e-13):
-    """Shift-invert Arnoldi on R^-1 F with R^-1 applied by GMRES (cell-block
-    Jacobi preconditioner), for operators whose LU does not fit."""
-    import scipy.sparse as sp
+def _nd_cells(side):
+    """Nested-dissection order of a side x side cell grid (cell = y side + x):
+    halves first, then the separator line, down to blocks of 64 cells."""
+    order = []
+
+    def rec(x0, x1, y0, y1):
+        nx, ny = x1 - x0, y1 - y0
+        if nx * ny <= 64:
+            order.extend(y * side + x for y in range(y0, y1) for x in range(x0, x1))
+            return
+        if nx >= ny:
+            m = (x0 + x1) // 2
+            rec(x0, m, y0, y1)
+            rec(m + 1, x1, y0, y1)
+            order.extend(y * side + m for y in range(y0, y1))
+        else:
+            m = (y0 + y1) // 2
+            rec(x0, x1, y0, m)
+            rec(x0, x1, m + 1, y1)
+            order.extend(m * side + x for x in range(x0, x1))
+
+    rec(0, side, 0, side)
+    return np.array(order)
+
+
+def _sinvert_nd(R, F, side, G, tol=1e-13):
+    """eigen_sinvert with the LU of R taken in nested-dissection order of the
+    cells (symmetric permutation, diagonal pivots)."""
     import scipy.sparse.linalg as sla
+    cells = _nd_cells(side)
+    assert len(cells) == side * side and len(np.unique(cells)) == side * side
+    p = (cells[:, None] * G + np.arange(G)[None, :]).reshape(-1)
+    inv = np.empty_like(p)
+    inv[p] = np.arange(len(p))
     Rs = R.scipy().tocsr()
     Fs = F.scipy().tocsr()
-    n = Rs.shape[0]
-    blocks = [np.linalg.inv(Rs[i:i + G, i:i + G].toarray()) for i in range(0, n, G)]
-    Binv = sp.block_diag(blocks).tocsr()
-    M = sla.LinearOperator(Rs.shape, matvec=lambda v: Binv @ v)
-    x_prev = [None]
-
-    def apply(v):
-        b = Fs @ v
-        x, info = sla.gmres(Rs, b, x0=x_prev[0], M=M, rtol=tol, atol=0.0, restart=60, maxiter=2000)
-        if info != 0:
-            raise RuntimeError(f"GMRES did not converge ({info})")
-        return x
-
-    op = sla.LinearOperator(Rs.shape, matvec=apply, dtype=np.float64)
-    vals, vecs = sla.eigs(op, k=1, which="LM", tol=1e-12, v0=np.ones(n), ncv=20)
+    lu = sla.splu(Rs[p][:, p].tocsc(), permc_spec="NATURAL", diag_pivot_thresh=0.0,
+                  options=dict(SymmetricMode=True))
+    op = sla.LinearOperator(Rs.shape, matvec=lambda v: lu.solve((Fs @ v)[p])[inv], dtype=np.float64)
+    vals, vecs = sla.eigs(op, k=1, which="LM", tol=tol, v0=np.ones(R.n), ncv=20)
     x = vecs[:, 0].real.copy()
     return float(vals[0].real), (x if x.sum() > 0 else -x)
 
@@ -83,8 +104,8 @@ def batched():
         bct, bcv = oracle.bc_table(mesh, s.bcs)
         R, F = oracle.assemble_diffusion(mesh, xs, bct, bcv)
         t0 = time.time()
-        if side * side * G > 600_000 and G == 16:
-            k, x = _sinvert_iterative(R, F, G)
+        if side * side * G > 600_000:
+            k, x = _sinvert_nd(R, F, side, G)
         else:
             k, x = oracle.eigen_sinvert(R, F, permc="COLAMD")
         phi, _, _ = oracle.normalize_flux(mesh, xs, x, s.power)
@@ -92,8 +113,8 @@ def batched():
               flush=True)
         out[f"inst{i}_keff"] = np.array(k)
         out[f"inst{i}_phi"] = phi[::STRIDE].copy()
-    out["stride"] = np.array(STRIDE)
-    np.savez_compressed(os.path.join(HERE, "batched_classes.npz"), **out)
+        out["stride"] = np.array(STRIDE)
+        np.savez_compressed(os.path.join(HERE, "batched_classes.npz"), **out)
 
 
 def lattice():
