@@ -694,3 +694,56 @@ def sn_sweep_eigen(mesh: Mesh, xs: XS, s: pmp.SolverInput, inner=4, max_outer=50
                          _p(coef), _p(xs.sigt), _p(xs.sigs), _p(xs.nusf), _p(xs.chi), _p(order),
                          inner, max_outer, tol_k, tol_phi, ct.byref(k), _p(psi), _p(phiw))
     return k.value, psi, phiw, its
+
+
+def sn_sweep_eigen_anderson(mesh: Mesh, xs: XS, s: pmp.SolverInput, depth=5, max_outer=5000,
+                            tol_k=1e-12, tol_phi=1e-10):
+    """The fixed point of orc_sn_eigen's iteration (inner = 1: source from the
+    moment, one sweep, k from the production ratio, the moment rescaled to the
+    first production) reached with Anderson mixing of depth `depth` (Walker and
+    Ni 2011, least squares over the last residual differences), for lattices
+    whose plain iteration needs thousands of sweeps.  The same orc_sn_source
+    and orc_sn_sweep do the work.  Stops when |dk| < tol_k and the relative L2
+    change of the moment < tol_phi.  Returns (keff, phiw, sweeps)."""
+    L = lib()
+    dirs, wts, refl, coef, _, _ = sn_setup(mesh, s)
+    order, _, _ = sn_order(mesh, dirs)
+    G, M = xs.G, len(wts)
+    n = mesh.C * G
+    nus = xs.nusf[mesh.cmat].reshape(-1) * np.repeat(mesh.vol, G)
+    psi = np.zeros(n * M)
+    q = np.zeros(n)
+    phi = np.ones(n)
+    p0 = float(nus @ phi)
+    k = 1.0
+    dF, dG = [], []
+    f_prev = g_prev = None
+    for it in range(1, max_outer + 1):
+        L.orc_sn_source(mesh.C, G, _p(mesh.vol), _p(mesh.cmat), _p(xs.sigs), _p(xs.nusf), _p(xs.chi),
+                        1.0 / k, _p(phi), _p(q))
+        g = np.zeros(n)
+        L.orc_sn_sweep(mesh.C, mesh.fmax, _p(mesh.nfaces), _p(mesh.vol), _p(mesh.cmat), _p(mesh.farea),
+                       _p(mesh.fnorm), _p(mesh.fnb), G, M, _p(dirs), _p(wts), _p(coef), _p(xs.sigt),
+                       _p(order), _p(q), _p(psi), _p(g))
+        p = float(nus @ g)
+        knew = k * p / p0
+        g *= p0 / p
+        f = g - phi
+        if abs(knew - k) < tol_k and np.linalg.norm(f) < tol_phi * np.linalg.norm(g):
+            return knew, g, it
+        k = knew
+        if f_prev is not None and depth > 0:
+            dF.append(f - f_prev)
+            dG.append(g - g_prev)
+            if len(dF) > depth:
+                dF.pop(0)
+                dG.pop(0)
+        f_prev, g_prev = f, g
+        if dF:
+            A = np.stack(dF, axis=1)
+            gam, *_ = np.linalg.lstsq(A, f, rcond=None)
+            phi = g - np.stack(dG, axis=1) @ gam
+        else:
+            phi = g
+        phi *= p0 / float(nus @ phi)
+    raise RuntimeError("Anderson-mixed source iteration did not converge")
