@@ -1811,8 +1811,12 @@ int pk_batch_set_member(pk_batch P, int index, pk_mat R, pk_mat F) {
    return 0;
 }
 
+// The options reach the tick's kernels as launch arguments: a change drops the
+// captured tick (the next tick captures it again); setting the same values
+// keeps it, so a caller that sets its options before every solve reuses it.
 int pk_batch_set_tolerances(pk_batch P, double tol, int max_outer) {
    PK_REQUIRE(P && tol > 0.0 && max_outer > 0, "wrong tolerances");
+   if (tol == P->opts.tol && max_outer == P->opts.max_outer) return 0;
    P->opts.tol = tol;
    P->opts.max_outer = max_outer;
    P->graph.clear();
@@ -1830,6 +1834,7 @@ int pk_batch_set_inner(pk_batch P, int restart, double reduction, int max_inner)
    PK_REQUIRE(P && restart >= 1 && restart <= kBMaxM && reduction > 0.0 && reduction < 1.0 && max_inner > 0,
               "wrong inner-solver options");
    PK_REQUIRE(!P->sd.p || restart == P->m, "the restart length must be set before the first solve");
+   if (restart == P->m && reduction == P->opts.reduction && max_inner == P->opts.max_inner) return 0;
    P->m = restart;
    P->opts.reduction = reduction;
    P->opts.max_inner = max_inner;
@@ -1839,6 +1844,7 @@ int pk_batch_set_inner(pk_batch P, int restart, double reduction, int max_inner)
 
 int pk_batch_set_preconditioner(pk_batch P, int poly) {
    PK_REQUIRE(P && (poly == 0 || poly == 1 || poly == 2), "wrong preconditioner");
+   if ((poly == 1) == (P->opts.poly != 0) && (poly != 2) == (P->opts.sweep != 0)) return 0;
    P->opts.poly = poly == 1;
    P->opts.sweep = poly != 2;
    P->graph.clear();
@@ -1847,6 +1853,7 @@ int pk_batch_set_preconditioner(pk_batch P, int poly) {
 
 int pk_batch_set_shift(pk_batch P, double delta) {
    PK_REQUIRE(P && delta >= 0.0, "wrong shift");
+   if (delta == P->opts.delta) return 0;
    P->opts.delta = delta;
    P->graph.clear();
    return 0;
