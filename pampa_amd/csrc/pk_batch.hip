@@ -106,6 +106,7 @@ struct BState {  // per-slot state (arrays over slots)
    double* resid;  // [S][2]: estimate, target
    double* ww;     // [S]
    int* flags;     // [S][4]: stop, steps in cycle, breakdown, steps in this solve
+   int* jcur;      // [S] the cycle's next step (0 at the cycle start); a cycle may span ticks
    // outer
    double* dots;   // [S][3]: <Fx,Rx>, <Fx,Fx>, <x,x>
    double* rq;     // [S][2]: |r|^2, |F phi/k|^2
@@ -122,6 +123,7 @@ struct BState {  // per-slot state (arrays over slots)
    int* inner;     // [S] GMRES steps in all corrections
    int* status;    // [S] 0 running, 1 converged, 2 failed
    int* slow;      // [S] consecutive shifted outer updates that reduced the residual by less than 0.8
+   int* klast;     // [S][2] GMRES steps of the last two inner solves (the host sizes ticks from them)
 };
 
 struct BOpts {
@@ -486,6 +488,7 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, BSell A, const
    fl[1] = 0;
    fl[2] = 0;
    fl[3] = 0;
+   S.jcur[b] = 0;
    S.start[b] = 0;
 }
 
@@ -557,10 +560,11 @@ __device__ __forceinline__ void count_act(const BLayout& L, int b, long long* ac
 
 template <class T>
 __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
-                                              T* w, const T* V, int j, double* partials, unsigned* tickets, unsigned* gtickets,
+                                              T* w, const T* V, double* partials, unsigned* tickets, unsigned* gtickets,
                                               const BState& S, long long* act) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
+   const int j = S.jcur[b];  // every workgroup reads it before the reduction; the epilogue advances it
    const int m = S.m;
    const double* vs = S.vs + (long)b * (m + 1);
    __shared__ double hd[kBMaxM + 2];
@@ -651,6 +655,7 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
       }
       sc[3] = vsj;
       stop_sh = stop;
+      S.jcur[b] = j + 1;
    }
    __syncthreads();
    // the rotated column jj, then (unless the cycle stops) column j
@@ -663,11 +668,11 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
 
 template <class T>
 __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSell A, const T* aval,
-                                                          const T* z, T* w, const T* V, int j,
+                                                          const T* z, T* w, const T* V,
                                                           double* partials, unsigned* tickets, unsigned* gtickets, BState S,
                                                           long long* act, unsigned long long* stamps) {
    const unsigned long long t0 = stamp_begin(stamps);
-   spmv_dot_step<T>(N, L, A, aval, z, w, V, j, partials, tickets, gtickets, S, act);
+   spmv_dot_step<T>(N, L, A, aval, z, w, V, partials, tickets, gtickets, S, act);
    stamp_end(stamps, t0);
 }
 
@@ -676,9 +681,10 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
 // its norm is reduced by the next step (spmv_dot_step)
 template <class T>
 __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* w, T* V,
-                                          T* z, const T* binv, int j, const BState& S, long long* act) {
+                                          T* z, const T* binv, const BState& S, long long* act) {
    const int b = slot_of_wg(L);
    if (!gmres_on(S, b)) return;
+   const int j = S.jcur[b] - 1;  // the step whose SpMV and dots ran last
    count_act(L, b, act, 3, 1, 4, j + 4);
    __shared__ T tl[kRows], xl[kRows];
    const int m = S.m;
@@ -705,10 +711,10 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell&
 
 template <class T>
 __global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, BSell A, const T* aval, const T* w, T* V,
-                                                      T* z, const T* binv, int j, BState S, long long* act,
+                                                      T* z, const T* binv, BState S, long long* act,
                                                       unsigned long long* stamps) {
    const unsigned long long t0 = stamp_begin(stamps);
-   orth_step<T>(N, L, A, aval, w, V, z, binv, j, S, act);
+   orth_step<T>(N, L, A, aval, w, V, z, binv, S, act);
    stamp_end(stamps, t0);
 }
 
@@ -719,7 +725,7 @@ __global__ void __launch_bounds__(kRows) borth_kernel(long N, BLayout L, BSell A
 // subtract its contribution (column order: a fixed order of operations).
 __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) {
    const int b = blockIdx.x;
-   if (b >= nslots || S.phase[b] != kInner) return;
+   if (b >= nslots || S.phase[b] != kInner || !S.flags[4 * b]) return;  // mid-cycle slots go on
    const int m = S.m;
    const int k = S.flags[4 * b + 1];
    if (k == 0) return;
@@ -756,27 +762,36 @@ __global__ void __launch_bounds__(kRows) btrisolve_kernel(int nslots, BState S) 
 
 // Second term of the preconditioner M^-1 = S (2 I - A S):
 // z = z0 + S (v - A z0), z0 = S v given.  mode 0: slots iterating in
-// this GMRES step; mode 1: slots whose cycle took steps (the update: also
-// e += z).
+// this GMRES step, v = V_j at the slot's step j (v points at the basis, N apart);
+// mode 1: slots whose cycle stopped after taking steps (the update: also e += z).
 template <class T>
-__device__ __forceinline__ void poly_step(const BLayout& L, const BSell& A, const T* aval, const T* v, const T* z0,
-                                          const T* binv, T* z, const BState& S, int mode, T* e, long long* act);
+__device__ __forceinline__ void poly_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* v,
+                                          const T* z0, const T* binv, T* z, const BState& S, int mode, T* e,
+                                          long long* act);
 
 template <class T>
-__global__ void __launch_bounds__(kRows) bpoly_kernel(BLayout L, BSell A, const T* aval, const T* v, const T* z0,
-                                                      const T* binv, T* z, BState S, int mode, T* e, long long* act,
-                                                      unsigned long long* stamps) {
+__global__ void __launch_bounds__(kRows) bpoly_kernel(long N, BLayout L, BSell A, const T* aval, const T* v,
+                                                      const T* z0, const T* binv, T* z, BState S, int mode, T* e,
+                                                      long long* act, unsigned long long* stamps) {
    const unsigned long long t0 = stamp_begin(stamps);
-   poly_step<T>(L, A, aval, v, z0, binv, z, S, mode, e, act);
+   poly_step<T>(N, L, A, aval, v, z0, binv, z, S, mode, e, act);
    stamp_end(stamps, t0);
 }
 
 template <class T>
-__device__ __forceinline__ void poly_step(const BLayout& L, const BSell& A, const T* aval, const T* v, const T* z0,
-                                          const T* binv, T* z, const BState& S, int mode, T* e, long long* act) {
+__device__ __forceinline__ void poly_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* v,
+                                          const T* z0, const T* binv, T* z, const BState& S, int mode, T* e,
+                                          long long* act) {
    const int b = slot_of_wg(L);
-   if (mode == 0 ? !gmres_on(S, b) : (b < 0 || S.phase[b] != kInner || S.flags[4 * b + 1] == 0)) return;
-   if (mode == 0) count_act(L, b, act, 5, 1, -1, 0);
+   if (mode == 0) {
+      if (!gmres_on(S, b)) return;
+      const int j = S.jcur[b];
+      if (j >= L.mlen[b]) return;  // the column past the cycle's last: norm only
+      v += (long)j * N;
+      count_act(L, b, act, 5, 1, -1, 0);
+   } else if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b] || S.flags[4 * b + 1] == 0) {
+      return;
+   }
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const SellRef ar = sell_ref_wave(A, row);
@@ -794,7 +809,7 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, BSell
                                                         const T* binv, BState S, T* e, int accumulate, T* uout,
                                                         T* zout) {
    const int b = slot_of_wg(L);
-   if (b < 0 || S.phase[b] != kInner) return;
+   if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b]) return;  // mid-cycle slots go on
    const int k = S.flags[4 * b + 1];
    if (k == 0) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
@@ -829,9 +844,10 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
                                                          const T* rhs, T* V0, T* z, const T* binv, BOpts o,
                                                          double* partials, unsigned* tickets, unsigned* gtickets, BState S) {
    const int b = slot_of_wg(L);
-   // slots not in the inner phase, or stopped before this cycle took a step,
-   // keep their flags
-   if (b < 0 || S.phase[b] != kInner || (S.flags[4 * b] && S.flags[4 * b + 1] == 0)) return;
+   // only slots whose cycle stopped after taking steps: the others are mid-cycle
+   // (it goes on next tick) or stopped before this cycle took a step (their
+   // flags stay)
+   if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b] || S.flags[4 * b + 1] == 0) return;
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const SellRef ar = sell_ref_wave(A, row);
@@ -853,6 +869,7 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    fl[0] = (beta <= S.resid[2 * b + 1] || !(beta > 0.0) || fl[3] >= o.max_inner) ? 1 : 0;
    fl[1] = 0;
    fl[2] = 0;
+   S.jcur[b] = 0;
 }
 
 // x = phi + e for slots whose inner solve ended
@@ -881,15 +898,25 @@ __global__ void bphase_kernel(int nslots, BState S) {
    S.rebuild[b] = 0;  // consumed by this tick's shift and block inverses
    if (S.phase[b] != kInner || !S.flags[4 * b]) return;
    S.inner[b] += S.flags[4 * b + 3];
+   S.klast[2 * b + 1] = S.klast[2 * b];
+   S.klast[2 * b] = S.flags[4 * b + 3];
    S.phase[b] = kOuter;
 }
 
-// Every slot's phase and the error flag into pinned host memory (one
-// workgroup), for a host that reads them after an event instead of a copy.
+// Every slot's phase, cycle position (next step, steps of the inner solve's
+// earlier cycles) and the longer of its last two inner solves, then the error
+// flag, into pinned host memory (one workgroup; kStatus values per slot), for a
+// host that reads them after an event instead of a copy.
+constexpr int kStatus = 4;
 __global__ void __launch_bounds__(kRows) bstatus_kernel(int nslots, BState S, const int* err, int* host) {
-   for (int b = threadIdx.x; b < nslots; b += kRows)
-      __hip_atomic_store(host + b, S.phase[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-   if (threadIdx.x == 0) __hip_atomic_store(host + nslots, *err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   for (int b = threadIdx.x; b < nslots; b += kRows) {
+      const int v[kStatus] = {S.phase[b], S.jcur[b], S.flags[4 * b + 3],
+                              max(S.klast[2 * b], S.klast[2 * b + 1])};
+      for (int q = 0; q < kStatus; q++)
+         __hip_atomic_store(host + kStatus * b + q, v[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+   }
+   if (threadIdx.x == 0)
+      __hip_atomic_store(host + kStatus * nslots, *err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace pk
@@ -965,15 +992,14 @@ struct Inner {
    DBuf<T> V, z, z0, w, e, rhs, binv, aval;
 };
 
-struct Graph {
-   hipGraphExec_t exec = nullptr;
-   long units = -1;
+// The captured tick: one executable per (grid, step launches).
+struct Graphs {
+   std::map<std::pair<long, int>, hipGraphExec_t> exec;
    void clear() {
-      if (exec) (void)hipGraphExecDestroy(exec);
-      exec = nullptr;
-      units = -1;
+      for (auto& e : exec) (void)hipGraphExecDestroy(e.second);
+      exec.clear();
    }
-   ~Graph() { clear(); }
+   ~Graphs() { clear(); }
 };
 
 }  // namespace
@@ -1007,7 +1033,7 @@ struct pk_batch_s {
    std::vector<int> h_unit_slot;
    std::vector<int> h_phase;
    long admitted = 0;
-   Graph graph;
+   Graphs graph;
    // pinned read-back of the phases and the error flag (two sets in turn) and
    // their events, for the fixed-batch run
    int* h_status = nullptr;
@@ -1096,7 +1122,7 @@ static int pool_alloc(pk_batch_s* P) {
       PK_REQUIRE(!alloc_inner(P->i64, false), "device allocation failed for the eigen pool");
    // per-slot state
    const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 6 * (size_t)m + 28);
-   const size_t ni = (size_t)S * 13;
+   const size_t ni = (size_t)S * 16;
    PK_REQUIRE(!P->sd.alloc(nd) && !P->si.alloc(ni), "device allocation failed for the eigen pool");
    PK_HIP(hipMemsetAsync(P->sd.p, 0, sizeof(double) * nd, c.stream));
    PK_HIP(hipMemsetAsync(P->si.p, 0, sizeof(int) * ni, c.stream));
@@ -1139,6 +1165,8 @@ static int pool_alloc(pk_batch_s* P) {
    B.inner = takei(S);
    B.status = takei(S);
    B.slow = takei(S);
+   B.jcur = takei(S);
+   B.klast = takei(2 * (size_t)S);
    // tables: every unit free, every slice empty
    P->h_unit_slot.assign(units, -1);
    PK_HIP(hipMemcpyAsync(P->unit_slot.p, P->h_unit_slot.data(), sizeof(int) * units, hipMemcpyHostToDevice,
@@ -1379,6 +1407,8 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    const double dl0 = P->opts.delta;
    PK_HIP(hipMemcpyAsync(P->S.dl + sidx, &dl0, sizeof(double), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.slow + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.jcur + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
+   PK_HIP(hipMemcpyAsync(P->S.klast + 2 * (long)sidx, zi, 2 * sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.rebuild + sidx, &one, sizeof(int), hipMemcpyHostToDevice, c.stream));
    PK_HIP(hipMemcpyAsync(P->S.phase + sidx, &outer, sizeof(int), hipMemcpyHostToDevice, c.stream));
    // the slot's units become live last
@@ -1470,11 +1500,14 @@ Inner<float>& inner_of<float>(pk_batch_s* P) {
    return P->i32;
 }
 
-// Launches one tick on the stream for row units [0, units).  With `prof`,
-// the GMRES step kernels carry profiler events and activity counters (the
-// same kernels, arguments and launches as the captured graph).
+// Launches one tick on the stream for row units [0, units): the outer part,
+// `steps` GMRES step launches (1 .. m + 1; each slot takes the next step of its
+// cycle, and a cycle longer than the tick goes on in the next one), then the
+// end of the cycles that stopped.  With `prof`, the GMRES step kernels carry
+// profiler events and activity counters (the same kernels, arguments and
+// launches as the captured graph).
 template <class T>
-static int launch_tick(pk_batch_s* P, long units, bool prof) {
+static int launch_tick(pk_batch_s* P, long units, int steps, bool prof) {
    Ctx& c = ctx();
    Inner<T>& I = inner_of<T>(P);
    const BLayout L = P->layout();
@@ -1506,11 +1539,14 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    T* zfirst = o.poly ? I.z0.p : I.z.p;
    bstart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, P->r.p, I.rhs.p, I.e.p, I.V.p, zfirst, I.binv.p, o,
                                                  part, tk, gtk, S);
-   // steps 0 .. m-1, then the column past the last (j = m: the norm of V_m
-   // that finishes column m-1 of the slots still iterating).  With `prof`
-   // the step kernels are launched with events that record their own start
-   // and end, and count each slot's activity (algorithmic bytes, summed by the
-   // host afterwards); every step is issued, as in the captured graph.
+   // `steps` step launches; a slot at step j of its cycle runs the
+   // preconditioner, the SpMV and dots, and the orthogonalisation of step j,
+   // or at j = mlen only the norm of V_mlen that finishes column mlen - 1.  At
+   // launch m every slot is at its cycle's last column or stopped, so only the
+   // SpMV/dot kernel is issued there.  With `prof` the step kernels are
+   // launched with events that record their own start and end, and count each
+   // slot's activity (algorithmic bytes, summed by the host afterwards); every
+   // step is issued, as in the captured graph.
    long long* act = prof ? P->act.p : nullptr;
    unsigned long long* st = prof ? P->stamps.p : nullptr;
    // the span of a profiled launch into the next record
@@ -1521,19 +1557,18 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
       P->rec_kind[P->nrec++] = which;
       return 0;
    };
-   for (int j = 0; j <= m; j++) {
-      const bool full = j < m;
+   for (int q = 0; q < steps; q++) {
+      const bool full = q < m;
       if (o.poly && full) {
-         bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.V.p + (size_t)j * N, I.z0.p, I.binv.p, I.z.p,
-                                                      S, 0, nullptr, act, st);
+         bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.z0.p, I.binv.p, I.z.p, S, 0, nullptr,
+                                                      act, st);
          PK_CALL(span(2));
       }
-      bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, j, part, tk, gtk, S,
-                                                       act, st);
+      bspmv_dot_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.z.p, I.w.p, I.V.p, part, tk, gtk, S, act,
+                                                       st);
       PK_CALL(span(0));
       if (full) {
-         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, j, S, act,
-                                                      st);
+         borth_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.V.p, zfirst, I.binv.p, S, act, st);
          PK_CALL(span(1));
       }
    }
@@ -1541,7 +1576,7 @@ static int launch_tick(pk_batch_s* P, long units, bool prof) {
    if (o.poly) {
       // e += M^-1 u: u and S u, then the second term, then the sum
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 0, I.w.p, I.z0.p);
-      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1, I.e.p,
+      bpoly_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.w.p, I.z0.p, I.binv.p, I.z.p, S, 1, I.e.p,
                                                    nullptr, nullptr);
    } else {
       bupdate_kernel<T><<<nwg, kRows, 0, c.stream>>>(N, L, Rs, aval, I.V.p, I.binv.p, S, I.e.p, 1, nullptr,
@@ -1567,7 +1602,7 @@ static bool graphs_disabled() {
 }
 
 template <class T>
-static int run_tick(pk_batch_s* P, long units) {
+static int run_tick(pk_batch_s* P, long units, int steps) {
    Ctx& c = ctx();
    if (profiling()) {
       if (!P->act.p) {
@@ -1582,28 +1617,30 @@ static int run_tick(pk_batch_s* P, long units) {
          P->nrec = 0;
          PK_HIP(hipMemsetAsync(P->act.p, 0, sizeof(long long) * P->max_slots * kAct, c.stream));
       }
-      return launch_tick<T>(P, units, true);
+      return launch_tick<T>(P, units, steps, true);
    }
-   if (graphs_disabled()) return launch_tick<T>(P, units, false);
-   if (P->graph.exec == nullptr || P->graph.units != units) {
+   if (graphs_disabled()) return launch_tick<T>(P, units, steps, false);
+   const std::pair<long, int> key(units, steps);
+   auto it = P->graph.exec.find(key);
+   if (it == P->graph.exec.end()) {
       // every quantity that varies between ticks is read from device memory,
-      // so one graph serves every tick for this grid size
-      P->graph.clear();
+      // so one graph serves every tick of this grid and length
       hipGraph_t g = nullptr;
       PK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      int rc = launch_tick<T>(P, units, false);
+      int rc = launch_tick<T>(P, units, steps, false);
       hipError_t e = hipStreamEndCapture(c.stream, &g);
       if (rc) {
          if (g) (void)hipGraphDestroy(g);
          return rc;
       }
       PK_HIP(e);
-      hipError_t ie = hipGraphInstantiate(&P->graph.exec, g, nullptr, nullptr, 0);
+      hipGraphExec_t x = nullptr;
+      hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       PK_HIP(ie);
-      P->graph.units = units;
+      it = P->graph.exec.emplace(key, x).first;
    }
-   PK_HIP(hipGraphLaunch(P->graph.exec, c.stream));
+   PK_HIP(hipGraphLaunch(it->second, c.stream));
    return 0;
 }
 
@@ -1619,9 +1656,9 @@ static int pool_step(pk_batch_s* P, int ticks, int* live, int* done) {
    if (nlive > 0 && units > 0)
       for (int t = 0; t < ticks; t++) {
          if (P->precision == PK_PREC_MIXED)
-            PK_CALL(run_tick<float>(P, units));
+            PK_CALL(run_tick<float>(P, units, P->m + 1));
          else
-            PK_CALL(run_tick<double>(P, units));
+            PK_CALL(run_tick<double>(P, units, P->m + 1));
          P->ticks++;
       }
    int herr = 0;
@@ -1670,42 +1707,91 @@ static int slot_result(pk_batch_s* P, int sidx, SlotResult& R, pk_vec_s* x) {
 
 // ---- fixed batch: every member admitted at once and run to the end ------------------
 
+// Step launches tick t + 1 needs, from every live slot's state after tick
+// t - 1 (st: kStatus values per slot) and the launches lt of tick t, assuming
+// that each inner solve takes as many GMRES steps as the longer of the slot's
+// last two (a whole cycle while none is known).  A cycle that stops after c
+// columns is seen by its launch c; a slot whose cycle ends in tick t starts its
+// next cycle (a restart, or a new inner solve after the outer update) in tick
+// t + 1.  The most any slot needs, plus two, at most m + 1.  Only the tick
+// length depends on it: a cycle longer than its tick goes on in the next.
+static int predict_steps(const pk_batch_s* P, const int* st, int lt) {
+   constexpr int kMargin = 2;
+   const int m1 = P->m + 1;
+   int need = 1;
+   for (int b = 0; b < P->max_slots; b++) {
+      const Slot& sl = P->slots[b];
+      const int* v = st + kStatus * b;
+      if (!sl.used || (v[0] != kOuter && v[0] != kInner)) continue;
+      const int mlen = sl.mlen, K = v[3];
+      if (K <= 0) return m1;
+      // an outer slot starts a new inner solve in tick t
+      int j = v[0] == kOuter ? 0 : v[1];
+      int done = v[0] == kOuter ? 0 : v[2];
+      auto stop = [&](int d) { return std::min(mlen, std::max(1, K - d)); };
+      // the column the current cycle stops at: not before its next launch
+      const int c = std::min(mlen, std::max(stop(done), j));
+      const int left = c - j + 1;  // launches to the end of the current cycle
+      int n;
+      if (lt < left) {
+         n = left - lt;
+      } else {
+         done += c;
+         n = stop(done >= K ? 0 : done) + 1;
+      }
+      need = std::max(need, n + kMargin);
+   }
+   return std::min(need, m1);
+}
+
 // Ticks run one ahead of the host's read of the slots' phases: tick t + 1 is
 // queued before the host waits for tick t's phases (a tick over slots that are
 // all DONE changes nothing), so the device does not idle during the read-back.
-// The phases come back through pinned memory, two sets in turn.
+// The phases come back through pinned memory, two sets in turn, with what the
+// length of the next tick is predicted from.
 static int fixed_run(pk_batch_s* P) {
    Ctx& c = ctx();
    const int ns = P->max_slots;
    const long units = P->ua.high_water();
    if (units <= 0) return 0;
+   const int set = kStatus * ns + 1;
    if (!P->h_status) {
-      PK_HIP(hipHostMalloc((void**)&P->h_status, sizeof(int) * 2 * (ns + 1)));
+      PK_HIP(hipHostMalloc((void**)&P->h_status, sizeof(int) * 2 * set));
       for (auto& e : P->status_ev) PK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
    }
    auto t0 = std::chrono::steady_clock::now();
-   auto issue = [&](int k) -> int {
-      int* h = P->h_status + k * (ns + 1);
-      for (int q = 0; q <= ns; q++) h[q] = -1;
+   auto issue = [&](int k, int steps) -> int {
+      int* h = P->h_status + k * set;
+      for (int q = 0; q < set; q++) h[q] = -1;
       if (P->precision == PK_PREC_MIXED)
-         PK_CALL(run_tick<float>(P, units));
+         PK_CALL(run_tick<float>(P, units, steps));
       else
-         PK_CALL(run_tick<double>(P, units));
+         PK_CALL(run_tick<double>(P, units, steps));
       P->ticks++;
       bstatus_kernel<<<1, kRows, 0, c.stream>>>(ns, P->S, P->err.p, h);
       PK_HIP(hipGetLastError());
       PK_HIP(hipEventRecord(P->status_ev[k], c.stream));
       return 0;
    };
-   PK_CALL(issue(0));
+   const int m1 = P->m + 1;
+   // PK_TICK_STEPS=n runs every tick with n step launches (1 .. m + 1; the same
+   // bits for every n; for tests)
+   const int forced = std::min(env_int("PK_TICK_STEPS", 0), m1);
+   std::vector<int> st;  // the last status read
+   int last_steps = forced > 0 ? forced : m1;  // launches of the last tick queued
+   PK_CALL(issue(0, last_steps));
    for (int cur = 0;; cur ^= 1) {
-      PK_CALL(issue(cur ^ 1));
+      const int steps = forced > 0 ? forced : st.empty() ? m1 : predict_steps(P, st.data(), last_steps);
+      PK_CALL(issue(cur ^ 1, steps));
+      last_steps = steps;
       PK_HIP(hipEventSynchronize(P->status_ev[cur]));
-      const int* h = P->h_status + cur * (ns + 1);
-      int herr = __atomic_load_n(&h[ns], __ATOMIC_ACQUIRE);
+      const int* h = P->h_status + cur * set;
+      st.resize(set);
+      for (int q = 0; q < set; q++) st[q] = __atomic_load_n(&h[q], __ATOMIC_ACQUIRE);
+      const int herr = st[kStatus * ns];
       int nl = 0;
       for (int q = 0; q < ns; q++) {
-         const int ph = __atomic_load_n(&h[q], __ATOMIC_ACQUIRE);
+         const int ph = st[kStatus * q];
          P->h_phase[q] = ph;
          if (P->slots[q].used && ph != kDone) nl++;
       }
@@ -1741,20 +1827,17 @@ static int time_pool_spmv(pk_batch_s* P, int reps, double* avg_seconds) {
    Inner<T>& I = inner_of<T>(P);
    const T* aval = sizeof(T) == sizeof(double) ? (const T*)P->a_val.p : I.aval.p;
    const long units = P->ua.high_water();
-   hipEvent_t e0, e1;
-   PK_HIP(hipEventCreate(&e0));
-   PK_HIP(hipEventCreate(&e1));
+   EventPair ev;
+   PK_HIP(ev.create());
    // one untimed launch, then `reps` between the events
    bspmv_kernel<T><<<(int)units, kRows, 0, c.stream>>>(P->layout(), P->rsell(), aval, I.z.p, I.w.p);
-   PK_HIP(hipEventRecord(e0, c.stream));
+   PK_HIP(hipEventRecord(ev.e0, c.stream));
    for (int r = 0; r < reps; r++)
       bspmv_kernel<T><<<(int)units, kRows, 0, c.stream>>>(P->layout(), P->rsell(), aval, I.z.p, I.w.p);
-   PK_HIP(hipEventRecord(e1, c.stream));
-   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipEventRecord(ev.e1, c.stream));
+   PK_HIP(hipGetLastError());
    float ms = 0.f;
-   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
-   (void)hipEventDestroy(e0);
-   (void)hipEventDestroy(e1);
+   PK_HIP(ev.ms(&ms));
    *avg_seconds = ms * 1e-3 / reps;
    return 0;
 }

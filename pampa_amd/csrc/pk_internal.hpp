@@ -154,6 +154,27 @@ struct DBuf {
    }
 };
 
+// Two timing events on one stream, destroyed with the scope (error returns
+// included).  ms() waits for the second one.
+struct EventPair {
+   hipEvent_t e0 = nullptr, e1 = nullptr;
+   EventPair() = default;
+   EventPair(const EventPair&) = delete;
+   EventPair& operator=(const EventPair&) = delete;
+   ~EventPair() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+   }
+   hipError_t create() {
+      const hipError_t e = hipEventCreate(&e0);
+      return e != hipSuccess ? e : hipEventCreate(&e1);
+   }
+   hipError_t ms(float* out) {
+      const hipError_t e = hipEventSynchronize(e1);
+      return e != hipSuccess ? e : hipEventElapsedTime(out, e0, e1);
+   }
+};
+
 // ---- deterministic reductions ----------------------------------------------
 // Sum over n of f(i) for `nv` values at once; result written to out[0..nv-1]
 // (device memory).  Fixed work split and fixed summation order, so the result

@@ -182,11 +182,10 @@ int time_spmv(pk_mat_s* A, bool f32, int block, int reps, double* seconds) {
       fill_kernel<float><<<grid_for(n), kBlock, 0, c.stream>>>((float*)x.p, n, 1.0f);
    else
       fill_kernel<double><<<grid_for(n), kBlock, 0, c.stream>>>((double*)x.p, n, 1.0);
-   hipEvent_t e0, e1;
-   PK_HIP(hipEventCreate(&e0));
-   PK_HIP(hipEventCreate(&e1));
+   EventPair ev;
+   PK_HIP(ev.create());
    for (int r = -1; r < reps; r++) {
-      if (r == 0) PK_HIP(hipEventRecord(e0, c.stream));
+      if (r == 0) PK_HIP(hipEventRecord(ev.e0, c.stream));
       if (f32)
          sell_spmv_kernel<float><<<grid_for(n, block), block, 0, c.stream>>>(
              n, S.h, S.off.p, S.width.p, S.col.p, S.val32.p, (const float*)x.p, (float*)y.p);
@@ -194,13 +193,10 @@ int time_spmv(pk_mat_s* A, bool f32, int block, int reps, double* seconds) {
          sell_spmv_kernel<double><<<grid_for(n, block), block, 0, c.stream>>>(
              n, S.h, S.off.p, S.width.p, S.col.p, S.val.p, (const double*)x.p, (double*)y.p);
    }
-   PK_HIP(hipEventRecord(e1, c.stream));
-   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipEventRecord(ev.e1, c.stream));
    PK_HIP(hipGetLastError());
    float ms = 0.f;
-   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
-   (void)hipEventDestroy(e0);
-   (void)hipEventDestroy(e1);
+   PK_HIP(ev.ms(&ms));
    *seconds = ms * 1e-3 / reps;
    return 0;
 }
@@ -349,38 +345,34 @@ int pk_mat_set_block_size(pk_mat M, int bs) {
 int pk_mat_time_mult(pk_mat M, pk_vec x, pk_vec y, int reps, int fp32, double* avg_seconds) {
    PK_REQUIRE(M && M->assembled && x && y && reps > 0, "wrong arguments");
    Ctx& c = ctx();
+   DBuf<float> xb, yb;
    float* xf = nullptr;
    float* yf = nullptr;
    if (fp32) {
       PK_CALL(M->ensure_fp32());
-      PK_HIP(hipMalloc(&xf, sizeof(float) * M->n));
-      PK_HIP(hipMalloc(&yf, sizeof(float) * M->n));
+      PK_REQUIRE(!xb.alloc(M->n) && !yb.alloc(M->n), "device allocation failed");
+      xf = xb.p;
+      yf = yb.p;
       to_f32_kernel<<<grid_for(M->n), kBlock, 0, c.stream>>>(x->d.p, xf, M->n);
    }
-   hipEvent_t e0, e1;
-   PK_HIP(hipEventCreate(&e0));
-   PK_HIP(hipEventCreate(&e1));
+   EventPair ev;
+   PK_HIP(ev.create());
    // warm-up
    if (fp32)
       PK_CALL(spmv_f(M, xf, yf));
    else
       PK_CALL(spmv(M, x->d.p, y->d.p));
-   PK_HIP(hipEventRecord(e0, c.stream));
+   PK_HIP(hipEventRecord(ev.e0, c.stream));
    for (int r = 0; r < reps; r++) {
       if (fp32)
          PK_CALL(spmv_f(M, xf, yf));
       else
          PK_CALL(spmv(M, x->d.p, y->d.p));
    }
-   PK_HIP(hipEventRecord(e1, c.stream));
-   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipEventRecord(ev.e1, c.stream));
    float ms = 0.f;
-   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
+   PK_HIP(ev.ms(&ms));
    *avg_seconds = (double)ms * 1e-3 / reps;
-   (void)hipEventDestroy(e0);
-   (void)hipEventDestroy(e1);
-   if (xf) (void)hipFree(xf);
-   if (yf) (void)hipFree(yf);
    return 0;
 }
 

@@ -266,19 +266,16 @@ int pk_stream_copy(long n, int reps, double* avg_seconds) {
    PK_REQUIRE(!x.alloc(n) && !y.alloc(n), "device allocation failed");
    PK_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * n, c.stream));
    const long n4 = n / 4;
-   hipEvent_t e0, e1;
-   PK_HIP(hipEventCreate(&e0));
-   PK_HIP(hipEventCreate(&e1));
+   EventPair ev;
+   PK_HIP(ev.create());
    stream_copy_kernel<<<grid_for(n4, 256), 256, 0, c.stream>>>((const float4*)x.p, (float4*)y.p, n4);
-   PK_HIP(hipEventRecord(e0, c.stream));
+   PK_HIP(hipEventRecord(ev.e0, c.stream));
    for (int r = 0; r < reps; r++)
       stream_copy_kernel<<<grid_for(n4, 256), 256, 0, c.stream>>>((const float4*)x.p, (float4*)y.p, n4);
-   PK_HIP(hipEventRecord(e1, c.stream));
-   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipEventRecord(ev.e1, c.stream));
+   PK_HIP(hipGetLastError());
    float ms = 0.f;
-   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
-   (void)hipEventDestroy(e0);
-   (void)hipEventDestroy(e1);
+   PK_HIP(ev.ms(&ms));
    *avg_seconds = ms * 1e-3 / reps;
    return 0;
 }

@@ -1609,23 +1609,19 @@ int pk_sn_time_source(pk_sn S, int reps, double* avg_seconds, double* bytes_per_
    const bool f32 = S->precision == PK_PREC_MIXED;
    sn_fill_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->phiw.p, CG, 1.0);
    sn_fill_kernel<double><<<1, 1, 0, c.stream>>>(S->st.p, 1, 1.0);
-   hipEvent_t e0, e1;
-   PK_HIP(hipEventCreate(&e0));
-   PK_HIP(hipEventCreate(&e1));
+   EventPair ev;
+   PK_HIP(ev.create());
    for (int r = -1; r < reps; r++) {
-      if (r == 0) PK_HIP(hipEventRecord(e0, c.stream));
+      if (r == 0) PK_HIP(hipEventRecord(ev.e0, c.stream));
       if (f32)
          sn_source<float>(S->contraction, C, G, S->mesh, S->xs, S->phiw.p, S->st.p, S->q32.p, c.stream);
       else
          sn_source<double>(S->contraction, C, G, S->mesh, S->xs, S->phiw.p, S->st.p, S->q64.p, c.stream);
    }
-   PK_HIP(hipEventRecord(e1, c.stream));
-   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipEventRecord(ev.e1, c.stream));
    PK_HIP(hipGetLastError());
    float ms = 0.f;
-   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
-   (void)hipEventDestroy(e0);
-   (void)hipEventDestroy(e1);
+   PK_HIP(ev.ms(&ms));
    if (avg_seconds) *avg_seconds = ms * 1e-3 / reps;
    // the moment read, the source written, volume and material per cell
    if (bytes_per_call) *bytes_per_call = (double)CG * (8.0 + (f32 ? 4.0 : 8.0)) + (double)C * 12.0;
@@ -1643,22 +1639,19 @@ int pk_sn_time_sweep(pk_sn S, int reps, double* avg_seconds, double* bytes_per_s
       sn_fill_kernel<float><<<grid_for(CG), kBlock, 0, c.stream>>>(S->q32.p, CG, 1.0f);
    else
       sn_fill_kernel<double><<<grid_for(CG), kBlock, 0, c.stream>>>(S->q64.p, CG, 1.0);
-   hipEvent_t e0, e1;
-   PK_HIP(hipEventCreate(&e0));
-   PK_HIP(hipEventCreate(&e1));
+   EventPair ev;
+   PK_HIP(ev.create());
    for (int r = -1; r < reps; r++) {
-      if (r == 0) PK_HIP(hipEventRecord(e0, c.stream));
+      if (r == 0) PK_HIP(hipEventRecord(ev.e0, c.stream));
       if (f32)
          PK_CALL(sn_sweep<float>(S, S->sigt32.p, S->q32.p, S->psi32.p));
       else
          PK_CALL(sn_sweep<double>(S, S->sigt64.p, S->q64.p, S->psi64.p));
    }
-   PK_HIP(hipEventRecord(e1, c.stream));
-   PK_HIP(hipEventSynchronize(e1));
+   PK_HIP(hipEventRecord(ev.e1, c.stream));
+   PK_HIP(hipGetLastError());
    float ms = 0.f;
-   PK_HIP(hipEventElapsedTime(&ms, e0, e1));
-   (void)hipEventDestroy(e0);
-   (void)hipEventDestroy(e1);
+   PK_HIP(ev.ms(&ms));
    if (avg_seconds) *avg_seconds = ms * 1e-3 / reps;
    if (bytes_per_sweep) *bytes_per_sweep = sweep_bytes(S, f32 ? 4.0 : 8.0);
    return 0;

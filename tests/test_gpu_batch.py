@@ -245,3 +245,31 @@ def test_autotuner_choices(pk, members, monkeypatch):
     for b in range(len(members)):
         k, x = batch2.eigenpair(b)
         assert k == first[b][0] and np.array_equal(x.get(), first[b][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tick_length_gives_the_same_bits(pk, members, monkeypatch, precision):
+    """A GMRES cycle may span ticks: each problem keeps its step on the device
+    and the tick's end finishes only the cycles that stopped.  Ticks of 1, 4
+    and 13 step launches (PK_TICK_STEPS; cycles then span several ticks, in a
+    batch with restart 60 and 16-group members capped at 30) give the same
+    eigenpairs, bit for bit, as the predicted tick lengths."""
+    prec = pk.PK_PREC_FP64 if precision == "fp64" else pk.PK_PREC_MIXED
+    res = {}
+    for n in (0, 13, 4, 1):
+        if n:
+            monkeypatch.setenv("PK_TICK_STEPS", str(n))
+        else:
+            monkeypatch.delenv("PK_TICK_STEPS", raising=False)
+        print(f"[case] batch of {len(members)} {precision}, tick steps {n or 'predicted'}", flush=True)
+        Rs, Fs = _mats(pk, members)
+        batch = pk.Batch(Rs, Fs, precision=prec)
+        batch.solve()
+        res[n] = [(k, x.get(), batch.stats(b)) for b, (k, x) in
+                  enumerate(batch.eigenpair(b) for b in range(len(members)))]
+    for n in (13, 4, 1):
+        for b, (name, _, _) in enumerate(members):
+            assert res[n][b][0] == res[0][b][0], (n, name)
+            assert np.array_equal(res[n][b][1], res[0][b][1]), (n, name)
+            assert res[n][b][2]["outer"] == res[0][b][2]["outer"], (n, name)
+            assert res[n][b][2]["inner"] == res[0][b][2]["inner"], (n, name)
