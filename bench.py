@@ -26,12 +26,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per gmres_spmv_dot launch (all launches of the captured cycles, idle ones
-# included) on the quarter-core workload: two rocprofv3 --pmc passes (FETCH_SIZE, then
-# WRITE_SIZE) over `bench.py --roofline-pass`, each scaled by what the same counter reads for
+# HBM bytes per gmres_spmv_dot launch (all step launches of the ticks, idle ones included)
+# on the quarter-core workload: two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
+# `bench.py --roofline-pass`, each scaled by what the same counter reads for
 # stream_copy_kernel moving 2^28 bytes each way (fetch x2.000, write x1.000):
-# 1926.8 KiB x 2.000 + 167.5 KiB = 4.118 MB.
-SPMV_DOT_TRAFFIC_BYTES = 4.118e6
+# 2532.1 KiB x 2.000 + 218.9 KiB = 5.410 MB.
+SPMV_DOT_TRAFFIC_BYTES = 5.410e6
 # the same counter passes over `bench.py --batched-pass` (256 instances in one mixed pool),
 # HBM bytes per launch of each GMRES step kernel
 BATCHED_TRAFFIC_BYTES = {"gmres_spmv_dot": 509.9e6, "gmres_orth": 726.3e6, "gmres_poly": 948.8e6}
