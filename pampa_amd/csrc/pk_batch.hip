@@ -61,13 +61,16 @@ struct BLayout {
 
 // Sliced-ELL operator of the pool.  Each problem keeps the slice height h of
 // its matrix (32, 64 or 128 rows, picked per size class by the autotuner
-// below); the tables have one entry per 32-row group g of the pool: the
+// below); the table has one 16-byte record per 32-row group g of the pool: the
 // offset of the group's first row in its slice, the slice width and h, so
-// entry k of row r sits at off[r / 32] + r % 32 + h k.
+// entry k of row r sits at grp[r / 32].off + r % 32 + h k.
+struct alignas(16) SellGroup {
+   long off;
+   int width, h;
+};
+
 struct BSell {
-   const long* off;
-   const int* width;
-   const int* h;
+   const SellGroup* grp;
    const int* col;
    const int* rlen;
 };
@@ -78,19 +81,18 @@ struct SellRef {
 };
 
 __device__ __forceinline__ SellRef sell_ref(const BSell& A, long row) {
-   const long g = row >> 5;
-   return SellRef{A.off[g] + (row & 31), A.width[g], A.h[g]};
+   const SellGroup g = A.grp[row >> 5];
+   return SellRef{g.off + (row & 31), g.width, g.h};
 }
 
 // The same for a row of the calling wavefront's own 64 rows (row = 64 w +
-// lane): the wavefront's two group entries are loaded once as uniform values
-// (scalar loads, issued early), each lane picks its half.
+// lane): the wavefront's two group records are loaded once, at a uniform
+// address (two 16-byte loads), and each lane picks its half.
 __device__ __forceinline__ SellRef sell_ref_wave(const BSell& A, long row) {
    const long g0 = 2L * __builtin_amdgcn_readfirstlane((int)(row >> 6));
-   const long off0 = A.off[g0], off1 = A.off[g0 + 1];
-   const int w0 = A.width[g0], w1 = A.width[g0 + 1], h0 = A.h[g0], h1 = A.h[g0 + 1];
+   const SellGroup a = A.grp[g0], b = A.grp[g0 + 1];
    const bool hi = (row & 32) != 0;
-   return SellRef{(hi ? off1 : off0) + (row & 31), hi ? w1 : w0, hi ? h1 : h0};
+   return SellRef{(hi ? b.off : a.off) + (row & 31), hi ? b.width : a.width, hi ? b.h : a.h};
 }
 
 struct BState {  // per-slot state (arrays over slots)
@@ -146,18 +148,20 @@ __device__ __forceinline__ int slot_of_row(const BLayout& L, long row) { return 
 // 32 j / h at row 32 j mod h of that slice; groups past the matrix's slices
 // are empty
 __global__ void place_groups_kernel(long ns_src, int h, long ng_dst, const long* soff, const int* swidth,
-                                    long dst_g0, long base, long* doff, int* dwidth, int* dh) {
+                                    long dst_g0, long base, SellGroup* dst) {
    const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (j >= ng_dst) return;
    const long s = 32 * j / h;
+   SellGroup g;
    if (s < ns_src) {
-      doff[dst_g0 + j] = base + (soff[s] - soff[0]) + (32 * j - s * h);
-      dwidth[dst_g0 + j] = swidth[s];
+      g.off = base + (soff[s] - soff[0]) + (32 * j - s * h);
+      g.width = swidth[s];
    } else {
-      doff[dst_g0 + j] = base;
-      dwidth[dst_g0 + j] = 0;
+      g.off = base;
+      g.width = 0;
    }
-   dh[dst_g0 + j] = h;
+   g.h = h;
+   dst[dst_g0 + j] = g;
 }
 
 __global__ void place_cols_kernel(long n, const int* scol, int shift, int* dcol) {
@@ -391,8 +395,14 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
 
 // ---- GMRES ------------------------------------------------------------------------------
 
-__device__ __forceinline__ bool gmres_on(const BState& S, int b) {
-   return b >= 0 && S.phase[b] == kInner && !S.flags[4 * b];
+// The next step of slot b's cycle, or -1 when the slot is not iterating: its
+// phase, stop flag, step index and restart length (*mlen) are loaded together
+// (one memory round trip).
+__device__ __forceinline__ int gmres_step(const BLayout& L, const BState& S, int b, int* mlen) {
+   if (b < 0) return -1;
+   const int ph = S.phase[b], stop = S.flags[4 * b], j = S.jcur[b];
+   *mlen = L.mlen[b];
+   return ph == kInner && !stop ? j : -1;
 }
 
 // z = B^-1 t for this workgroup's rows of slot b: t of the whole workgroup is
@@ -421,11 +431,12 @@ __device__ __forceinline__ T cell_block_apply(const BLayout& L, int b, const T* 
 // whose column is one of the workgroup's rows): z += B^-1 (t - A_loc z).  A
 // fixed linear map of t (the same for a problem alone or in a mix: its
 // workgroups start at its first row).  Every thread calls it (it holds
-// barriers); padding rows get 0.
+// barriers); padding rows get 0.  ar: the thread's row of A (sell_ref_wave),
+// which the step kernels look up at their start, beside the slot lookup.
 
 template <class T>
 __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* binv, const BSell& A, const T* aval,
-                                           T t, T* tl, T* xl) {
+                                           const SellRef& ar, T t, T* tl, T* xl) {
    const T x = cell_block_apply<T>(L, b, binv, t, tl);
    if (!L.sweep) {
       __syncthreads();  // tl is staged again by the caller's next apply
@@ -434,8 +445,6 @@ __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* bin
    xl[threadIdx.x] = x;
    __syncthreads();
    const long row0 = blockIdx.x * (long)kRows;
-   const long row = row0 + threadIdx.x;
-   const SellRef ar = sell_ref_wave(A, row);
    const int wd = ar.wd;
    T acc = 0;
    // entries 4 at a time: their loads first, then the sum in order
@@ -472,7 +481,7 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, BSell A, const
    rhs[row] = v;
    e[row] = 0;
    V0[row] = v;
-   z[row] = precond_apply<T>(L, b, binv, A, aval, v, tl, xl);
+   z[row] = precond_apply<T>(L, b, binv, A, aval, sell_ref_wave(A, row), v, tl, xl);
    const double u = (double)v;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
@@ -562,14 +571,15 @@ template <class T>
 __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* z,
                                               T* w, const T* V, double* partials, unsigned* tickets, unsigned* gtickets,
                                               const BState& S, long long* act) {
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const SellRef ar = sell_ref_wave(A, row);  // issued beside the slot lookup
    const int b = slot_of_wg(L);
-   if (!gmres_on(S, b)) return;
-   const int j = S.jcur[b];  // every workgroup reads it before the reduction; the epilogue advances it
+   int mlen = 0;
+   const int j = gmres_step(L, S, b, &mlen);  // read before the reduction; the epilogue advances it
+   if (j < 0) return;
    const int m = S.m;
    const double* vs = S.vs + (long)b * (m + 1);
    __shared__ double hd[kBMaxM + 2];
-   const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const int mlen = L.mlen[b];
    const bool edge = j == mlen;
    if (edge)
       count_act(L, b, act, 2, 1, -1, 0);
@@ -603,7 +613,6 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
           1, [&](int) { return vj * vj; }, part, tickets + b, gtickets + L.wg_first[b], hd + j + 1,
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    } else {
-      const SellRef ar = sell_ref_wave(A, row);
       const T acc = sell_row_dot<T, T>(A.col, aval, z, ar.base, ar.wd, (T)0, ar.h);
       w[row] = acc;
       const double wr = (double)acc;
@@ -682,13 +691,15 @@ __global__ void __launch_bounds__(kRows) bspmv_dot_kernel(long N, BLayout L, BSe
 template <class T>
 __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* w, T* V,
                                           T* z, const T* binv, const BState& S, long long* act) {
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const SellRef ar = sell_ref_wave(A, row);  // issued beside the slot lookup
    const int b = slot_of_wg(L);
-   if (!gmres_on(S, b)) return;
-   const int j = S.jcur[b] - 1;  // the step whose SpMV and dots ran last
+   int mlen = 0;
+   const int j = gmres_step(L, S, b, &mlen) - 1;  // the step whose SpMV and dots ran last
+   if (j < 0) return;
    count_act(L, b, act, 3, 1, 4, j + 4);
    __shared__ T tl[kRows], xl[kRows];
    const int m = S.m;
-   const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const double* h = S.H + (long)b * (m + 1) * m + (long)j * (m + 1);
    const double* vs = S.vs + (long)b * (m + 1);
    T t = (T)vs[j] * w[row];
@@ -706,7 +717,7 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell&
          if (i0 + u <= j) t -= cf[u] * vv[u];
    }
    V[(long)(j + 1) * N + row] = t;
-   z[row] = precond_apply<T>(L, b, binv, A, aval, t, tl, xl);
+   z[row] = precond_apply<T>(L, b, binv, A, aval, ar, t, tl, xl);
 }
 
 template <class T>
@@ -782,21 +793,21 @@ template <class T>
 __device__ __forceinline__ void poly_step(long N, const BLayout& L, const BSell& A, const T* aval, const T* v,
                                           const T* z0, const T* binv, T* z, const BState& S, int mode, T* e,
                                           long long* act) {
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const SellRef ar = sell_ref_wave(A, row);  // issued beside the slot lookup
    const int b = slot_of_wg(L);
    if (mode == 0) {
-      if (!gmres_on(S, b)) return;
-      const int j = S.jcur[b];
-      if (j >= L.mlen[b]) return;  // the column past the cycle's last: norm only
+      int mlen = 0;
+      const int j = gmres_step(L, S, b, &mlen);
+      if (j < 0 || j >= mlen) return;  // at j = mlen (the column past the cycle's last): norm only
       v += (long)j * N;
       count_act(L, b, act, 5, 1, -1, 0);
    } else if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b] || S.flags[4 * b + 1] == 0) {
       return;
    }
    __shared__ T tl[kRows], xl[kRows];
-   const long row = blockIdx.x * (long)kRows + threadIdx.x;
-   const SellRef ar = sell_ref_wave(A, row);
    const T az = sell_row_dot<T, T>(A.col, aval, z0, ar.base, ar.wd, (T)0, ar.h);
-   const T c = precond_apply<T>(L, b, binv, A, aval, v[row] - az, tl, xl);
+   const T c = precond_apply<T>(L, b, binv, A, aval, ar, v[row] - az, tl, xl);
    const T zr = z0[row] + c;
    z[row] = zr;
    if (mode == 1) e[row] += zr;
@@ -829,7 +840,7 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, BSell
       for (int u = 0; u < 8; u++)
          if (i0 + u < k) acc += cf[u] * vv[u];
    }
-   const T s = precond_apply<T>(L, b, binv, A, aval, acc, u, xl);
+   const T s = precond_apply<T>(L, b, binv, A, aval, sell_ref_wave(A, row), acc, u, xl);
    if (accumulate) {
       if (local < L.nrows[b]) e[row] += s;
    } else {
@@ -854,7 +865,7 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    const T acc = sell_row_dot<T, T>(A.col, aval, e, ar.base, ar.wd, (T)0, ar.h);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
-   z[row] = precond_apply<T>(L, b, binv, A, aval, rv, tl, xl);
+   z[row] = precond_apply<T>(L, b, binv, A, aval, ar, rv, tl, xl);
    const double u = (double)rv;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
@@ -1014,8 +1025,8 @@ struct pk_batch_s {
    // device arrays
    DBuf<int> unit_slot, wg_first, wg_count, nrows, pbs, fbs, mlen;
    DBuf<long> row_off, binv_off;
-   DBuf<long> r_off, f_off;  // per 32-row group
-   DBuf<int> r_width, f_width, r_h, f_h, r_col, f_col, rlen;
+   DBuf<SellGroup> r_grp, f_grp;  // per 32-row group
+   DBuf<int> r_col, f_col, rlen;
    DBuf<double> r_val, f_val, a_val;
    DBuf<double> x, phi, y1, y2, r;
    DBuf<double> sd;
@@ -1063,8 +1074,8 @@ struct pk_batch_s {
       return BLayout{unit_slot.p, wg_first.p, wg_count.p, row_off.p, nrows.p, pbs.p, fbs.p, binv_off.p, mlen.p,
                      opts.sweep};
    }
-   BSell rsell() const { return BSell{r_off.p, r_width.p, r_h.p, r_col.p, rlen.p}; }
-   BSell fsell() const { return BSell{f_off.p, f_width.p, f_h.p, f_col.p, rlen.p}; }
+   BSell rsell() const { return BSell{r_grp.p, r_col.p, rlen.p}; }
+   BSell fsell() const { return BSell{f_grp.p, f_col.p, rlen.p}; }
 };
 
 static int preconditioner_block(int bs) {
@@ -1103,8 +1114,7 @@ static int pool_alloc(pk_batch_s* P) {
    bool bad = P->unit_slot.alloc(units) || P->wg_first.alloc(S) || P->wg_count.alloc(S) || P->nrows.alloc(S) ||
               P->mlen.alloc(S) ||
               P->pbs.alloc(S) || P->fbs.alloc(S) || P->row_off.alloc(S) || P->binv_off.alloc(S) ||
-              P->r_off.alloc(sl + 1) || P->f_off.alloc(sl + 1) || P->r_width.alloc(sl) || P->f_width.alloc(sl) ||
-              P->r_h.alloc(sl) || P->f_h.alloc(sl) ||
+              P->r_grp.alloc(sl) || P->f_grp.alloc(sl) ||
               P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
               P->f_col.alloc(fent) || P->f_val.alloc(fent) || P->x.alloc(rows) || P->phi.alloc(rows) ||
               P->y1.alloc(rows) || P->y2.alloc(rows) || P->r.alloc(rows) ||
@@ -1171,12 +1181,9 @@ static int pool_alloc(pk_batch_s* P) {
    P->h_unit_slot.assign(units, -1);
    PK_HIP(hipMemcpyAsync(P->unit_slot.p, P->h_unit_slot.data(), sizeof(int) * units, hipMemcpyHostToDevice,
                          c.stream));
-   PK_HIP(hipMemsetAsync(P->r_width.p, 0, sizeof(int) * sl, c.stream));
-   PK_HIP(hipMemsetAsync(P->f_width.p, 0, sizeof(int) * sl, c.stream));
-   PK_HIP(hipMemsetAsync(P->r_h.p, 0, sizeof(int) * sl, c.stream));  // read only where a width is set
-   PK_HIP(hipMemsetAsync(P->f_h.p, 0, sizeof(int) * sl, c.stream));
-   PK_HIP(hipMemsetAsync(P->r_off.p, 0, sizeof(long) * (sl + 1), c.stream));
-   PK_HIP(hipMemsetAsync(P->f_off.p, 0, sizeof(long) * (sl + 1), c.stream));
+   // zero width everywhere (the height is read only where a width is set)
+   PK_HIP(hipMemsetAsync(P->r_grp.p, 0, sizeof(SellGroup) * sl, c.stream));
+   PK_HIP(hipMemsetAsync(P->f_grp.p, 0, sizeof(SellGroup) * sl, c.stream));
    PK_HIP(hipMemsetAsync(P->rlen.p, 0, sizeof(int) * rows, c.stream));
    PK_HIP(hipMemsetAsync(P->x.p, 0, sizeof(double) * rows, c.stream));
    PK_HIP(hipMemsetAsync(P->phi.p, 0, sizeof(double) * rows, c.stream));
@@ -1355,11 +1362,9 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    const long g0 = row0 / 32, ng = padded / 32;
    // SELL arrays of R and F, columns moved by the row offset
    place_groups_kernel<<<grid_for(ng), kBlock, 0, c.stream>>>(R->sell.nslices, R->sell.h, ng, R->sell.off.p,
-                                                             R->sell.width.p, g0, r0 * 64, P->r_off.p,
-                                                             P->r_width.p, P->r_h.p);
+                                                             R->sell.width.p, g0, r0 * 64, P->r_grp.p);
    place_groups_kernel<<<grid_for(ng), kBlock, 0, c.stream>>>(F->sell.nslices, F->sell.h, ng, F->sell.off.p,
-                                                             F->sell.width.p, g0, f0 * 64, P->f_off.p,
-                                                             P->f_width.p, P->f_h.p);
+                                                             F->sell.width.p, g0, f0 * 64, P->f_grp.p);
    if (R->sell.padded) {
       place_cols_kernel<<<grid_for(R->sell.padded), kBlock, 0, c.stream>>>(R->sell.padded, R->sell.col.p,
                                                                           (int)row0, P->r_col.p + r0 * 64);
