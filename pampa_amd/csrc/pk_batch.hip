@@ -1629,7 +1629,14 @@ static int run_tick(pk_batch_s* P, long units, int steps) {
    auto it = P->graph.exec.find(key);
    if (it == P->graph.exec.end()) {
       // every quantity that varies between ticks is read from device memory,
-      // so one graph serves every tick of this grid and length
+      // so one graph serves every tick of this grid and length; a batch whose
+      // members change size keeps at most kMaxGraphs of them, dropped once the
+      // stream has finished the ticks in flight
+      constexpr size_t kMaxGraphs = 64;
+      if (P->graph.exec.size() >= kMaxGraphs) {
+         PK_HIP(hipStreamSynchronize(c.stream));
+         P->graph.clear();
+      }
       hipGraph_t g = nullptr;
       PK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
       int rc = launch_tick<T>(P, units, steps, false);
