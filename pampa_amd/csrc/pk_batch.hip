@@ -613,12 +613,23 @@ __device__ __forceinline__ void spmv_dot_step(long N, const BLayout& L, const BS
           1, [&](int) { return vj * vj; }, part, tickets + b, gtickets + L.wg_first[b], hd + j + 1,
           (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
    } else {
+      // the first batch of the dots' basis entries and scales, requested with
+      // the row's entries (ahead of the SpMV's gathers, not after them)
+      T vpre[kWaveBatch];
+      double spre[kWaveBatch];
+#pragma unroll
+      for (int u = 0; u < kWaveBatch; u++) {
+         const int i = min(u, j);
+         vpre[u] = V[(long)i * N + row];
+         spre[u] = vs[i];
+      }
       const T acc = sell_row_dot<T, T>(A.col, aval, z, ar.base, ar.wd, (T)0, ar.h);
       w[row] = acc;
       const double wr = (double)acc;
       const int nv = j + 2;
       last = seg_reduce<kBMaxM + 2>(
           nv,
+          [&](int c) { return c < j ? (double)vpre[c] * spre[c] * wr : (c == j ? vj * wr : vj * vj); },
           [&](int k) {
              return k < j ? (double)V[(long)k * N + row] * vs[k] * wr : (k == j ? vj * wr : vj * vj);
           },

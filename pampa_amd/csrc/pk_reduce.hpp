@@ -32,14 +32,22 @@ __device__ __forceinline__ double wave_sum(double v) {
 // past nv - 1 are clamped (the value is computed again and not written).
 constexpr int kWaveBatch = 8;
 
-template <int MAXV, class Get>
-__device__ __forceinline__ void wave_sums(int nv, Get get, double (*sh)[kBlock / 64]) {
+// The first batch's values come from first(c), c = 0 .. CH - 1 a compile-time
+// index standing for value min(c, nv - 1): a caller that loaded what they need
+// ahead gives them from registers.
+template <int MAXV, class First, class Get>
+__device__ __forceinline__ void wave_sums(int nv, First first, Get get, double (*sh)[kBlock / 64]) {
    constexpr int CH = MAXV < kWaveBatch ? MAXV : kWaveBatch;
    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
    for (int k0 = 0; k0 < nv; k0 += CH) {
       double v[CH];
+      if (k0 == 0) {
 #pragma unroll
-      for (int c = 0; c < CH; c++) v[c] = get(min(k0 + c, nv - 1));
+         for (int c = 0; c < CH; c++) v[c] = first(c);
+      } else {
+#pragma unroll
+         for (int c = 0; c < CH; c++) v[c] = get(min(k0 + c, nv - 1));
+      }
 #pragma unroll
       for (int c = 0; c < CH; c++) v[c] = wave_sum(v[c]);
       if (lane == 0) {
@@ -73,7 +81,8 @@ template <int MAXV, class Get>
 __device__ bool grid_reduce(int nv, Get get, double* partials, unsigned* ticket, double* out) {
    __shared__ double sh[MAXV][kBlock / 64];
    __shared__ int is_last;
-   wave_sums<MAXV>(nv, get, sh);
+   wave_sums<MAXV>(
+       nv, [&](int c) { return get(min(c, nv - 1)); }, get, sh);
    __syncthreads();
    // Partials are written with system-scope stores, which go through to memory
    // instead of sitting dirty in this XCD's L2, and waited for before the
@@ -170,12 +179,13 @@ __device__ __forceinline__ int seg_group(int seg_blocks) {
    return seg_blocks <= 256 ? 16 : (seg_blocks <= 1024 ? 32 : 64);
 }
 
-template <int MAXV, class Get>
-__device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, unsigned* gticket, double* out,
-                           int seg_block, int seg_blocks) {
+// `first` gives the first kWaveBatch values as wave_sums takes them.
+template <int MAXV, class First, class Get>
+__device__ bool seg_reduce(int nv, First first, Get get, double* partials, unsigned* ticket, unsigned* gticket,
+                           double* out, int seg_block, int seg_blocks) {
    __shared__ double sh[MAXV][kBlock / 64];
    __shared__ int is_last;
-   wave_sums<MAXV>(nv, get, sh);
+   wave_sums<MAXV>(nv, first, get, sh);
    __syncthreads();
    if ((int)threadIdx.x < nv) {
       double s = 0.0;
@@ -218,6 +228,13 @@ __device__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, 
    __threadfence_block();
    __syncthreads();
    return true;
+}
+
+template <int MAXV, class Get>
+__device__ __forceinline__ bool seg_reduce(int nv, Get get, double* partials, unsigned* ticket, unsigned* gticket,
+                                          double* out, int seg_block, int seg_blocks) {
+   return seg_reduce<MAXV>(
+       nv, [&](int c) { return get(min(c, nv - 1)); }, get, partials, ticket, gticket, out, seg_block, seg_blocks);
 }
 
 template <int MAXV, class Get>
