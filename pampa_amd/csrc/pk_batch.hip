@@ -408,21 +408,42 @@ __device__ __forceinline__ int gmres_step(const BLayout& L, const BState& S, int
 // z = B^-1 t for this workgroup's rows of slot b: t of the whole workgroup is
 // staged in LDS (a cell block never straddles a workgroup); every thread calls
 // it (it holds a barrier), padding rows get 0
+// A thread's part in the preconditioner of slot b, looked up once per kernel
+// (with the kernel's other first loads, not behind the applies' barriers):
+// the row of its cell block's inverse (null for a padding row), the block
+// size, the thread of the block's first row, and the first entries of its
+// row of A for the workgroup-block sweep.
 template <class T>
-__device__ __forceinline__ T cell_block_apply(const BLayout& L, int b, const T* binv, T t, T* tl) {
-   tl[threadIdx.x] = t;
-   __syncthreads();
+struct PrecondRow {
+   const T* bi;
+   int bs, first;
+   SellHead<T> head;
+};
+
+template <class T>
+__device__ __forceinline__ PrecondRow<T> precond_row(const BLayout& L, int b, const T* binv, const BSell& A,
+                                                     const T* aval, const SellRef& ar) {
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long local = row - L.row_off[b];
+   const int bs = L.pbs[b], nrows = L.nrows[b];
+   const long boff = L.binv_off[b];
+   const long cell = local / bs;
+   const int gr = (int)(local - cell * bs);
+   PrecondRow<T> pr;
+   pr.bi = local < nrows ? binv + boff + cell * bs * bs + (long)gr * bs : nullptr;
+   pr.bs = bs;
+   pr.first = (int)(threadIdx.x - gr);
+   pr.head = sell_row_head<T>(A.col, aval, ar.base, L.sweep ? ar.wd : 0, ar.h);
+   return pr;
+}
+
+template <class T>
+__device__ __forceinline__ T cell_block_apply(const PrecondRow<T>& pr, T t, T* tl) {
+   tl[threadIdx.x] = t;
+   __syncthreads();
    T acc = 0;
-   if (local < L.nrows[b]) {
-      const int bs = L.pbs[b];
-      const long cell = local / bs;
-      const int gr = (int)(local - cell * bs);
-      const T* bi = binv + L.binv_off[b] + cell * bs * bs + (long)gr * bs;
-      const int first = (int)(threadIdx.x - gr);
-      for (int q = 0; q < bs; q++) acc += bi[q] * tl[first + q];
-   }
+   if (pr.bi)
+      for (int q = 0; q < pr.bs; q++) acc += pr.bi[q] * tl[pr.first + q];
    return acc;
 }
 
@@ -432,12 +453,13 @@ __device__ __forceinline__ T cell_block_apply(const BLayout& L, int b, const T* 
 // fixed linear map of t (the same for a problem alone or in a mix: its
 // workgroups start at its first row).  Every thread calls it (it holds
 // barriers); padding rows get 0.  ar: the thread's row of A (sell_ref_wave),
-// which the step kernels look up at their start, beside the slot lookup.
+// which the step kernels look up at their start, beside the slot lookup; pr:
+// its precond_row.
 
 template <class T>
-__device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* binv, const BSell& A, const T* aval,
-                                           const SellRef& ar, T t, T* tl, T* xl) {
-   const T x = cell_block_apply<T>(L, b, binv, t, tl);
+__device__ __forceinline__ T precond_apply(const BLayout& L, const BSell& A, const T* aval, const SellRef& ar,
+                                           const PrecondRow<T>& pr, T t, T* tl, T* xl) {
+   const T x = cell_block_apply<T>(pr, t, tl);
    if (!L.sweep) {
       __syncthreads();  // tl is staged again by the caller's next apply
       return x;
@@ -453,9 +475,14 @@ __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* bin
       T aa[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-         const long p = ar.base + (long)min(k0 + u, wd - 1) * ar.h;
-         cc[u] = A.col[p];
-         aa[u] = aval[p];
+         if (k0 == 0) {
+            cc[u] = pr.head.c[u];
+            aa[u] = pr.head.v[u];
+         } else {
+            const long p = ar.base + (long)min(k0 + u, wd - 1) * ar.h;
+            cc[u] = A.col[p];
+            aa[u] = aval[p];
+         }
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -464,7 +491,7 @@ __device__ __forceinline__ T precond_apply(const BLayout& L, int b, const T* bin
       }
    }
    __syncthreads();  // tl is staged again below
-   return x + cell_block_apply<T>(L, b, binv, t - acc, tl);
+   return x + cell_block_apply<T>(pr, t - acc, tl);
 }
 
 // V0 = rhs = r (in T), z = S V0, e = 0, beta = |V0|, target = reduction beta.
@@ -477,11 +504,13 @@ __global__ void __launch_bounds__(kRows) bstart_kernel(BLayout L, BSell A, const
    if (b < 0 || S.phase[b] != kInner || !S.start[b]) return;
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const SellRef ar = sell_ref_wave(A, row);
+   const PrecondRow<T> pr = precond_row<T>(L, b, binv, A, aval, ar);
    const T v = (T)r[row];
    rhs[row] = v;
    e[row] = 0;
    V0[row] = v;
-   z[row] = precond_apply<T>(L, b, binv, A, aval, sell_ref_wave(A, row), v, tl, xl);
+   z[row] = precond_apply<T>(L, A, aval, ar, pr, v, tl, xl);
    const double u = (double)v;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,
@@ -709,6 +738,7 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell&
    const int j = gmres_step(L, S, b, &mlen) - 1;  // the step whose SpMV and dots ran last
    if (j < 0) return;
    count_act(L, b, act, 3, 1, 4, j + 4);
+   const PrecondRow<T> pr = precond_row<T>(L, b, binv, A, aval, ar);
    __shared__ T tl[kRows], xl[kRows];
    const int m = S.m;
    const double* h = S.H + (long)b * (m + 1) * m + (long)j * (m + 1);
@@ -728,7 +758,7 @@ __device__ __forceinline__ void orth_step(long N, const BLayout& L, const BSell&
          if (i0 + u <= j) t -= cf[u] * vv[u];
    }
    V[(long)(j + 1) * N + row] = t;
-   z[row] = precond_apply<T>(L, b, binv, A, aval, ar, t, tl, xl);
+   z[row] = precond_apply<T>(L, A, aval, ar, pr, t, tl, xl);
 }
 
 template <class T>
@@ -816,9 +846,10 @@ __device__ __forceinline__ void poly_step(long N, const BLayout& L, const BSell&
    } else if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b] || S.flags[4 * b + 1] == 0) {
       return;
    }
+   const PrecondRow<T> pr = precond_row<T>(L, b, binv, A, aval, ar);
    __shared__ T tl[kRows], xl[kRows];
    const T az = sell_row_dot<T, T>(A.col, aval, z0, ar.base, ar.wd, (T)0, ar.h);
-   const T c = precond_apply<T>(L, b, binv, A, aval, ar, v[row] - az, tl, xl);
+   const T c = precond_apply<T>(L, A, aval, ar, pr, v[row] - az, tl, xl);
    const T zr = z0[row] + c;
    z[row] = zr;
    if (mode == 1) e[row] += zr;
@@ -836,6 +867,8 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, BSell
    if (k == 0) return;
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const long local = row - L.row_off[b];
+   const SellRef ar = sell_ref_wave(A, row);
+   const PrecondRow<T> pr = precond_row<T>(L, b, binv, A, aval, ar);
    __shared__ T u[kRows], xl[kRows];
    T acc = 0;
    const double* vs = S.vs + (long)b * (S.m + 1);
@@ -851,7 +884,7 @@ __global__ void __launch_bounds__(kRows) bupdate_kernel(long N, BLayout L, BSell
       for (int u = 0; u < 8; u++)
          if (i0 + u < k) acc += cf[u] * vv[u];
    }
-   const T s = precond_apply<T>(L, b, binv, A, aval, sell_ref_wave(A, row), acc, u, xl);
+   const T s = precond_apply<T>(L, A, aval, ar, pr, acc, u, xl);
    if (accumulate) {
       if (local < L.nrows[b]) e[row] += s;
    } else {
@@ -873,10 +906,11 @@ __global__ void __launch_bounds__(kRows) brestart_kernel(BLayout L, BSell A, con
    __shared__ T tl[kRows], xl[kRows];
    const long row = blockIdx.x * (long)kRows + threadIdx.x;
    const SellRef ar = sell_ref_wave(A, row);
+   const PrecondRow<T> pr = precond_row<T>(L, b, binv, A, aval, ar);
    const T acc = sell_row_dot<T, T>(A.col, aval, e, ar.base, ar.wd, (T)0, ar.h);
    const T rv = rhs[row] - acc;
    V0[row] = rv;
-   z[row] = precond_apply<T>(L, b, binv, A, aval, ar, rv, tl, xl);
+   z[row] = precond_apply<T>(L, A, aval, ar, pr, rv, tl, xl);
    const double u = (double)rv;
    bool last = seg_reduce<1>(
        1, [&](int) { return u * u; }, partials + L.wg_first[b], tickets + b, gtickets + L.wg_first[b], S.ww + b,

@@ -47,4 +47,26 @@ __device__ __forceinline__ A sell_row_dot(const int* __restrict__ col, const T* 
    return acc;
 }
 
+// A row's first four stored entries (indices clamped to its last entry),
+// loaded ahead: a caller issues them with its other first loads instead of
+// after the barriers that precede their use.
+template <class T>
+struct SellHead {
+   int c[4];
+   T v[4];
+};
+
+template <class T>
+__device__ __forceinline__ SellHead<T> sell_row_head(const int* __restrict__ col, const T* __restrict__ val,
+                                                     long base, int wd, int h) {
+   SellHead<T> hd;
+#pragma unroll
+   for (int u = 0; u < 4; u++) {
+      const long p = base + (long)min(u, max(wd - 1, 0)) * h;
+      hd.c[u] = wd > 0 ? col[p] : 0;
+      hd.v[u] = wd > 0 ? val[p] : T(0);
+   }
+   return hd;
+}
+
 }  // namespace pk
