@@ -8,7 +8,7 @@
 // MatSetValues(INSERT_VALUES) rows (DiffusionSolver.cxx:293-296).  Values are
 // computed with the reference's operation order and without FMA contraction,
 // so FP64 entries can be compared bit-for-bit.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
@@ -258,10 +258,12 @@ int diffusion_pattern(pk_mesh_s* mesh, int G, DBuf<int>& ptr, long& nnz) {
    PK_HIP(hipGetLastError());
    PK_HIP(hipMemsetAsync(rlen.p + N, 0, sizeof(int), c.stream));
    size_t tmp_bytes = 0;
-   PK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, rlen.p, ptr.p, (int)(N + 1), c.stream));
+   PK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, rlen.p, ptr.p, 0, (size_t)(N + 1), rocprim::plus<int>(),
+                                  c.stream));
    DBuf<char> tmp;
    PK_REQUIRE(!tmp.alloc(tmp_bytes ? tmp_bytes : 1), "device allocation failed");
-   PK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, rlen.p, ptr.p, (int)(N + 1), c.stream));
+   PK_HIP(rocprim::exclusive_scan(tmp.p, tmp_bytes, rlen.p, ptr.p, 0, (size_t)(N + 1), rocprim::plus<int>(),
+                                  c.stream));
    int h = 0;
    PK_HIP(hipMemcpyAsync(&h, ptr.p + N, sizeof(int), hipMemcpyDeviceToHost, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));

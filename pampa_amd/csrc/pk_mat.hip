@@ -13,7 +13,7 @@
 #include <algorithm>
 #include <numeric>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "pk_internal.hpp"
 #include "pk_reduce.hpp"
@@ -83,10 +83,12 @@ int build_sell(pk_mat_s* A, int h) {
    sell_width_kernel<<<grid_for(ns + 1), kBlock, 0, c.stream>>>(n, ns, h, A->ptr.p, S.width.p, wl.p);
    PK_HIP(hipGetLastError());
    size_t tmp_bytes = 0;
-   PK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, wl.p, S.off.p, (int)(ns + 1), c.stream));
+   PK_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, wl.p, S.off.p, 0L, (size_t)(ns + 1), rocprim::plus<long>(),
+                                  c.stream));
    DBuf<char> tmp;
    PK_REQUIRE(!tmp.alloc(tmp_bytes ? tmp_bytes : 1), "device allocation failed");
-   PK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, wl.p, S.off.p, (int)(ns + 1), c.stream));
+   PK_HIP(rocprim::exclusive_scan(tmp.p, tmp_bytes, wl.p, S.off.p, 0L, (size_t)(ns + 1), rocprim::plus<long>(),
+                                  c.stream));
    long tot = 0;
    PK_HIP(hipMemcpyAsync(&tot, S.off.p + ns, sizeof(long), hipMemcpyDeviceToHost, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));
