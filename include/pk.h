@@ -67,6 +67,8 @@ int pk_vec_scale(pk_vec v, double a);                   /* VecScale */
 int pk_vec_axpy(pk_vec y, double a, pk_vec x);          /* VecAXPY */
 int pk_vec_dot(pk_vec x, pk_vec y, double* r);          /* VecDot (fixed-order FP64 reduction) */
 int pk_vec_norm2(pk_vec x, double* r);                  /* VecNorm(NORM_2) */
+int pk_vec_pointwise_mult(pk_vec w, pk_vec x, pk_vec y); /* VecPointwiseMult: w_i = x_i y_i (the delayed
+                                                            source S = beta P, NeutronicSolver.cxx:102) */
 int pk_vec_device_ptr(pk_vec v, void** dev);            /* device address (interop) */
 /* Entries in [-rel max|v|, 0) set to 0 (round-off of zeros of an iterative
  * solution); *negatives = entries below -rel max|v| (the check of

@@ -53,7 +53,7 @@ def _declare(L):
         "pk_vec_create": [_L, _PP], "pk_vec_destroy": [_PP], "pk_vec_set": [_P, _D],
         "pk_vec_set_values": [_P, _P], "pk_vec_get_values": [_P, _P], "pk_vec_copy": [_P, _P],
         "pk_vec_scale": [_P, _D], "pk_vec_axpy": [_P, _D, _P], "pk_vec_dot": [_P, _P, _P],
-        "pk_vec_norm2": [_P, _P], "pk_vec_device_ptr": [_P, _PP],
+        "pk_vec_norm2": [_P, _P], "pk_vec_device_ptr": [_P, _PP], "pk_vec_pointwise_mult": [_P, _P, _P],
         "pk_mat_create": [_L, _I, _PP], "pk_mat_set_values": [_P, _I, _P, _I, _P, _P, _I],
         "pk_mat_assemble": [_P], "pk_mat_destroy": [_PP], "pk_mat_get_size": [_P, _P, _P],
         "pk_mat_get_csr": [_P, _P, _P, _P], "pk_mat_mult": [_P, _P, _P],

@@ -170,12 +170,16 @@ int NeutronicSolver::powerAndProduction(bool normalize) {
    double tot = 0.0;
    PK_CHECK(pk_normalize_flux(dmesh, dxs, phi, normalize ? power : -1.0, keff, q, P, &tot));
    power_total = tot;
-   // delayed source S = beta P (per material)
-   long C = mesh->num_cells;
-   std::vector<double> hP(C), hS(C);
-   PK_CHECK(pk_vec_get_values(P, hP.data()));
-   for (long i = 0; i < C; i++) hS[i] = materials[mesh->mat[i]].beta_total * hP[i];
-   PK_CHECK(pk_vec_set_values(S, hS.data()));
+   // delayed source S = beta P (per material, NeutronicSolver.cxx:102), on the
+   // device with each cell's beta (fixed once the materials are read)
+   if (!beta) {
+      const long C = mesh->num_cells;
+      std::vector<double> hb(C);
+      for (long i = 0; i < C; i++) hb[i] = materials[mesh->mat[i]].beta_total;
+      PK_CHECK(pk_vec_create(C, &beta));
+      PK_CHECK(pk_vec_set_values(beta, hb.data()));
+   }
+   PK_CHECK(pk_vec_pointwise_mult(S, beta, P));
    return 0;
 }
 
@@ -225,7 +229,7 @@ int DiffusionSolver::writePETSc(int n) const { return write_petsc_vec("scalar_fl
 int SNSolver::writePETSc(int n) const { return write_petsc_vec("angular_flux", n, psi); }
 
 int NeutronicSolver::finalize() {
-   pk_vec* vs[] = {&phi, &phi0, &T, &S, &q, &P};
+   pk_vec* vs[] = {&phi, &phi0, &T, &S, &q, &P, &beta};
    for (auto* v : vs) PK_CHECK(pk_vec_destroy(v));
    PK_CHECK(pk_xs_destroy(&dxs));
    PK_CHECK(pk_mesh_destroy(&dmesh));
@@ -585,14 +589,10 @@ int SNSolver::solve(int n, double dt, double t) {
    }
    // phi = 4 pi sum w psi; phi scaled to the power; psi scaled with sum w psi kapf V
    if (!sweepable) PAMPA_CHECK(scalarFluxFromAngular(), "unable to calculate the scalar flux");
-   long C = mesh->num_cells;
-   std::vector<double> hphi(C * G);
-   PK_CHECK(pk_vec_get_values(phi, hphi.data()));
+   // the power of the flux before its normalisation (sum phi kapf V, on the
+   // device: power < 0 leaves phi as it is)
    double p0 = 0.0;
-   for (long i = 0; i < C; i++) {
-      const Material& m = materials[mesh->mat[i]];
-      for (int g = 0; g < G; g++) p0 += hphi[i * G + g] * m.kappa_sigma_fission[g] * mesh->vol[i];
-   }
+   PK_CHECK(pk_normalize_flux(dmesh, dxs, phi, -1.0, keff, q, P, &p0));
    PAMPA_CHECK(!(p0 != 0.0), "zero power in the scalar-flux solution");
    PAMPA_CHECK(powerAndProduction(true), "unable to normalize the scalar flux");
    // angular normalisation: p0 of psi is p0 of phi / (4 pi)

@@ -130,6 +130,7 @@ class NeutronicSolver : public Solver {
    pk_mesh dmesh = nullptr;
    pk_xs dxs = nullptr;
    pk_vec phi = nullptr, phi0 = nullptr, T = nullptr, S = nullptr, q = nullptr, P = nullptr;
+   pk_vec beta = nullptr;  // delayed neutron fraction of each cell's material (built on first use)
    std::vector<int> bc_type;
    std::vector<double> bc_value;
    int stat_outer = 0;

@@ -201,6 +201,11 @@ __global__ void scale_kernel(double* x, long n, double a) {
    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (i < n) x[i] *= a;
 }
+__global__ void pointwise_mult_kernel(double* w, long n, const double* x, const double* y) {
+   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+   if (i < n) w[i] = x[i] * y[i];
+}
+
 __global__ void axpy_kernel(double* y, long n, double a, const double* x) {
    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
    if (i < n) y[i] += a * x[i];
@@ -386,6 +391,14 @@ int pk_vec_clip_noise(pk_vec v, double rel, long* negatives) {
    PK_HIP(hipMemcpyAsync(&h, w.p + 1, sizeof(h), hipMemcpyDeviceToHost, c.stream));
    PK_HIP(hipStreamSynchronize(c.stream));
    if (negatives) *negatives = (long)h;
+   return 0;
+}
+
+int pk_vec_pointwise_mult(pk_vec w, pk_vec x, pk_vec y) {
+   PK_REQUIRE(w && x && y && x->n == w->n && y->n == w->n, "vector size mismatch");
+   if (w->n == 0) return 0;
+   pointwise_mult_kernel<<<grid_for(w->n), kBlock, 0, ctx().stream>>>(w->d.p, w->n, x->d.p, y->d.p);
+   PK_HIP(hipGetLastError());
    return 0;
 }
 
