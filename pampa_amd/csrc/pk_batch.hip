@@ -959,6 +959,53 @@ __global__ void bphase_kernel(int nslots, BState S) {
    S.phase[b] = kOuter;
 }
 
+// A slot's tables and initial state, written at its admission, then its
+// units mapped to it.
+struct SlotInit {
+   int first, count, nrows, fbs, pbs, mlen;
+   long row_off, binv_off;
+   double delta;
+};
+
+__global__ void __launch_bounds__(kRows) slot_init_kernel(int b, SlotInit v, int* wg_first, int* wg_count,
+                                                          int* nrows, int* fbs, int* pbs, int* mlen, long* row_off,
+                                                          long* binv_off, BState S, int* unit_slot, long u0,
+                                                          long units) {
+   if (threadIdx.x == 0) {
+      wg_first[b] = v.first;
+      wg_count[b] = v.count;
+      nrows[b] = v.nrows;
+      fbs[b] = v.fbs;
+      pbs[b] = v.pbs;
+      mlen[b] = v.mlen;
+      row_off[b] = v.row_off;
+      binv_off[b] = v.binv_off;
+      for (int q = 0; q < 4; q++) S.flags[4 * b + q] = 0;
+      S.start[b] = 0;
+      S.shifted[b] = 0;
+      S.it[b] = 0;
+      S.inner[b] = 0;
+      S.status[b] = 0;
+      S.ks[b] = 0.0;
+      S.kprev[b] = 0.0;
+      S.res[b] = 0.0;
+      S.dl[b] = v.delta;
+      S.slow[b] = 0;
+      S.jcur[b] = 0;
+      S.klast[2 * b] = 0;
+      S.klast[2 * b + 1] = 0;
+      S.rebuild[b] = 1;
+      S.phase[b] = kOuter;
+   }
+   for (long u = threadIdx.x; u < units; u += kRows) unit_slot[u0 + u] = b;
+}
+
+// A released slot: FREE, its units unmapped.
+__global__ void __launch_bounds__(kRows) slot_release_kernel(int b, BState S, int* unit_slot, long u0, long units) {
+   if (threadIdx.x == 0) S.phase[b] = kFree;
+   for (long u = threadIdx.x; u < units; u += kRows) unit_slot[u0 + u] = -1;
+}
+
 // Every slot's phase, cycle position (next step, steps of the inner solve's
 // earlier cycles) and the longer of its last two inner solves, then the error
 // flag, into pinned host memory (one workgroup; kStatus values per slot), for a
@@ -1433,39 +1480,24 @@ static int pool_admit(pk_batch_s* P, pk_mat_s* R, pk_mat_s* F, const double* x0_
    const int i_mlen = bs >= 16 ? std::min(P->m, 30) : P->m;
    P->slots[sidx].mlen = i_mlen;
    const long l_row0 = row0, l_b0 = b0 * 64;
-   PK_HIP(hipMemcpyAsync(P->wg_first.p + sidx, &i_first, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->wg_count.p + sidx, &i_count, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->nrows.p + sidx, &i_n, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->fbs.p + sidx, &i_bs, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->pbs.p + sidx, &i_pbs, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->mlen.p + sidx, &i_mlen, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->row_off.p + sidx, &l_row0, sizeof(long), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->binv_off.p + sidx, &l_b0, sizeof(long), hipMemcpyHostToDevice, c.stream));
-   // state: OUTER, rebuild (A = R, block inverses) on the first tick
-   const int zi[4] = {0, 0, 0, 0};
-   const int one = 1, outer = kOuter;
-   const double zd = 0.0;
-   PK_HIP(hipMemcpyAsync(P->S.flags + 4 * sidx, zi, sizeof(int) * 4, hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.start + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.shifted + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.it + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.inner + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.status + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.ks + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.kprev + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.res + sidx, &zd, sizeof(double), hipMemcpyHostToDevice, c.stream));
-   const double dl0 = P->opts.delta;
-   PK_HIP(hipMemcpyAsync(P->S.dl + sidx, &dl0, sizeof(double), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.slow + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.jcur + sidx, zi, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.klast + 2 * (long)sidx, zi, 2 * sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.rebuild + sidx, &one, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.phase + sidx, &outer, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   // the slot's units become live last
+   // the slot's tables and state (OUTER, rebuild A = R and the block inverses
+   // on the first tick), then its units, in one launch
+   SlotInit v;
+   v.first = i_first;
+   v.count = i_count;
+   v.nrows = i_n;
+   v.fbs = i_bs;
+   v.pbs = i_pbs;
+   v.mlen = i_mlen;
+   v.row_off = l_row0;
+   v.binv_off = l_b0;
+   v.delta = P->opts.delta;
+   slot_init_kernel<<<1, kRows, 0, c.stream>>>(sidx, v, P->wg_first.p, P->wg_count.p, P->nrows.p, P->fbs.p, P->pbs.p,
+                                              P->mlen.p, P->row_off.p, P->binv_off.p, P->S, P->unit_slot.p, u0, units);
+   PK_HIP(hipGetLastError());
    for (long u = u0; u < u0 + units; u++) P->h_unit_slot[u] = sidx;
-   PK_HIP(hipMemcpyAsync(P->unit_slot.p + u0, P->h_unit_slot.data() + u0, sizeof(int) * units,
-                         hipMemcpyHostToDevice, c.stream));
-   // the host values above live on this stack frame: finish the copies here
+   // the admission reads the caller's operators and initial vector: done
+   // before returning
    PK_HIP(hipStreamSynchronize(c.stream));
    P->h_phase[sidx] = kOuter;
    *slot = sidx;
@@ -1523,11 +1555,8 @@ static int pool_release(pk_batch_s* P, int sidx) {
    PK_CALL(fold_activity(P));
    Slot& S = P->slots[sidx];
    for (long u = S.unit0; u < S.unit0 + S.units; u++) P->h_unit_slot[u] = -1;
-   const int fr = kFree;
-   PK_HIP(hipMemcpyAsync(P->unit_slot.p + S.unit0, P->h_unit_slot.data() + S.unit0, sizeof(int) * S.units,
-                         hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipMemcpyAsync(P->S.phase + sidx, &fr, sizeof(int), hipMemcpyHostToDevice, c.stream));
-   PK_HIP(hipStreamSynchronize(c.stream));
+   slot_release_kernel<<<1, kRows, 0, c.stream>>>(sidx, P->S, P->unit_slot.p, S.unit0, S.units);
+   PK_HIP(hipGetLastError());
    P->ua.release(S.unit0, S.units);
    P->ra.release(S.re0, S.re);
    P->fa.release(S.fe0, S.fe);
