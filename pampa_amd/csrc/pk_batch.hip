@@ -364,10 +364,20 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
    if (o.delta > 0.0) {
       // Adaptive Wielandt shift: the outer residual falls by about
       // (1/k - 1/k_s) / (1/k_2 - 1/k_s) per update, which tends to 1 when the
-      // dominance ratio k_2/k does; three shifted updates in a row that reduce
-      // the residual by less than 0.8 halve the shift distance (down to 0.004),
-      // at the price of harder inner solves
+      // dominance ratio k_2/k does; a closer shift takes fewer updates and
+      // harder inner solves.  Once shifted, a problem whose last inner solve
+      // took at most 24 GMRES steps (cheap) halves its shift distance once,
+      // down to 0.05; three shifted updates in a row that reduce the residual
+      // by less than 0.8 halve it too (down to 0.004)
       double dl = dl0;
+      const int klast = S.klast[2 * b];
+      if (shifted && dl > 0.05 && klast > 0 && klast <= 24) {
+         dl = fmax(0.5 * dl, 0.05);
+         S.dl[b] = dl;
+         ks = k + dl;
+         S.ks[b] = ks;
+         S.rebuild[b] = 1;
+      }
       if (shifted && rprev > 0.0) {
          int slow = res > 0.8 * rprev ? slow0 + 1 : 0;
          if (slow >= 3 && dl > 0.004) {
