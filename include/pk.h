@@ -312,8 +312,8 @@ int pk_sn_prepare(pk_sn S);
  * (host copy on the handle's stream; production-normalised as iterated). */
 int pk_sn_get_moment(pk_sn S, double* phiw);
 /* Anderson acceleration of the source iteration (Walker and Ni 2011) over the
- * last `depth` iterates, 0 <= depth <= 6 (0: plain source iteration; default
- * 5).  The fixed point, so the eigenpair at convergence, is the same. */
+ * last `depth` iterates, 0 <= depth <= 8 (0: plain source iteration; default
+ * 8).  The fixed point, so the eigenpair at convergence, is the same. */
 int pk_sn_set_acceleration(pk_sn S, int depth);
 /* Forget the previous solution: the next solve starts cold. */
 int pk_sn_reset(pk_sn S);

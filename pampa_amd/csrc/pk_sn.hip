@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(kBlock) sn_update_kernel(long CG, double* phiw
 // next iterate is g_n - sum_j gamma_j dg_j with gamma minimising
 // |f_n - sum_j gamma_j df_j| (Walker and Ni, SIAM J. Numer. Anal. 49 (2011)).
 // The fixed point, so the eigenpair, is that of the plain iteration.
-constexpr int kSnAAMax = 6;
+constexpr int kSnAAMax = 8;
 
 // f = g - phi; when have_prev: df = f - fprev, dg = g - gprev; then fprev = f, gprev = g
 __global__ void __launch_bounds__(kBlock) sn_aa_diff_kernel(long n, const double* __restrict__ g,
@@ -720,7 +720,7 @@ struct pk_sn_s {
    // read per incoming interior face, per (i,g) one source read and one moment write
    double value_units = 0.0;
    // Anderson acceleration: depth (0 = plain source iteration) and history
-   int aa_depth = 5;
+   int aa_depth = 8;
    int contraction = 0;  // source contraction: 0 by group count, 1 vector, 2 matrix cores
    // mixed precision: FP32 sweeps until the iterate's relative change falls
    // below switch64 (the FP32 round-off floor is near 1e-7), then FP64 sweeps
