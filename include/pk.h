@@ -187,7 +187,11 @@ int pk_eig_set_operators(pk_eig E, pk_mat R, pk_mat F);
 int pk_eig_set_tolerances(pk_eig E, double tol, int max_outer);
 int pk_eig_set_precision(pk_eig E, int precision);
 int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner);
-int pk_eig_set_shift(pk_eig E, double delta); /* Wielandt shift k_s = k + delta (0: none) */
+/* first Wielandt shift distance (default 0.1; 0: no shift): k_s = k + delta
+ * once k settles, then per problem halved once down to 0.05 when an inner
+ * solve took <= 24 GMRES steps, and down to 0.004 when the outer residual
+ * stalls (three updates in a row reducing it by less than 0.8) */
+int pk_eig_set_shift(pk_eig E, double delta);
 /* inner preconditioner, built on S = cell-block Jacobi B^-1 followed by one
  * block-Jacobi sweep over each 256-row workgroup's diagonal block of A:
  * 0 = S; 1 (default) = S (2 I - A S), one Richardson correction (one more SpMV
