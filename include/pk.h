@@ -192,6 +192,11 @@ int pk_eig_set_inner(pk_eig E, int restart, double reduction, int max_inner);
  * solve took <= 24 GMRES steps, and down to 0.004 when the outer residual
  * stalls (three updates in a row reducing it by less than 0.8) */
 int pk_eig_set_shift(pk_eig E, double delta);
+/* Anderson mixing of the outer iterates (Walker and Ni 2011): the update
+ * phi -> g = phi + e (e the inner solve's correction) is mixed over the last
+ * `depth` differences once the shift is applied, 0 <= depth <= 3 (default 3;
+ * 0: the plain update x = phi + e); the eigenpair is that of the plain update */
+int pk_eig_set_acceleration(pk_eig E, int depth);
 /* inner preconditioner, built on S = cell-block Jacobi B^-1 followed by one
  * block-Jacobi sweep over each 256-row workgroup's diagonal block of A:
  * 0 = S; 1 (default) = S (2 I - A S), one Richardson correction (one more SpMV
@@ -217,6 +222,7 @@ int pk_batch_set_tolerances(pk_batch B, double tol, int max_outer);
 int pk_batch_set_precision(pk_batch B, int precision);
 int pk_batch_set_inner(pk_batch B, int restart, double reduction, int max_inner);
 int pk_batch_set_shift(pk_batch B, double delta);
+int pk_batch_set_acceleration(pk_batch B, int depth); /* as pk_eig_set_acceleration; pools too */
 int pk_batch_set_preconditioner(pk_batch B, int poly); /* as pk_eig_set_preconditioner */
 int pk_batch_set_initial(pk_batch B, int index, pk_vec x); /* all-zero x: no initial space */
 int pk_batch_solve(pk_batch B);

@@ -64,14 +64,14 @@ def _declare(L):
         "pk_normalize_flux": [_P, _P, _P, _D, _D, _P, _P, _P],
         "pk_eig_create": [_P, _P, _PP], "pk_eig_set_tolerances": [_P, _D, _I],
         "pk_eig_set_precision": [_P, _I], "pk_eig_set_inner": [_P, _I, _D, _I],
-        "pk_eig_set_shift": [_P, _D], "pk_eig_set_initial": [_P, _P], "pk_eig_solve": [_P],
+        "pk_eig_set_shift": [_P, _D], "pk_eig_set_acceleration": [_P, _I], "pk_eig_set_initial": [_P, _P], "pk_eig_solve": [_P],
         "pk_eig_get_eigenpair": [_P, _P, _P], "pk_eig_get_stats": [_P, _P, _P, _P, _P],
         "pk_eig_destroy": [_PP],
         "pk_lin_create": [_P, _PP], "pk_lin_set_tolerances": [_P, _D, _D, _I],
         "pk_lin_solve": [_P, _P, _P], "pk_lin_get_stats": [_P, _P, _P], "pk_lin_destroy": [_PP],
         "pk_batch_create": [_I, _P, _P, _PP], "pk_batch_set_tolerances": [_P, _D, _I],
         "pk_batch_set_precision": [_P, _I], "pk_batch_set_inner": [_P, _I, _D, _I],
-        "pk_batch_set_shift": [_P, _D], "pk_batch_set_initial": [_P, _I, _P], "pk_batch_solve": [_P],
+        "pk_batch_set_shift": [_P, _D], "pk_batch_set_acceleration": [_P, _I], "pk_batch_set_initial": [_P, _I, _P], "pk_batch_solve": [_P],
         "pk_batch_get_eigenpair": [_P, _I, _P, _P], "pk_batch_get_stats": [_P, _I, _P, _P, _P, _P],
         "pk_batch_destroy": [_PP],
         "pk_tune_count": [_P], "pk_tune_get": [_I, _P, _P, _P, _P, _P, _P, _P], "pk_tune_reset": [],

@@ -67,6 +67,8 @@ int Driver::setOption(const std::string& name, const std::string& value) {
       options.precision = value == "mixed" ? PK_PREC_MIXED : PK_PREC_FP64;
    } else if (n == "pk_shift") {
       PAMPA_CHECK(input::read(options.wielandt_shift, 0.0, DBL_MAX, value), "wrong pk_shift");
+   } else if (n == "pk_outer_acceleration") {
+      PAMPA_CHECK(input::read(options.outer_acceleration, 0, 3, value), "wrong pk_outer_acceleration");
    } else if (n == "pk_inner_reduction") {
       PAMPA_CHECK(input::read(options.inner_reduction, 1e-12, 0.99, value), "wrong pk_inner_reduction");
    } else if (n == "pk_sn_tol") {

@@ -304,6 +304,7 @@ int DiffusionSolver::solve(int n, double dt, double t) {
       PK_CHECK(pk_eig_set_precision(eps, options.precision));
       PK_CHECK(pk_eig_set_inner(eps, options.gmres_restart, options.inner_reduction, 2000));
       PK_CHECK(pk_eig_set_shift(eps, options.wielandt_shift));
+      PK_CHECK(pk_eig_set_acceleration(eps, options.outer_acceleration));
       PK_CHECK(pk_eig_set_preconditioner(eps, options.preconditioner < 0 ? 1 : options.preconditioner));
       PK_CHECK(pk_eig_set_initial(eps, phi));
       int rc = pk_eig_solve(eps);
@@ -569,6 +570,7 @@ int SNSolver::solve(int n, double dt, double t) {
       PK_CHECK(pk_eig_set_precision(eps, options.precision));
       PK_CHECK(pk_eig_set_inner(eps, options.gmres_restart, options.inner_reduction, 20000));
       PK_CHECK(pk_eig_set_shift(eps, options.wielandt_shift));
+      PK_CHECK(pk_eig_set_acceleration(eps, options.outer_acceleration));
       // with downwind face interpolation (delta < 1) the workgroup-block sweep
       // of the preconditioner amplifies instead of smoothing (|I - A S| ~ 40 on
       // such cores, where B^-1 alone converges): cell-block Jacobi there

@@ -18,6 +18,7 @@ struct SolverOptions {
    int gmres_restart = 30;  // PETSc's default GMRES restart
    double inner_reduction = 0.1;
    double wielandt_shift = 0.1;
+   int outer_acceleration = 3;  // Anderson depth of the outer iterates (0: plain updates)
    // 0 cell-block Jacobi + workgroup-block sweep (bj), 1 with one Richardson
    // correction (bj2), 2 cell-block Jacobi alone (cell); -1: the solver's choice
    int preconditioner = -1;

@@ -42,6 +42,8 @@ namespace pk {
 
 constexpr int kRows = 256;   // rows per unit (= workgroup)
 constexpr int kBMaxM = 64;   // largest GMRES restart
+constexpr int kAA = 3;       // deepest Anderson history of the outer iterates
+constexpr int kAANv = kAA * (kAA + 1) / 2 + kAA;  // its reduced values: Gram entries, right-hand sides
 constexpr int kBMaxBs = 16;  // largest preconditioner block
 
 enum Phase { kFree = 0, kOuter = 1, kInner = 2, kDone = 3 };
@@ -126,6 +128,12 @@ struct BState {  // per-slot state (arrays over slots)
    int* status;    // [S] 0 running, 1 converged, 2 failed
    int* slow;      // [S] consecutive shifted outer updates that reduced the residual by less than 0.8
    int* klast;     // [S][2] GMRES steps of the last two inner solves (the host sizes ticks from them)
+   // Anderson mixing of the outer iterates
+   double* aa_red;   // [S][kAANv] the normal equations' entries, reduced
+   double* aa_gam;   // [S][kAA] mixing coefficients of the last update
+   double* aa_best;  // [S] smallest outer residual since the history started
+   int* aa_cnt;      // [S] iterates recorded in the history (0: none)
+   int* aa_nc;       // [S] coefficients in aa_gam (-1: singular system, the history restarts)
 };
 
 struct BOpts {
@@ -133,6 +141,7 @@ struct BOpts {
    int max_outer, max_inner;
    int poly;  // 1: preconditioner S (2 I - A S) (one Richardson correction), 0: S (precond_apply)
    int sweep = 1;  // 1: S = B^-1 followed by the workgroup-block sweep; 0: S = B^-1
+   int aa = kAA;   // Anderson depth of the outer iterates (0: plain update x = phi + e)
 };
 
 __device__ __forceinline__ int slot_of_wg(const BLayout& L) { return L.unit_slot[blockIdx.x]; }
@@ -359,6 +368,16 @@ __global__ void __launch_bounds__(kRows) bresid_kernel(BLayout L, const double* 
       S.status[b] = 2;
       S.phase[b] = kDone;
       return;
+   }
+   if (o.aa > 0) {
+      // the Anderson history restarts when the residual grows 100x past its best
+      const int cnt = S.aa_cnt[b];
+      const double best = S.aa_best[b];
+      if (cnt == 0 || res < best) S.aa_best[b] = res;
+      if (cnt > 0 && res > 100.0 * best) {
+         S.aa_cnt[b] = 0;
+         S.aa_best[b] = res;
+      }
    }
    double ks = ks0;
    if (o.delta > 0.0) {
@@ -948,6 +967,129 @@ __global__ void bfinish_kernel(long N, BLayout L, const double* phi, const T* e,
    x[row] = phi[row] + (double)e[row];
 }
 
+// ---- Anderson mixing of the outer iterates ----------------------------------------------
+// The outer update is the fixed-point map phi -> g = phi + e (e the inner
+// solve's correction), whose residual is f = g - phi = e.  Anderson mixing
+// (Walker and Ni 2011) of depth d takes x = g - sum_i gamma_i dG_i, gamma the
+// least-squares solution of min |f - sum_i gamma_i dF_i| over the last d
+// differences dF_i, dG_i of successive f and g.  A slot mixes only while
+// shifted, and its history restarts when the outer residual grows 100x past
+// its best (bresid) or the small system is singular.  The fixed point, so
+// the eigenpair, is that of the plain update.
+
+struct BAnderson {
+   double* df;  // [kAA][N] differences of f, a ring filled in order
+   double* dg;  // [kAA][N] differences of g
+   double* f;   // [N] f of the last recorded iterate
+   double* g;   // [N] g of the last recorded iterate
+   long N;
+};
+
+// index of Gram entry (i, j), i <= j < kAA, among the reduced values
+__host__ __device__ constexpr int aa_pair(int i, int j) { return i * kAA - i * (i - 1) / 2 + (j - i); }
+
+// For slots whose inner solve ended this tick (shifted ones): record f and g,
+// write the newest differences into the ring, reduce the normal equations
+// dF^T dF gamma = dF^T f; the slot's last workgroup solves them (Cholesky, in
+// a fixed order).  A pivot below 1e-12 of the largest diagonal entry marks
+// the system singular: no mixing this time, and the history restarts.
+template <class T>
+__global__ void __launch_bounds__(kRows) baa_dots_kernel(BLayout L, const double* phi, const T* e, BAnderson H,
+                                                         int depth, double* partials, unsigned* tickets,
+                                                         unsigned* gtickets, BState S) {
+   const int b = slot_of_wg(L);
+   if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b] || !S.shifted[b]) return;
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   const int cnt = S.aa_cnt[b];
+   const double f = (double)e[row], g = phi[row] + f;
+   const bool mix = cnt >= 1;
+   const int nd = mix ? min(cnt, depth) : 0;
+   const int q = mix ? (cnt - 1) % depth : 0;
+   double d[kAA];
+#pragma unroll
+   for (int i = 0; i < kAA; i++) d[i] = 0.0;
+   if (mix) {
+      const double dfn = f - H.f[row], dgn = g - H.g[row];
+#pragma unroll
+      for (int i = 0; i < kAA; i++)
+         if (i < nd) d[i] = i == q ? dfn : H.df[(long)i * H.N + row];
+      H.df[(long)q * H.N + row] = dfn;
+      H.dg[(long)q * H.N + row] = dgn;
+   }
+   H.f[row] = f;
+   H.g[row] = g;
+   if (!mix) return;
+   const bool last = seg_reduce<kAANv>(
+       kAANv,
+       [&](int k) {
+          if (k >= kAANv - kAA) return d[k - (kAANv - kAA)] * f;
+          int i = 0;
+          while (k >= kAA - i) {
+             k -= kAA - i;
+             i++;
+          }
+          return d[i] * d[i + k];
+       },
+       partials + (long)L.wg_first[b] * kAANv, tickets + b, gtickets + L.wg_first[b], S.aa_red + (long)b * kAANv,
+       (int)blockIdx.x - L.wg_first[b], L.wg_count[b]);
+   if (!last || threadIdx.x != 0) return;
+   const double* red = S.aa_red + (long)b * kAANv;
+   double a[kAA][kAA], r[kAA], gam[kAA];
+   double dmax = 0.0;
+   for (int i = 0; i < nd; i++) {
+      for (int j = 0; j < nd; j++) a[i][j] = red[aa_pair(min(i, j), max(i, j))];
+      r[i] = red[kAANv - kAA + i];
+      dmax = fmax(dmax, a[i][i]);
+   }
+   // Cholesky a = L L^T in place (lower triangle), then the two triangular solves
+   bool ok = dmax > 0.0 && dmax < 1e300;
+   for (int j = 0; j < nd && ok; j++) {
+      double djj = a[j][j];
+      for (int p = 0; p < j; p++) djj -= a[j][p] * a[j][p];
+      if (!(djj > 1e-12 * dmax)) {
+         ok = false;
+         break;
+      }
+      a[j][j] = sqrt(djj);
+      for (int i = j + 1; i < nd; i++) {
+         double v = a[i][j];
+         for (int p = 0; p < j; p++) v -= a[i][p] * a[j][p];
+         a[i][j] = v / a[j][j];
+      }
+   }
+   if (ok) {
+      for (int i = 0; i < nd; i++) {
+         double v = r[i];
+         for (int p = 0; p < i; p++) v -= a[i][p] * gam[p];
+         gam[i] = v / a[i][i];
+      }
+      for (int i = nd - 1; i >= 0; i--) {
+         double v = gam[i];
+         for (int p = i + 1; p < nd; p++) v -= a[p][i] * gam[p];
+         gam[i] = v / a[i][i];
+      }
+      for (int i = 0; i < nd; i++) ok = ok && gam[i] == gam[i] && fabs(gam[i]) < 1e300;
+   }
+   for (int i = 0; i < kAA; i++) S.aa_gam[(long)b * kAA + i] = ok && i < nd ? gam[i] : 0.0;
+   S.aa_nc[b] = ok ? nd : -1;
+}
+
+// x = g - sum_i gamma_i dG_i (x = g where the slot does not mix), for slots
+// whose inner solve ended this tick
+template <class T>
+__global__ void __launch_bounds__(kRows) baa_mix_kernel(BLayout L, const double* phi, const T* e, BAnderson H,
+                                                        BState S, double* x) {
+   const int b = slot_of_wg(L);
+   if (b < 0 || S.phase[b] != kInner || !S.flags[4 * b]) return;
+   const long row = blockIdx.x * (long)kRows + threadIdx.x;
+   double v = phi[row] + (double)e[row];
+   if (S.shifted[b] && S.aa_cnt[b] >= 1) {
+      const int nc = S.aa_nc[b];
+      for (int i = 0; i < nc; i++) v -= S.aa_gam[(long)b * kAA + i] * H.dg[(long)i * H.N + row];
+   }
+   x[row] = v;
+}
+
 // y = A x over the units of live slots (timing of the pool's operator)
 template <class T>
 __global__ void __launch_bounds__(kRows) bspmv_kernel(BLayout L, BSell A, const T* aval, const T* x, T* y) {
@@ -966,6 +1108,10 @@ __global__ void bphase_kernel(int nslots, BState S) {
    S.inner[b] += S.flags[4 * b + 3];
    S.klast[2 * b + 1] = S.klast[2 * b];
    S.klast[2 * b] = S.flags[4 * b + 3];
+   // the Anderson history: one more iterate recorded while shifted (restarted
+   // at this one after a singular system), none before the shift
+   const int c = S.aa_cnt[b];
+   S.aa_cnt[b] = S.shifted[b] ? (c >= 1 && S.aa_nc[b] < 0 ? 1 : c + 1) : 0;
    S.phase[b] = kOuter;
 }
 
@@ -1004,6 +1150,9 @@ __global__ void __launch_bounds__(kRows) slot_init_kernel(int b, SlotInit v, int
       S.jcur[b] = 0;
       S.klast[2 * b] = 0;
       S.klast[2 * b + 1] = 0;
+      S.aa_cnt[b] = 0;
+      S.aa_nc[b] = 0;
+      S.aa_best[b] = 0.0;
       S.rebuild[b] = 1;
       S.phase[b] = kOuter;
    }
@@ -1131,6 +1280,7 @@ struct pk_batch_s {
    DBuf<int> r_col, f_col, rlen;
    DBuf<double> r_val, f_val, a_val;
    DBuf<double> x, phi, y1, y2, r;
+   DBuf<double> aa_df, aa_dg, aa_f, aa_g;  // Anderson history of the outer iterates
    DBuf<double> sd;
    DBuf<int> si;
    DBuf<double> part;
@@ -1201,10 +1351,11 @@ static double pool_bytes(long units, long rent, long fent, long binv, int slots,
    b += units * 4.0 + ng * 2 * (8.0 + 4.0 + 4.0) + rows * 4.0;  // unit map, group tables, rlen
    b += rent * 64.0 * (4.0 + 8.0 + 8.0) + fent * 64.0 * 12.0;    // R cols+vals, A (FP64), F
    b += rows * 8.0 * 5;                                          // x, phi, y1, y2, r
+   b += rows * 8.0 * (2 * kAA + 2);                              // Anderson history
    b += rows * t * (m + 1 + 5) + binv * 64.0 * t;                // basis, z z0 w e rhs, block inverses
    if (precision == PK_PREC_MIXED) b += rent * 64.0 * t;         // A copy in FP32
-   b += (double)units * (m + 4) * 8.0;                           // reduction partials
-   b += (double)slots * ((m + 1.0) * m + 6.0 * m + 28.0) * 8.0;  // per-slot state
+   b += (double)units * std::max(m + 4, kAANv) * 8.0;            // reduction partials
+   b += (double)slots * ((m + 1.0) * m + 6.0 * m + 40.0) * 8.0;  // per-slot state
    return b;
 }
 
@@ -1220,7 +1371,9 @@ static int pool_alloc(pk_batch_s* P) {
               P->rlen.alloc(rows) || P->r_col.alloc(rent) || P->r_val.alloc(rent) || P->a_val.alloc(rent) ||
               P->f_col.alloc(fent) || P->f_val.alloc(fent) || P->x.alloc(rows) || P->phi.alloc(rows) ||
               P->y1.alloc(rows) || P->y2.alloc(rows) || P->r.alloc(rows) ||
-              P->part.alloc((size_t)units * (m + 4)) || P->tick.alloc(S) || P->gtick.alloc(units) ||
+              P->aa_df.alloc((size_t)kAA * rows) || P->aa_dg.alloc((size_t)kAA * rows) || P->aa_f.alloc(rows) ||
+              P->aa_g.alloc(rows) ||
+              P->part.alloc((size_t)units * std::max(m + 4, kAANv)) || P->tick.alloc(S) || P->gtick.alloc(units) ||
               P->err.alloc(1);
    PK_REQUIRE(!bad, "device allocation failed for the eigen pool");
    auto alloc_inner = [&](auto& I, bool own_a) {
@@ -1233,8 +1386,8 @@ static int pool_alloc(pk_batch_s* P) {
    else
       PK_REQUIRE(!alloc_inner(P->i64, false), "device allocation failed for the eigen pool");
    // per-slot state
-   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 6 * (size_t)m + 28);
-   const size_t ni = (size_t)S * 16;
+   const size_t nd = (size_t)S * ((size_t)(m + 1) * m + 6 * (size_t)m + 40);
+   const size_t ni = (size_t)S * 20;
    PK_REQUIRE(!P->sd.alloc(nd) && !P->si.alloc(ni), "device allocation failed for the eigen pool");
    PK_HIP(hipMemsetAsync(P->sd.p, 0, sizeof(double) * nd, c.stream));
    PK_HIP(hipMemsetAsync(P->si.p, 0, sizeof(int) * ni, c.stream));
@@ -1262,6 +1415,9 @@ static int pool_alloc(pk_batch_s* P) {
    B.ks = take(S);
    B.res = take(S);
    B.dl = take(S);
+   B.aa_red = take((size_t)S * kAANv);
+   B.aa_gam = take((size_t)S * kAA);
+   B.aa_best = take(S);
    int* q = P->si.p;
    auto takei = [&](size_t k) {
       int* p = q;
@@ -1279,6 +1435,8 @@ static int pool_alloc(pk_batch_s* P) {
    B.slow = takei(S);
    B.jcur = takei(S);
    B.klast = takei(2 * (size_t)S);
+   B.aa_cnt = takei(S);
+   B.aa_nc = takei(S);
    // tables: every unit free, every slice empty
    P->h_unit_slot.assign(units, -1);
    PK_HIP(hipMemcpyAsync(P->unit_slot.p, P->h_unit_slot.data(), sizeof(int) * units, hipMemcpyHostToDevice,
@@ -1673,7 +1831,13 @@ static int launch_tick(pk_batch_s* P, long units, int steps, bool prof) {
    }
    brestart_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, Rs, aval, I.e.p, I.rhs.p, I.V.p, zfirst, I.binv.p, o, part,
                                                    tk, gtk, S);
-   bfinish_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, P->phi.p, I.e.p, S, P->x.p);
+   if (o.aa > 0) {
+      const BAnderson H{P->aa_df.p, P->aa_dg.p, P->aa_f.p, P->aa_g.p, N};
+      baa_dots_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->phi.p, I.e.p, H, o.aa, part, tk, gtk, S);
+      baa_mix_kernel<T><<<nwg, kRows, 0, c.stream>>>(L, P->phi.p, I.e.p, H, S, P->x.p);
+   } else {
+      bfinish_kernel<T><<<grid_for(rows), kBlock, 0, c.stream>>>(rows, L, P->phi.p, I.e.p, S, P->x.p);
+   }
    bphase_kernel<<<grid_for(ns), kBlock, 0, c.stream>>>(ns, S);
    PK_HIP(hipGetLastError());
    return 0;
@@ -2030,6 +2194,14 @@ int pk_batch_set_preconditioner(pk_batch P, int poly) {
    return 0;
 }
 
+int pk_batch_set_acceleration(pk_batch P, int depth) {
+   PK_REQUIRE(P && depth >= 0 && depth <= kAA, "wrong Anderson depth (0 to 3)");
+   if (depth == P->opts.aa) return 0;
+   P->opts.aa = depth;
+   P->graph.clear();
+   return 0;
+}
+
 int pk_batch_set_shift(pk_batch P, double delta) {
    PK_REQUIRE(P && delta >= 0.0, "wrong shift");
    if (delta == P->opts.delta) return 0;
@@ -2135,7 +2307,7 @@ int pk_pool_set_options(pk_batch P, double tol, int max_outer, double reduction,
    PK_REQUIRE(P && tol > 0.0 && max_outer > 0 && reduction > 0.0 && reduction < 1.0 && max_inner > 0 &&
                   delta >= 0.0,
               "wrong pool options");
-   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner, P->opts.poly, P->opts.sweep};
+   P->opts = BOpts{tol, delta, reduction, max_outer, max_inner, P->opts.poly, P->opts.sweep, P->opts.aa};
    P->graph.clear();
    return 0;
 }

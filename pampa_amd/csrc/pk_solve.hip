@@ -508,6 +508,11 @@ int pk_eig_set_shift(pk_eig E, double delta) {
    return pk_batch_set_shift(E->B, delta);
 }
 
+int pk_eig_set_acceleration(pk_eig E, int depth) {
+   PK_REQUIRE(E, "null eigen solver");
+   return pk_batch_set_acceleration(E->B, depth);
+}
+
 int pk_eig_set_initial(pk_eig E, pk_vec x) {
    PK_REQUIRE(E, "null eigen solver");
    return pk_batch_set_initial(E->B, 0, x);
