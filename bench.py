@@ -30,11 +30,11 @@ HBM_PEAK_GBS = 8000.0
 # on the quarter-core workload: two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
 # `bench.py --roofline-pass`, each scaled by what the same counter reads for
 # stream_copy_kernel moving 2^28 bytes each way (fetch x2.000, write x1.000):
-# 2686.0 KiB x 2.000 + 219.4 KiB = 5.726 MB.
-SPMV_DOT_TRAFFIC_BYTES = 5.726e6
+# 2541.1 KiB x 2.000 + 203.4 KiB = 5.412 MB.
+SPMV_DOT_TRAFFIC_BYTES = 5.412e6
 # the same counter passes over `bench.py --batched-pass` (256 instances in one mixed pool),
 # HBM bytes per launch of each GMRES step kernel
-BATCHED_TRAFFIC_BYTES = {"gmres_spmv_dot": 507.4e6, "gmres_orth": 711.8e6, "gmres_poly": 835.5e6}
+BATCHED_TRAFFIC_BYTES = {"gmres_spmv_dot": 543.1e6, "gmres_orth": 745.2e6, "gmres_poly": 874.1e6}
 METRIC = ("eigen solves/s + p50/p99 latency, mixed vs FP64 at |dk|<=1pcm; sweep/SpMV HBM GB/s")
 
 

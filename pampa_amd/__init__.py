@@ -311,7 +311,7 @@ def diffusion_assemble(mesh: DeviceMesh, xs: DeviceXS, bc_type, bc_value):
 
 class Eigen(Handle):
     def __init__(self, R: Mat, F: Mat, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=60,
-                 reduction=0.1, max_inner=2000, max_outer=20000):
+                 reduction=0.1, max_inner=2000, max_outer=20000, acceleration=3):
         L = lib()
         h = ct.c_void_p()
         check(L.pk_eig_create(R.h, F.h, ct.byref(h)))
@@ -321,6 +321,7 @@ class Eigen(Handle):
         check(L.pk_eig_set_precision(h, precision))
         check(L.pk_eig_set_inner(h, restart, reduction, max_inner))
         check(L.pk_eig_set_shift(h, shift))
+        check(L.pk_eig_set_acceleration(h, acceleration))
 
     def solve(self, x0: Vec | None = None):
         L = lib()
@@ -344,7 +345,7 @@ class Batch(Handle):
     """Batched eigen solver over pairs (R_b, F_b): one launch per step for all."""
 
     def __init__(self, Rs, Fs, tol=1e-8, precision=PK_PREC_FP64, shift=0.1, restart=60,
-                 reduction=0.1, max_inner=2000, max_outer=20000):
+                 reduction=0.1, max_inner=2000, max_outer=20000, acceleration=3):
         L = lib()
         n = len(Rs)
         assert n == len(Fs) and n > 0
@@ -359,6 +360,7 @@ class Batch(Handle):
         check(L.pk_batch_set_precision(h, precision))
         check(L.pk_batch_set_inner(h, restart, reduction, max_inner))
         check(L.pk_batch_set_shift(h, shift))
+        check(L.pk_batch_set_acceleration(h, acceleration))
 
     def set_initial(self, index: int, x: "Vec"):
         check(lib().pk_batch_set_initial(self.h, index, x.h))

@@ -94,6 +94,34 @@ def test_batch_parity_and_alone_vs_mix(pk, members, precision):
         assert np.array_equal(x.get(), mix[b][1]), members[b][0]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_outer_acceleration_keeps_the_eigenpair(pk, members, precision):
+    """Anderson mixing of the outer iterates (depth 1 to 3) against the plain
+    update (depth 0): every member still meets the oracle tolerances, the mixed
+    iteration needs fewer outer updates in all, and a depth outside 0..3 is
+    refused."""
+    prec = pk.PK_PREC_FP64 if precision == "fp64" else pk.PK_PREC_MIXED
+    Rs, Fs = _mats(pk, members)
+    outer = {}
+    for depth in (0, 1, 3):
+        print(f"[case] batch of {len(Rs)} {precision}, Anderson depth {depth}", flush=True)
+        batch = pk.Batch(Rs, Fs, precision=prec, acceleration=depth)
+        batch.solve()
+        outer[depth] = 0
+        for b, (name, G, ref) in enumerate(members):
+            k, x = batch.eigenpair(b)
+            st = batch.stats(b)
+            dk = abs(k - ref.keff) * 1e5
+            l2 = np.linalg.norm(_shape(x.get()) - _shape(ref.phi)) / np.linalg.norm(_shape(ref.phi))
+            print(f"   {name}: dk={dk:.2e} pcm l2={l2:.2e} outer={st['outer']} inner={st['inner']}", flush=True)
+            assert dk <= DK_PCM, (name, depth)
+            assert l2 <= FLUX_L2, (name, depth)
+            outer[depth] += st["outer"]
+    assert outer[3] < outer[0] and outer[1] < outer[0], outer
+    with pytest.raises(pk.PkError):
+        pk.Batch(Rs[:1], Fs[:1], acceleration=4)
+
+
 def test_batch_matches_single_eigen(pk, members):
     """The batch engine and pk_eig agree to the parity tolerances."""
     Rs, Fs = _mats(pk, members[:3])
