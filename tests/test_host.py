@@ -83,6 +83,18 @@ def test_cli_input_errors(tmp_path, text, msg):
     assert "error:" in r.stdout and "Error in pampa.initialize()." in r.stdout
 
 
+@pytest.mark.parametrize("opt,value", [("-pk_outer_acceleration", "4"), ("-pk_outer_acceleration", "9"),
+                                       ("-pk_inner_reduction", "1.5")])
+def test_cli_option_errors(tmp_path, opt, value):
+    """Out-of-range build options are refused while the command line is read,
+    before any device call."""
+    (tmp_path / "mesh.pmp").write_text("dx 2\n1 1\nbc -x vacuum\nbc +x vacuum\nmaterials 2\n1 1\n")
+    (tmp_path / "input.pmp").write_text("mesh cartesian mesh.pmp\n")
+    r = _cli(["input.pmp", opt, value], str(tmp_path))
+    assert r.returncode == 1
+    assert f"wrong {opt[1:]}" in r.stdout and "wrong command-line option" in r.stdout
+
+
 def test_cli_missing_input(tmp_path):
     r = _cli([], str(tmp_path))
     assert r.returncode == 1 and "missing input file" in r.stdout
